@@ -1,0 +1,187 @@
+/*
+ * mpulse_hip.h -- C ABI of the MI355X (gfx950) time-stepping core of MPulse.
+ *
+ * The library owns the field state on one GPU and advances it with hand-written HIP
+ * kernels.  Every entry point takes plain pointers and sizes; no C++ or torch types
+ * cross this boundary.  All functions return MPX_OK (0) or a negative MPX_ERR_* code;
+ * mpx_last_error() then returns a message for the calling thread.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the MPulse tree):
+ *   mpx_create        FieldSolver::initStorage              src/fieldsolver.h:12
+ *                     + CurrentFactory::initCurrents        src/currentfactory.h:14
+ *                     + Storage::resize/addGrid             src/storage.cpp:30-76
+ *                     + Boundary::init                      src/boundary.h:44
+ *   mpx_step          FDTD_PLRCSolver<Impl>::stepScheme     src/fdtd_plrc.t:127-168
+ *                     (and FDTD_Plain/Nonlinear/Dispersion::stepScheme,
+ *                      src/fdtd_plain.cpp:31-35, src/fdtd_nl.cpp:32-36, src/fdtd_disp.cpp:78-92)
+ *                     driven by the time loop of Process::run  src/process.cpp:17-31
+ *   mpx_step_init     FieldSolver::stepSchemeInit           src/fdtd_plrc.t:90-124
+ *   mpx_get_field     Storage::getGrid / getBorderLayer     src/storage.cpp:48-90
+ *   mpx_set_field     (same grids, written by initialisers, e.g. src/waveinit.cpp:9-60)
+ *   mpx_energy        FieldEnergyDiag::calculate            src/fielddiag.cpp:255-286
+ *   mpx_set_probes /  point histories (the reference has no point-probe diagnostic;
+ *   mpx_get_probes    FieldLineDiag src/fielddiag.cpp:122-223 is the nearest)
+ *   mpx_last_error    std::cerr + exit(-1) error reporting  (e.g. src/fdtd_plrc.t:20-24)
+ *
+ * Grid convention (src/globals.cpp:44-45): interior cells are 1..N in each dimension,
+ * storage spans 0..N+1.  Host-side arrays exchanged through this ABI are float64,
+ * shape (Nx+2, Ny+2, Nz+2), z fastest (C order).
+ */
+#ifndef MPULSE_HIP_H
+#define MPULSE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPX_ABI_VERSION 1
+#define MPX_MAX_FACTORIES 8
+#define MPX_MAX_POLES 3
+
+/* status codes */
+#define MPX_OK               0
+#define MPX_ERR_INVALID     -1   /* invalid descriptor or argument */
+#define MPX_ERR_HIP         -2   /* a HIP runtime call failed */
+#define MPX_ERR_NOMEM       -3   /* device allocation failed */
+#define MPX_ERR_UNSUPPORTED -4   /* combination not provided by this build */
+#define MPX_ERR_NAME        -5   /* unknown grid name */
+#define MPX_ERR_NUMERIC     -6   /* non-finite value or Newton failure detected on device */
+
+/* FieldSolver registry keys, src/fieldsim.cpp:177-195 */
+#define MPX_SOLVER_PLRC_NONLIN 0  /* "fdtd-plrc-nonlin" */
+#define MPX_SOLVER_PLRC_LIN    1  /* "fdtd-plrc-lin"    */
+#define MPX_SOLVER_PLAIN       2  /* "fdtd-plain"       */
+#define MPX_SOLVER_NL          3  /* "fdtd-nl"          */
+#define MPX_SOLVER_DISP        4  /* "fdtd-disp"        */
+
+/* Boundary registry keys, src/fieldsim.cpp:198-204 */
+#define MPX_BOUNDARY_PERIODIC    0  /* "single_periodic"    */
+#define MPX_BOUNDARY_XY_PERIODIC 1  /* "single_xy_periodic" */
+
+/* storage precision of the bulk fields */
+#define MPX_PREC_FP64      0  /* every word float64 */
+#define MPX_PREC_FP32_BULK 1  /* E, B and CPML psi float32; PLRC accumulators and Kerr solve float64 */
+
+/* kernel structure */
+#define MPX_PATH_FUSED     0  /* one fused E-side and one fused B-side kernel per step */
+#define MPX_PATH_REFERENCE 1  /* one kernel per reference loop: currents, totals, update, wrap */
+
+/* CurrentFactory registry keys, src/fdtd_plrc.t:431-469 */
+#define MPX_FACTORY_CPML          1  /* "cpml_border"          src/cpml_border.cpp:14-140   */
+#define MPX_FACTORY_CPML_1D       2  /* "cpml_border_1d"       src/cpml_border.cpp:158-255  */
+#define MPX_FACTORY_SHORT_PULSE   3  /* "short_pulse_inject"   src/shortpulseinject.cpp      */
+#define MPX_FACTORY_PLANE_WAVE    4  /* "wave_inject"          src/sources.cpp:8-173         */
+#define MPX_FACTORY_PLANE_GAUSS   5  /* "plane_gauss_inject"   src/sources.cpp:179-331       */
+
+typedef struct mpx_factory {
+    int    kind;          /* MPX_FACTORY_* */
+    int    d;             /* cpml: thickness "d" (default 8); sources: distance "d" (default 15) */
+    /* cpml_border keys, src/cpml_border.cpp:146-149 */
+    double kappa_max;     /* "kappaMax" (15)   */
+    double a_max;         /* "aMax"     (0.25) */
+    double sigma_max;     /* "sigmaMax" (3)    */
+    /* short_pulse_inject keys, src/shortpulseinject.cpp:38-47 */
+    double length;        /* "length" T            */
+    double width;         /* "width"  w0           */
+    double om0;           /* "om0"    (2 pi)       */
+    double tshift;        /* "TShift"              */
+    double zshift;        /* "ZShift"              */
+    double phase;         /* "Phase" (units of 2 pi) */
+    double amp;           /* "amp"                 */
+    double eps;           /* "eps" (also wave_inject / plane_gauss_inject "eps") */
+    /* wave_inject / plane_gauss_inject keys, src/sources.cpp:58-82, 225-242 */
+    double k[3];          /* "kx" "ky" "kz"        */
+    double h[3];          /* "Hx" "Hy" "Hz"        */
+    double h_bg[3];       /* "Hx_bg" ...           */
+    double e_bg[3];       /* "Ex_bg" ...           */
+    double ramp;          /* "ramp" (0.5)          */
+    double gwidth;        /* plane_gauss "width"  (10) */
+    double goffset;       /* plane_gauss "offset" (40) */
+} mpx_factory;
+
+typedef struct mpx_desc {
+    int    abi_version;   /* MPX_ABI_VERSION */
+    /* globals block, src/globals.cpp:21-36 */
+    int    nx, ny, nz;    /* "grid-x" ... */
+    double dx, dy, dz;    /* "grid-dx" ... */
+    double dt;            /* "dt" */
+    int    solver;        /* MPX_SOLVER_* */
+    int    boundary;      /* MPX_BOUNDARY_* */
+    int    precision;     /* MPX_PREC_* */
+    int    path;          /* MPX_PATH_* */
+    /* FDTD_PLRCSolver keys, src/fdtd_plrc.t:417-429, src/fdtd_plrc.cpp:463;
+       FDTD_Nonlinear src/fdtd_nl.cpp:150-151 */
+    double eps;           /* "eps" (1)   */
+    double chi;           /* "chi" (0.1) */
+    double l_eps[MPX_MAX_POLES];    /* "L1eps".. (0) */
+    double l_delta[MPX_MAX_POLES];  /* "L1delta".. (0) */
+    double l_om2[MPX_MAX_POLES];    /* "L1Om2".. (1) */
+    /* FDTD_Dispersion keys, src/fdtd_disp.cpp:256-265 */
+    double d_eps2[MPX_MAX_POLES];   /* "L1eps2".. */
+    double d_om2[MPX_MAX_POLES];    /* "L1Om2"..  */
+    int    n_factories;   /* current factories in setup-file order */
+    mpx_factory factories[MPX_MAX_FACTORIES];
+} mpx_desc;
+
+typedef struct mpx_solver mpx_solver;
+
+/* Kernel timing filled by mpx_time_steps: mean device time per launch of each kernel class. */
+#define MPX_MAX_KERNEL_CLASSES 16
+typedef struct mpx_timing {
+    int    n_classes;
+    char   name[MPX_MAX_KERNEL_CLASSES][32];
+    double mean_ms[MPX_MAX_KERNEL_CLASSES];     /* mean duration of one launch */
+    int    launches[MPX_MAX_KERNEL_CLASSES];    /* launches timed */
+    double step_ms;                              /* mean wall time of one whole step */
+} mpx_timing;
+
+/* Fill a descriptor with the reference defaults (MakeParamMap defaults). */
+void mpx_desc_defaults(mpx_desc *desc);
+void mpx_factory_defaults(mpx_factory *f, int kind);
+
+int  mpx_create(const mpx_desc *desc, mpx_solver **out);
+int  mpx_destroy(mpx_solver *s);
+
+/* Advance nsteps full steps.  The time index seen by the sources is the solver's own
+   counter, starting at 0, exactly as Process::time (src/process.cpp:22). */
+int  mpx_step(mpx_solver *s, int nsteps);
+/* FieldSolver::stepSchemeInit: never called by the reference driver; provided for parity. */
+int  mpx_step_init(mpx_solver *s);
+int  mpx_get_time(mpx_solver *s, int *time);
+int  mpx_set_time(mpx_solver *s, int time);
+
+/* Full-storage grids by reference name: "Ex".."Bz", "PsiR1x".."PsiI3z" (PLRC),
+   "P1x".."P3zp" (fdtd-disp), "JxTotal".."MzTotal" (reference path only).
+   Buffer: (Nx+2)*(Ny+2)*(Nz+2) float64, z fastest. */
+int  mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count);
+int  mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count);
+/* CPML border layers by reference name and Direction (0 north,1 south,2 west,3 east,4 up,5 down;
+   src/mpulse.h:29): "CPMLPsiEx".."CPMLPsiBz".  Buffer is the layer's inclusive box
+   lo..hi (returned through lo/hi), z fastest. */
+int  mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_t count,
+                   int lo[3], int hi[3]);
+
+/* Point histories: after every step the listed components are sampled.
+   comp: 0..2 = Ex,Ey,Ez, 3..5 = Bx,By,Bz.  Capacity = max steps recorded. */
+int  mpx_set_probes(mpx_solver *s, int nprobes, const int *ijk, const int *comp, int capacity);
+int  mpx_get_probes(mpx_solver *s, double *host, int *nrecorded);
+
+/* sum over interior cells of (E^2 + B^2) * dx*dy*dz  (src/fielddiag.cpp:255-286) */
+int  mpx_energy(mpx_solver *s, double *energy);
+
+int  mpx_sync(mpx_solver *s);
+/* Run nsteps with per-kernel HIP events on the solver's stream. */
+int  mpx_time_steps(mpx_solver *s, int nsteps, mpx_timing *out);
+/* Device-side byte size of the resident state. */
+int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
+
+const char *mpx_last_error(void);
+int  mpx_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPULSE_HIP_H */

@@ -1,0 +1,256 @@
+"""ctypes mirror of include/mpulse_hip.h and a thin Python handle over the C ABI.
+
+The HIP library is the only compute path: if libmpulse_hip.so is missing this module
+raises on load instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmpulse_hip.so")
+
+ABI_VERSION = 1
+MAX_FACTORIES = 8
+
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
+
+SOLVER_PLRC_NONLIN, SOLVER_PLRC_LIN, SOLVER_PLAIN, SOLVER_NL, SOLVER_DISP = 0, 1, 2, 3, 4
+BOUNDARY_PERIODIC, BOUNDARY_XY_PERIODIC = 0, 1
+PREC_FP64, PREC_FP32_BULK = 0, 1
+PATH_FUSED, PATH_REFERENCE = 0, 1
+FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
+
+# Direction enum of the reference (src/mpulse.h:28)
+NORTH, SOUTH, WEST, EAST, UP, DOWN = 0, 1, 2, 3, 4, 5
+
+SOLVER_KEYS = {"fdtd-plrc-nonlin": SOLVER_PLRC_NONLIN, "fdtd-plrc-lin": SOLVER_PLRC_LIN,
+               "fdtd-plain": SOLVER_PLAIN, "fdtd-nl": SOLVER_NL, "fdtd-disp": SOLVER_DISP}
+
+
+class Factory(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("d", C.c_int),
+        ("kappa_max", C.c_double), ("a_max", C.c_double), ("sigma_max", C.c_double),
+        ("length", C.c_double), ("width", C.c_double), ("om0", C.c_double),
+        ("tshift", C.c_double), ("zshift", C.c_double), ("phase", C.c_double),
+        ("amp", C.c_double), ("eps", C.c_double),
+        ("k", C.c_double * 3), ("h", C.c_double * 3), ("h_bg", C.c_double * 3), ("e_bg", C.c_double * 3),
+        ("ramp", C.c_double), ("gwidth", C.c_double), ("goffset", C.c_double),
+    ]
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("solver", C.c_int), ("boundary", C.c_int), ("precision", C.c_int), ("path", C.c_int),
+        ("eps", C.c_double), ("chi", C.c_double),
+        ("l_eps", C.c_double * 3), ("l_delta", C.c_double * 3), ("l_om2", C.c_double * 3),
+        ("d_eps2", C.c_double * 3), ("d_om2", C.c_double * 3),
+        ("n_factories", C.c_int),
+        ("factories", Factory * MAX_FACTORIES),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("n_classes", C.c_int),
+        ("name", (C.c_char * 32) * 16),
+        ("mean_ms", C.c_double * 16),
+        ("launches", C.c_int * 16),
+        ("step_ms", C.c_double),
+    ]
+
+
+def desc_defaults():
+    """Descriptor with the reference's MakeParamMap defaults (src/globals.cpp:21-36,
+    src/fdtd_plrc.t:411-473)."""
+    d = Desc()
+    d.abi_version = ABI_VERSION
+    d.nx = d.ny = d.nz = 128
+    d.dx = d.dy = d.dz = 0.1
+    d.dt = 1e-2
+    d.solver = SOLVER_PLRC_NONLIN
+    d.eps, d.chi = 1.0, 0.1
+    for n in range(3):
+        d.l_om2[n] = 1.0
+    return d
+
+
+def factory_defaults(kind):
+    f = Factory()
+    f.kind = kind
+    if kind in (FACTORY_CPML, FACTORY_CPML_1D):
+        f.d, f.kappa_max, f.a_max, f.sigma_max = 8, 15.0, 0.25, 3.0
+    else:
+        f.d = 15
+        f.length, f.width, f.om0, f.amp, f.eps = 1.0, 1.0, 2 * np.pi, 1.0, 1.0
+        f.k[2] = 1.0
+        f.ramp, f.gwidth, f.goffset = 0.5, 10.0, 40.0
+    return f
+
+
+def add_factory(desc, fac):
+    if desc.n_factories >= MAX_FACTORIES:
+        raise ValueError("too many factories")
+    desc.factories[desc.n_factories] = fac
+    desc.n_factories += 1
+    return desc
+
+
+class MpxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libmpulse_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(there is no fallback compute path)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "mpx_create": (C.c_int, [C.POINTER(Desc), C.POINTER(P)]),
+        "mpx_destroy": (C.c_int, [P]),
+        "mpx_step": (C.c_int, [P, C.c_int]),
+        "mpx_step_init": (C.c_int, [P]),
+        "mpx_get_time": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "mpx_set_time": (C.c_int, [P, C.c_int]),
+        "mpx_get_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]),
+        "mpx_set_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]),
+        "mpx_get_layer": (C.c_int, [P, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_size_t,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "mpx_set_probes": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
+        "mpx_get_probes": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "mpx_energy": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "mpx_sync": (C.c_int, [P]),
+        "mpx_time_steps": (C.c_int, [P, C.c_int, C.POINTER(Timing)]),
+        "mpx_state_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "mpx_last_error": (C.c_char_p, []),
+        "mpx_abi_version": (C.c_int, []),
+        "mpx_desc_defaults": (None, [C.POINTER(Desc)]),
+        "mpx_factory_defaults": (None, [C.POINTER(Factory), C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.mpx_abi_version() != ABI_VERSION:
+        raise ImportError("libmpulse_hip.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """Owner of one mpx_solver handle (device state on the current GPU)."""
+
+    def __init__(self, desc):
+        self.lib = load()
+        self.desc = desc
+        self.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
+        h = C.c_void_p()
+        self._check(self.lib.mpx_create(C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != OK:
+            raise MpxError(rc, self.lib.mpx_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n=1):
+        self._check(self.lib.mpx_step(self.h, n))
+
+    def step_init(self):
+        self._check(self.lib.mpx_step_init(self.h))
+
+    @property
+    def time(self):
+        t = C.c_int()
+        self._check(self.lib.mpx_get_time(self.h, C.byref(t)))
+        return t.value
+
+    def sync(self):
+        self._check(self.lib.mpx_sync(self.h))
+
+    def get(self, name):
+        out = np.empty(self.shape, dtype=np.float64)
+        self._check(self.lib.mpx_get_field(self.h, name.encode(), _dptr(out), out.size))
+        return out
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected shape {self.shape}")
+        self._check(self.lib.mpx_set_field(self.h, name.encode(), _dptr(a), a.size))
+
+    def get_layer(self, name, direction):
+        lo = (C.c_int * 3)()
+        hi = (C.c_int * 3)()
+        self._check(self.lib.mpx_get_layer(self.h, name.encode(), direction, None, 0, lo, hi))
+        shape = tuple(hi[d] - lo[d] + 1 for d in range(3))
+        out = np.empty(shape, dtype=np.float64)
+        self._check(self.lib.mpx_get_layer(self.h, name.encode(), direction, _dptr(out), out.size, lo, hi))
+        return out, tuple(lo), tuple(hi)
+
+    def set_probes(self, ijk, comps, capacity):
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        comps = np.ascontiguousarray(comps, dtype=np.int32).reshape(-1)
+        self._check(self.lib.mpx_set_probes(self.h, len(comps), ijk.ctypes.data_as(C.POINTER(C.c_int)),
+                                            comps.ctypes.data_as(C.POINTER(C.c_int)), capacity))
+        self._nprobe = len(comps)
+
+    def get_probes(self):
+        n = C.c_int()
+        self._check(self.lib.mpx_get_probes(self.h, None, C.byref(n)))
+        out = np.empty((n.value, self._nprobe), dtype=np.float64)
+        self._check(self.lib.mpx_get_probes(self.h, _dptr(out), C.byref(n)))
+        return out
+
+    def energy(self):
+        e = C.c_double()
+        self._check(self.lib.mpx_energy(self.h, C.byref(e)))
+        return e.value
+
+    def time_steps(self, nsteps):
+        t = Timing()
+        self._check(self.lib.mpx_time_steps(self.h, nsteps, C.byref(t)))
+        res = {"step_ms": t.step_ms, "kernels": {}}
+        for q in range(t.n_classes):
+            res["kernels"][t.name[q].value.decode()] = (t.mean_ms[q], t.launches[q])
+        return res
+
+    def state_bytes(self):
+        b = C.c_size_t()
+        self._check(self.lib.mpx_state_bytes(self.h, C.byref(b)))
+        return b.value

@@ -1,0 +1,129 @@
+"""Workload descriptors in the reference's normalised units (eps0 = mu0 = c = 1).
+
+Unit mapping (one choice, used everywhere in this package):
+  length unit  L0 = lambda / (2 pi) with lambda = 800 nm, so the laser frequency omega_L = 1;
+  time unit    t0 = L0 / c = 0.42471 fs;
+  field unit   E0 = 1e9 V/m, so chi3 [m^2/V^2] maps to chi = chi3 * E0^2 and a peak field of
+               1e9 V/m is amp = 1.
+  grid         dx = dy = dz = lambda/20 = 2 pi / 20, dt = 0.5 dx / sqrt(3).
+  Lorentz pole chi(w) = deps Om^2 / (Om^2 - w^2 - 2 i delta w), the form of the PLRC solver's
+               kernel gamma e^{-delta t} sin(beta t); a damping "gamma" quoted in the
+               Om^2 - w^2 - i gamma w convention is delta = gamma / 2.
+  pulse        the FWHM is the intensity FWHM of the exp(-t^2/T^2) field envelope,
+               T = FWHM / sqrt(2 ln 2); the peak arrives at the source plane at TShift = 2 T;
+               the focus sits on the source plane (ZShift = (d - c_z) dz); the pulse's own
+               eps is the medium's eps at omega_L (the real part of the Lorentz response).
+  CPML         sigma_max = -(m + 1) ln(R0) / (2 d dx) with m = 3 (the cubic grading of the
+               reference), kappaMax and aMax at the reference defaults 15 and 0.25.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+LAMBDA = 800e-9
+C0 = 299792458.0
+L0 = LAMBDA / (2 * math.pi)
+T0 = L0 / C0
+E0 = 1e9
+DX = 2 * math.pi / 20
+DT = 0.5 * DX / math.sqrt(3)
+
+
+def fs_to_t(fs):
+    return fs * 1e-15 / T0
+
+
+def sigma_max(d, r0=1e-6, m=3, dx=DX):
+    return -(m + 1) * math.log(r0) / (2 * d * dx)
+
+
+def splitmix64(seed):
+    """SplitMix64 (S. Vigna): a language-independent stream of 64-bit integers."""
+    state = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def probe_cells(n, d_cpml, sheet_planes, count=8, seed=0):
+    """count interior cells drawn uniformly in [d+1, N-d]^3 (SplitMix64, seed), avoiding the
+    source-sheet planes."""
+    rng = splitmix64(seed)
+    lo, hi = d_cpml + 1, n - d_cpml
+    cells = []
+    while len(cells) < count:
+        c = [lo + int(next(rng) % (hi - lo + 1)) for _ in range(3)]
+        if c[2] in sheet_planes or c in cells:
+            continue
+        cells.append(c)
+    return np.array(cells, dtype=np.int32)
+
+
+def medium_eps_at_laser(eps_inf, poles):
+    e = eps_inf
+    for deps, om2, _delta in poles:
+        e += deps * om2 / (om2 - 1.0)
+    return e
+
+
+def plrc_kerr_pulse(n, d_cpml, fwhm_fs, waist_cells, peak_field, steps, chi3_si=2e-22, d_src=12,
+                    precision=abi.PREC_FP64, path=abi.PATH_FUSED, nz=None, boundary=abi.BOUNDARY_PERIODIC,
+                    poles=((1.1, 4.0, 0.005),)):
+    """Single-pole Lorentz + Kerr medium, CPML on every face, short pulse injected upward from
+    the plane k = d_src (E sheet at d_src, H sheet at d_src - 1)."""
+    nz = n if nz is None else nz
+    d = abi.desc_defaults()
+    d.nx, d.ny, d.nz = n, n, nz
+    d.dx = d.dy = d.dz = DX
+    d.dt = DT
+    d.solver = abi.SOLVER_PLRC_NONLIN
+    d.boundary = boundary
+    d.precision = precision
+    d.path = path
+    d.eps = 1.0
+    d.chi = chi3_si * E0 * E0
+    for q in range(3):
+        d.l_eps[q], d.l_om2[q], d.l_delta[q] = 0.0, 1.0, 0.0
+    for q, (deps, om2, delta) in enumerate(poles):
+        d.l_eps[q], d.l_om2[q], d.l_delta[q] = deps, om2, delta
+    cp = abi.factory_defaults(abi.FACTORY_CPML)
+    cp.d = d_cpml
+    cp.sigma_max = sigma_max(d_cpml)
+    abi.add_factory(d, cp)
+    sp = abi.factory_defaults(abi.FACTORY_SHORT_PULSE)
+    T = fs_to_t(fwhm_fs) / math.sqrt(2 * math.log(2))
+    sp.d = d_src
+    sp.length = T
+    sp.width = waist_cells * DX
+    sp.om0 = 1.0
+    sp.amp = peak_field / E0
+    sp.eps = medium_eps_at_laser(1.0, poles)
+    sp.tshift = 2 * T
+    sp.zshift = (d_src - 0.5 * (nz + 1)) * DX
+    sp.phase = 0.0
+    abi.add_factory(d, sp)
+    meta = {"steps": steps, "d_cpml": d_cpml, "d_src": d_src,
+            "probes": probe_cells(n, d_cpml, (d_src - 1, d_src), seed=0)}
+    return d, meta
+
+
+def config_small(precision=abi.PREC_FP64, path=abi.PATH_FUSED):
+    """64^3, one pole + Kerr, 10-cell CPML, 10 fs pulse of waist 15 cells at 1e9 V/m, 400 steps."""
+    return plrc_kerr_pulse(64, 10, 10.0, 15, 1e9, 400, precision=precision, path=path)
+
+
+def config_roofline(precision=abi.PREC_FP32_BULK, path=abi.PATH_FUSED, n=512, steps=1000):
+    """512^3, one pole + Kerr, 16-cell CPML, 20 fs pulse of waist 60 cells at 5e9 V/m, 1000 steps."""
+    return plrc_kerr_pulse(n, 16, 20.0, 60, 5e9, steps, precision=precision, path=path)
+
+
+def desc_summary(d):
+    fac = [d.factories[q] for q in range(d.n_factories)]
+    return {"grid": [d.nx, d.ny, d.nz], "dx": d.dx, "dt": d.dt, "solver": d.solver, "precision": d.precision,
+            "path": d.path, "chi": d.chi, "poles": [(d.l_eps[q], d.l_om2[q], d.l_delta[q]) for q in range(3)],
+            "factories": [f.kind for f in fac]}

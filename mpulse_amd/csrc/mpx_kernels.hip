@@ -1,0 +1,862 @@
+// gfx950 kernels of the MPulse time-stepping core.
+//
+// Two kernel structures advance the same state:
+//  * the reference-structured path, one kernel per loop of FDTD_PLRCSolver::stepScheme
+//    (src/fdtd_plrc.t:127-168): CPML currents, source sheets, current totals, E update,
+//    ghost wrap, and the same for the magnetic side;
+//  * the fused path, one E-side kernel (Yee curl + PLRC pole recursion + Kerr Newton solve +
+//    CPML E auxiliaries + source sheet) and one B-side kernel (curl + CPML H auxiliaries +
+//    source sheet), with the periodic ghost wrap done by index arithmetic.
+// Threads run along z (the contiguous dimension), 64 per wavefront, so every field load of a
+// wavefront is one coalesced 256/512-byte segment.
+#include <hip/hip_runtime.h>
+
+#include "mpx_launch.h"
+
+namespace mpx {
+
+// ------------------------------------------------------------------------------------
+// source sheets: IncidentSource{E,H}Current::stepScheme, src/incsource.t:40-73, 112-145
+// ------------------------------------------------------------------------------------
+
+__device__ inline void plane_getH(const PlaneParams &W, int i, int j, int l, int time, double out[3])
+{
+    // PlaneWaveSourceEFunc::getHField src/sources.cpp:107-125 / PlaneGaussSourceEFunc src/sources.cpp:266-284
+    double realtime = W.dt * (time - 0.5);
+    double posx = W.k[0] * i * W.dx + W.k[1] * (j + 0.5) * W.dy + W.k[2] * (l + 0.5) * W.dz + W.om * realtime;
+    double posy = W.k[0] * (i + 0.5) * W.dx + W.k[1] * j * W.dy + W.k[2] * (l + 0.5) * W.dz + W.om * realtime;
+    double posz = W.k[0] * (i + 0.5) * W.dx + W.k[1] * (j + 0.5) * W.dy + W.k[2] * l * W.dz + W.om * realtime;
+    double pos[3] = {posx, posy, posz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!W.gauss) {
+            double h = W.H[d] * sin(pos[d]) + W.H_bg[d];
+            if (pos[d] < W.ramp) h *= pos[d] / W.ramp;
+            out[d] = h;
+        } else {
+            double q = (pos[d] - W.offset) / W.width;
+            out[d] = (W.H[d] * exp(-(q * q))) * sin(pos[d]);
+        }
+    }
+}
+
+__device__ inline void plane_getE(const PlaneParams &W, int i, int j, int l, int time, double out[3])
+{
+    // PlaneWaveSourceHFunc::getEField src/sources.cpp:150-173 / PlaneGaussSourceHFunc src/sources.cpp:308-331
+    double realtime = W.dt * time;
+    double posx = W.k[0] * (i + 0.5) * W.dx + W.k[1] * j * W.dy + W.k[2] * l * W.dz + W.om * realtime;
+    double posy = W.k[0] * i * W.dx + W.k[1] * (j + 0.5) * W.dy + W.k[2] * l * W.dz + W.om * realtime;
+    double posz = W.k[0] * i * W.dx + W.k[1] * j * W.dy + W.k[2] * (l + 0.5) * W.dz + W.om * realtime;
+    double pos[3] = {posx, posy, posz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!W.gauss) {
+            double e = W.E[d] * sin(pos[d]) + W.E_bg[d];
+            if (pos[d] < W.ramp) e *= pos[d] / W.ramp;
+            out[d] = e;
+        } else {
+            double q = (pos[d] - W.offset) / W.width;
+            out[d] = (W.E[d] * exp(-(q * q))) * sin(pos[d]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables ft)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= a.ny2 || i >= a.nx2) return;
+    int z = a.sheet.kp + a.off;
+    double f[3] = {0, 0, 0};
+    const PulseParams &P = a.pulse;
+    if (a.kind == 1) {
+        if (!a.isH) {
+            // getHField, src/shortpulseinject.cpp:151-178
+            double posxo = (i - P.centrex) * P.DX;
+            double posxh = (i + 0.5 - P.centrex) * P.DX;
+            double posyo = (j - P.centrey) * P.DY;
+            double posyh = (j + 0.5 - P.centrey) * P.DY;
+            double poszh = (z - 0.5 - P.centrez) * P.DZ - P.ZShift;
+            double posTime = (a.time - 0.5) * P.DT - P.TShift;
+            f[0] = P.amp * pulse_bfunc(P, ft, posxo, posyh, poszh, posTime, true).re;
+            f[1] = P.amp * pulse_bfunc(P, ft, posxh, posyo, poszh, posTime, false).re;
+        } else {
+            // getEField, src/shortpulseinject.cpp:126-149
+            double posxh = (i + 0.5 - P.centrex) * P.DX;
+            double posyo = (j - P.centrey) * P.DY;
+            double poszo = (z - P.centrez) * P.DZ - P.ZShift;
+            double posTime = a.time * P.DT - P.TShift;
+            f[0] = P.amp * pulse_efunc(P, ft, posxh, posyo, poszo, posTime).re;
+        }
+    } else {
+        if (!a.isH)
+            plane_getH(a.plane, i, j, z, a.time, f);
+        else
+            plane_getE(a.plane, i, j, z, a.time, f);
+    }
+    size_t q = (size_t)i * a.ny2 + j;
+    a.sheet.J[a.t1][q] = f[a.t2] / a.DX;
+    a.sheet.J[a.t2][q] = f[a.t1] / a.DX;
+}
+
+void launch_sheet_eval(const SheetEval &a, const FaddeevaTables &ft, hipStream_t s)
+{
+    dim3 blk(64);
+    dim3 grd((a.ny2 + 63) / 64, a.nx2);
+    hipLaunchKernelGGL(k_sheet_eval, grd, blk, 0, s, a, ft);
+}
+
+// ------------------------------------------------------------------------------------
+// reference-structured path
+// ------------------------------------------------------------------------------------
+
+// CPMLBorderECurrent::stepScheme, src/cpml_border.cpp:398-425
+__global__ __launch_bounds__(256) void k_ref_cpml_E(StepArgs<double> a, int fi)
+{
+    const CpmlFace &f = a.face[fi];
+    int k = f.lo[2] + blockIdx.x * 64 + threadIdx.x;
+    int j = f.lo[1] + blockIdx.y * 4 + threadIdx.y;
+    int i = f.lo[0] + blockIdx.z;
+    if (k > f.hi[2] || j > f.hi[1]) return;
+    int ind[3] = {i, j, k};
+    int m[3] = {i, j, k};
+    m[f.dim] -= 1;
+    int q = ind[f.dim] - f.lo[f.dim];
+    const GridDims &g = a.g;
+    size_t c = g.at(i, j, k), cm = g.at(m[0], m[1], m[2]);
+    const double *B0 = a.B[f.f0], *B1 = a.B[f.f1];
+    double *p0 = (double *)f.psi0, *p1 = (double *)f.psi1;
+    size_t l = f.at(i, j, k);
+    p0[l] = f.b[q] * p0[l] - f.c[q] * (B1[c] - B1[cm]) / f.dxn;
+    p1[l] = f.b[q] * p1[l] + f.c[q] * (B0[c] - B0[cm]) / f.dxn;
+}
+
+// CPMLBorderHCurrent::stepScheme, src/cpml_border.cpp:499-527
+__global__ __launch_bounds__(256) void k_ref_cpml_H(StepArgs<double> a, int fi)
+{
+    const CpmlFace &f = a.face[fi];
+    int k = f.lo[2] + blockIdx.x * 64 + threadIdx.x;
+    int j = f.lo[1] + blockIdx.y * 4 + threadIdx.y;
+    int i = f.lo[0] + blockIdx.z;
+    if (k > f.hi[2] || j > f.hi[1]) return;
+    int ind[3] = {i, j, k};
+    int p[3] = {i, j, k};
+    p[f.dim] += 1;
+    int q = ind[f.dim] - f.lo[f.dim];
+    const GridDims &g = a.g;
+    size_t c = g.at(i, j, k), cp = g.at(p[0], p[1], p[2]);
+    const double *E0 = a.E[f.f0], *E1 = a.E[f.f1];
+    double *p0 = (double *)f.psi0, *p1 = (double *)f.psi1;
+    size_t l = f.at(i, j, k);
+    p0[l] = f.b[q] * p0[l] + f.c[q] * (E1[cp] - E1[c]) / f.dxn;
+    p1[l] = f.b[q] * p1[l] - f.c[q] * (E0[cp] - E0[c]) / f.dxn;
+}
+
+void launch_ref_cpml(const StepArgs<double> &a, int fi, bool isH, hipStream_t s)
+{
+    const CpmlFace &f = a.face[fi];
+    dim3 blk(64, 4);
+    dim3 grd((f.hi[2] - f.lo[2] + 64) / 64, (f.hi[1] - f.lo[1] + 4) / 4, f.hi[0] - f.lo[0] + 1);
+    if (isH)
+        hipLaunchKernelGGL(k_ref_cpml_H, grd, blk, 0, s, a, fi);
+    else
+        hipLaunchKernelGGL(k_ref_cpml_E, grd, blk, 0, s, a, fi);
+}
+
+// FDTD_PLRCSolver::sumCurrents / sumMagCurrents, src/fdtd_plrc.t:342-409: totals := 0, then
+// += every current's layer in list order, cell by cell.
+__global__ __launch_bounds__(256) void k_ref_sum(GridDims g, SumList L, double *T0, double *T1, double *T2)
+{
+    int k = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    int i = blockIdx.z;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < L.n; ++v) {
+        const SumView &w = L.v[v];
+        if (i < w.lo[0] || i > w.hi[0] || j < w.lo[1] || j > w.hi[1] || k < w.lo[2] || k > w.hi[2]) continue;
+        size_t l = ((size_t)(i - w.lo[0]) * w.n1 + (size_t)(j - w.lo[1])) * w.n2 + (size_t)(k - w.lo[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (w.comp[c]) t[c] += w.comp[c][l];
+    }
+    size_t c = g.at(i, j, k);
+    T0[c] = t[0];
+    T1[c] = t[1];
+    T2[c] = t[2];
+}
+
+void launch_ref_sum(const GridDims &g, const SumList &L, double *const T[3], hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 2 + 63) / 64, (g.n[1] + 2 + 3) / 4, g.n[0] + 2);
+    hipLaunchKernelGGL(k_ref_sum, grd, blk, 0, s, g, L, T[0], T[1], T[2]);
+}
+
+// Storage::applyBoundary + SinglePeriodicBoundary::exchange{X,Y,Z}, src/storage.cpp:194-204,
+// src/periodic_bound.cpp:10-71: f(0) = f(N), f(N+1) = f(1) over the full planes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wrap(GridDims g, T *f0, T *f1, T *f2, int dim)
+{
+    T *f = blockIdx.z == 0 ? f0 : (blockIdx.z == 1 ? f1 : f2);
+    int a = blockIdx.x * 64 + threadIdx.x;  // fastest remaining dimension
+    int b = blockIdx.y * 4 + threadIdx.y;
+    int N = g.n[dim];
+    if (dim == 0) {
+        if (a > g.n[2] + 1 || b > g.n[1] + 1) return;
+        f[g.at(0, b, a)] = f[g.at(N, b, a)];
+        f[g.at(N + 1, b, a)] = f[g.at(1, b, a)];
+    } else if (dim == 1) {
+        if (a > g.n[2] + 1 || b > g.n[0] + 1) return;
+        f[g.at(b, 0, a)] = f[g.at(b, N, a)];
+        f[g.at(b, N + 1, a)] = f[g.at(b, 1, a)];
+    } else {
+        if (a > g.n[1] + 1 || b > g.n[0] + 1) return;
+        f[g.at(b, a, 0)] = f[g.at(b, a, N)];
+        f[g.at(b, a, N + 1)] = f[g.at(b, a, 1)];
+    }
+}
+
+template <typename T>
+void launch_wrap(const GridDims &g, T *const f[3], int dim, hipStream_t s)
+{
+    int na = dim == 2 ? g.n[1] + 2 : g.n[2] + 2;
+    int nb = dim == 0 ? g.n[1] + 2 : g.n[0] + 2;
+    dim3 blk(64, 4);
+    dim3 grd((na + 63) / 64, (nb + 3) / 4, 3);
+    hipLaunchKernelGGL(k_wrap<T>, grd, blk, 0, s, g, f[0], f[1], f[2], dim);
+}
+template void launch_wrap<double>(const GridDims &, double *const[3], int, hipStream_t);
+template void launch_wrap<float>(const GridDims &, float *const[3], int, hipStream_t);
+
+// ------------------------------------------------------------------------------------
+// PLRC + Kerr E update core, shared by both paths:
+// FDTD_PLRCNonlinCore::plrcStepD src/fdtd_plrc.cpp:304-427,
+// FDTD_PLRCLinCore::plrcStepD    src/fdtd_plrc.cpp:94-261.
+// d* are the B differences in the reference's order, kd* the kappa*delta factors.
+// ------------------------------------------------------------------------------------
+template <int NP, bool NONLIN>
+__device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
+                                              double dBx_z, double dBz_x, double dBy_x, double dBx_y,
+                                              double kdx, double kdy, double kdz, double jx, double jy, double jz,
+                                              double2 *const psi[3][3], size_t c, const PoleConsts &pc,
+                                              double dt, double denom, double numer, double chi)
+{
+    double Psix = 0, Psiy = 0, Psiz = 0;
+#pragma unroll
+    for (int n = 0; n < NP; ++n) {
+        double2 px = psi[n][0][c], py = psi[n][1][c], pz = psi[n][2][c];
+        const double2 D = pc.dd[n], X = pc.dxi0[n], C = pc.crec[n];
+        double pxr = D.x * ex + px.x, pxi = D.y * ex + px.y;
+        double pyr = D.x * ey + py.x, pyi = D.y * ey + py.y;
+        double pzr = D.x * ez + pz.x, pzi = D.y * ez + pz.y;
+        Psix += pxr;
+        Psiy += pyr;
+        Psiz += pzr;
+        psi[n][0][c] = make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr));
+        psi[n][1][c] = make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr));
+        psi[n][2][c] = make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr));
+    }
+    if (NONLIN) {
+        double E2 = ex * ex + ey * ey + ez * ez;
+        double f = numer + chi * E2;
+        double cx = (f * ex + dt * ((dBz_y / kdy - dBy_z / kdz) + jx) + Psix) / denom;
+        double cy = (f * ey + dt * ((dBx_z / kdz - dBz_x / kdx) + jy) + Psiy) / denom;
+        double cz = (f * ez + dt * ((dBy_x / kdx - dBx_y / kdy) + jz) + Psiz) / denom;
+        double E = sqrt(E2);
+        double A = chi / denom;
+        double Cm = sqrt(cx * cx + cy * cy + cz * cz);
+        if (Cm > 0) {
+            // the reference's stopping rule |E - Eold| <= 1e-9; the cap only guards a cell that
+            // could never meet it (non-finite data, or fields far from O(1))
+            double Eold;
+            int it = 0;
+            do {
+                Eold = E;
+                double Q = E * E;
+                E = (2 * A * Q * E + Cm) / (3 * A * Q + 1);
+            } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
+            ex = E * cx / Cm;
+            ey = E * cy / Cm;
+            ez = E * cz / Cm;
+        } else {
+            ex = 0.0;
+            ey = 0.0;
+            ez = 0.0;
+        }
+    } else {
+        double exn = (numer * ex + (dt * ((dBz_y / kdy - dBy_z / kdz) + jx) + Psix)) / denom;
+        double eyn = (numer * ey + (dt * ((dBx_z / kdz - dBz_x / kdx) + jy) + Psiy)) / denom;
+        double ezn = (numer * ez + (dt * ((dBy_x / kdx - dBx_y / kdy) + jz) + Psiz)) / denom;
+        ex = exn;
+        ey = eyn;
+        ez = ezn;
+    }
+}
+
+// FDTD_PLRCSolver::stepD loop, src/fdtd_plrc.t:221-233, reading the current totals
+template <int NP, bool NONLIN>
+__global__ __launch_bounds__(256) void k_ref_stepD(StepArgs<double> a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x + 1;
+    int j = blockIdx.y * 4 + threadIdx.y + 1;
+    int i = blockIdx.z + 1;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    size_t c = g.at(i, j, k);
+    const double *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
+    double bx = Bx[c], by = By[c], bz = Bz[c];
+    double dBz_y = bz - Bz[g.at(i, j - 1, k)];
+    double dBy_z = by - By[g.at(i, j, k - 1)];
+    double dBx_z = bx - Bx[g.at(i, j, k - 1)];
+    double dBz_x = bz - Bz[g.at(i - 1, j, k)];
+    double dBy_x = by - By[g.at(i - 1, j, k)];
+    double dBx_y = bx - Bx[g.at(i, j - 1, k)];
+    double jx = 0, jy = 0, jz = 0;
+    if (a.haveJ) {
+        jx = a.Jt[0][c];
+        jy = a.Jt[1][c];
+        jz = a.Jt[2][c];
+    }
+    double ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
+    plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
+                              a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
+    a.E[0][c] = ex;
+    a.E[1][c] = ey;
+    a.E[2][c] = ez;
+}
+
+// FDTD_PLRCSolver::stepB + plrcStepB, src/fdtd_plrc.t:312-339, src/fdtd_plrc.cpp:429-459;
+// Jt holds the magnetic current totals here.
+__global__ __launch_bounds__(256) void k_ref_stepB(StepArgs<double> a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    int i = blockIdx.z;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    size_t c = g.at(i, j, k);
+    const double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+    double jx = 0, jy = 0, jz = 0;
+    if (a.haveJ) {
+        jx = a.Jt[0][c];
+        jy = a.Jt[1][c];
+        jz = a.Jt[2][c];
+    }
+    double kdx = a.kdH[0][i], kdy = a.kdH[1][j], kdz = a.kdH[2][k];
+    double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    a.B[0][c] = a.B[0][c] + a.dt * ((Ey[g.at(i, j, k + 1)] - ey) / kdz - (Ez[g.at(i, j + 1, k)] - ez) / kdy + jx);
+    a.B[1][c] = a.B[1][c] + a.dt * ((Ez[g.at(i + 1, j, k)] - ez) / kdx - (Ex[g.at(i, j, k + 1)] - ex) / kdz + jy);
+    a.B[2][c] = a.B[2][c] + a.dt * ((Ex[g.at(i, j + 1, k)] - ex) / kdy - (Ey[g.at(i + 1, j, k)] - ey) / kdx + jz);
+}
+
+void launch_ref_stepD(const StepArgs<double> &a, int np, bool nonlin, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, g.n[0]);
+#define MPX_REF_D(NPV, NL) hipLaunchKernelGGL((k_ref_stepD<NPV, NL>), grd, blk, 0, s, a)
+    if (nonlin) {
+        switch (np) {
+        case 0: MPX_REF_D(0, true); break;
+        case 1: MPX_REF_D(1, true); break;
+        case 2: MPX_REF_D(2, true); break;
+        default: MPX_REF_D(3, true); break;
+        }
+    } else {
+        switch (np) {
+        case 0: MPX_REF_D(0, false); break;
+        case 1: MPX_REF_D(1, false); break;
+        case 2: MPX_REF_D(2, false); break;
+        default: MPX_REF_D(3, false); break;
+        }
+    }
+#undef MPX_REF_D
+}
+
+void launch_ref_stepB(const StepArgs<double> &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[0] + 1);
+    hipLaunchKernelGGL(k_ref_stepB, grd, blk, 0, s, a);
+}
+
+// FDTD_PLRCSolver::initAccumulator, src/fdtd_plrc.t:273-308 (stepSchemeInit only)
+template <int NP>
+__global__ __launch_bounds__(256) void k_init_accumulator(StepArgs<double> a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x + 1;
+    int j = blockIdx.y * 4 + threadIdx.y + 1;
+    int i = blockIdx.z + 1;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    size_t c = g.at(i, j, k);
+    double e[3] = {a.E[0][c], a.E[1][c], a.E[2][c]};
+#pragma unroll
+    for (int n = 0; n < NP; ++n)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double2 p = a.psi[n][d][c];
+            const double2 X = a.pc.dxi0[n], C = a.pc.crec[n];
+            a.psi[n][d][c] = make_double2(X.x * e[d] + (C.x * p.x - C.y * p.y), X.y * e[d] + (C.x * p.y + C.y * p.x));
+        }
+}
+
+void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, g.n[0]);
+    switch (np) {
+    case 0: break;
+    case 1: hipLaunchKernelGGL(k_init_accumulator<1>, grd, blk, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_init_accumulator<2>, grd, blk, 0, s, a); break;
+    default: hipLaunchKernelGGL(k_init_accumulator<3>, grd, blk, 0, s, a); break;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// fused path
+// ------------------------------------------------------------------------------------
+
+// CPML E auxiliaries of one face (CPMLBorderECurrent::stepScheme, src/cpml_border.cpp:398-425)
+// updated in the E kernel; returns the two current contributions.  The face map is
+// x: (Jy, Jz; Bz, By), y: (Jz, Jx; Bx, Bz), z: (Jx, Jy; By, Bx) (src/cpml_border.cpp:357-386).
+template <typename T>
+__device__ __forceinline__ void cpml_e_face(const CpmlFace &f, int coord, int i, int j, int k, double d1,
+                                            double d0, double &v0, double &v1)
+{
+    int q = coord - f.lo[f.dim];
+    size_t l = f.at(i, j, k);
+    T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
+    double bq = f.b[q], cq = f.c[q];
+    T n0 = (T)(bq * (double)p0[l] - cq * d1 / f.dxn);
+    T n1 = (T)(bq * (double)p1[l] + cq * d0 / f.dxn);
+    p0[l] = n0;
+    p1[l] = n1;
+    v0 = (double)n0;
+    v1 = (double)n1;
+}
+
+// CPMLBorderHCurrent::stepScheme, src/cpml_border.cpp:499-527, for the B kernel
+template <typename T>
+__device__ __forceinline__ void cpml_h_face(const CpmlFace &f, int coord, int i, int j, int k, double d1,
+                                            double d0, double &v0, double &v1)
+{
+    int q = coord - f.lo[f.dim];
+    size_t l = f.at(i, j, k);
+    T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
+    double bq = f.b[q], cq = f.c[q];
+    T n0 = (T)(bq * (double)p0[l] + cq * d1 / f.dxn);
+    T n1 = (T)(bq * (double)p1[l] - cq * d0 / f.dxn);
+    p0[l] = n0;
+    p1[l] = n1;
+    v0 = (double)n0;
+    v1 = (double)n1;
+}
+
+// add the current-sheet layers that contain cell (i, j, k), in list order
+template <typename T>
+__device__ __forceinline__ void add_sheets(const StepArgs<T> &a, int i, int j, int k, double &x, double &y,
+                                           double &z)
+{
+    for (int s = 0; s < a.nsheets; ++s)
+        if (k == a.sheet[s].kp) {
+            size_t q = (size_t)i * a.g.py + j;
+            x += a.sheet[s].J[0][q];
+            y += a.sheet[s].J[1][q];
+            z += a.sheet[s].J[2][q];
+        }
+}
+
+// Map the linear block id to a tile so that each XCD (blocks are dealt round-robin over
+// the 8 XCDs) walks a contiguous slab of x planes and keeps the x-neighbour planes in its L2.
+__device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int &by, int &bz)
+{
+    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    unsigned total = (unsigned)nbx * nby * nbz;
+    unsigned per = total / 8u;
+    unsigned xcd = lin & 7u, idx = lin >> 3;
+    unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
+    bx = t % nbx;
+    t /= nbx;
+    by = t % nby;
+    bz = t / nby;
+}
+
+template <typename T, int NP, bool NONLIN, bool CPML>
+__global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby)
+{
+    int tbx, tby, tbz;
+    xcd_tile(nbx, nby, a.g.n[0], tbx, tby, tbz);
+    const int k = tbx * 64 + threadIdx.x + 1;
+    const int j = tby * 4 + threadIdx.y + 1;
+    const int i = tbz + 1;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    const int im = (i == 1 && a.periodic[0]) ? g.n[0] : i - 1;
+    const int jm = (j == 1 && a.periodic[1]) ? g.n[1] : j - 1;
+    const int km = (k == 1 && a.periodic[2]) ? g.n[2] : k - 1;
+    const size_t c = g.at(i, j, k);
+    const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
+    const double bx = Bx[c], by = By[c], bz = Bz[c];
+    const double dBz_y = bz - (double)Bz[g.at(i, jm, k)];
+    const double dBy_z = by - (double)By[g.at(i, j, km)];
+    const double dBx_z = bx - (double)Bx[g.at(i, j, km)];
+    const double dBz_x = bz - (double)Bz[g.at(im, j, k)];
+    const double dBy_x = by - (double)By[g.at(im, j, k)];
+    const double dBx_y = bx - (double)Bx[g.at(i, jm, k)];
+    double jx = 0.0, jy = 0.0, jz = 0.0;
+    if (a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+    if (CPML) {
+        // current totals in the reference's list order N,S (y), E,W (x), U,D (z); within a
+        // dimension the high face comes first in that list (face_by_dim[d][0]).
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[1][side];
+            if (f >= 0 && j >= a.face[f].lo[1] && j <= a.face[f].hi[1]) {
+                double v0, v1;  // psi0 -> Jz from dBx_y, psi1 -> Jx from dBz_y
+                cpml_e_face<T>(a.face[f], j, i, j, k, dBx_y, dBz_y, v0, v1);
+                jz += v0;
+                jx += v1;
+            }
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[0][side];
+            if (f >= 0 && i >= a.face[f].lo[0] && i <= a.face[f].hi[0]) {
+                double v0, v1;  // psi0 -> Jy from dBz_x, psi1 -> Jz from dBy_x
+                cpml_e_face<T>(a.face[f], i, i, j, k, dBz_x, dBy_x, v0, v1);
+                jy += v0;
+                jz += v1;
+            }
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[2][side];
+            if (f >= 0 && k >= a.face[f].lo[2] && k <= a.face[f].hi[2]) {
+                double v0, v1;  // psi0 -> Jx from dBy_z, psi1 -> Jy from dBx_z
+                cpml_e_face<T>(a.face[f], k, i, j, k, dBy_z, dBx_z, v0, v1);
+                jx += v0;
+                jy += v1;
+            }
+        }
+    }
+    if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+    double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
+    plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
+                              a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
+    a.E[0][c] = (T)ex;
+    a.E[1][c] = (T)ey;
+    a.E[2][c] = (T)ez;
+}
+
+template <typename T, bool CPML>
+__global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby, int nbz)
+{
+    int tbx, tby, tbz;
+    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    const GridDims &g = a.g;
+    const int k = tbx * 64 + threadIdx.x + a.blo[2];
+    const int j = tby * 4 + threadIdx.y + a.blo[1];
+    const int i = tbz + a.blo[0];
+    if (k > g.n[2] || j > g.n[1]) return;
+    const int ip = (i == g.n[0] && a.periodic[0]) ? 1 : i + 1;
+    const int jp = (j == g.n[1] && a.periodic[1]) ? 1 : j + 1;
+    const int kp = (k == g.n[2] && a.periodic[2]) ? 1 : k + 1;
+    const size_t c = g.at(i, j, k);
+    const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+    const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    const double dEy_z = (double)Ey[g.at(i, j, kp)] - ey;
+    const double dEz_y = (double)Ez[g.at(i, jp, k)] - ez;
+    const double dEz_x = (double)Ez[g.at(ip, j, k)] - ez;
+    const double dEx_z = (double)Ex[g.at(i, j, kp)] - ex;
+    const double dEx_y = (double)Ex[g.at(i, jp, k)] - ex;
+    const double dEy_x = (double)Ey[g.at(ip, j, k)] - ey;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    if (a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
+    if (CPML) {
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[1][side];
+            if (f >= 0 && j >= a.face[f].lo[1] && j <= a.face[f].hi[1]) {
+                double v0, v1;  // psi0 -> Mz from dEx_y, psi1 -> Mx from dEz_y
+                cpml_h_face<T>(a.face[f], j, i, j, k, dEx_y, dEz_y, v0, v1);
+                mz += v0;
+                mx += v1;
+            }
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[0][side];
+            if (f >= 0 && i >= a.face[f].lo[0] && i <= a.face[f].hi[0]) {
+                double v0, v1;  // psi0 -> My from dEz_x, psi1 -> Mz from dEy_x
+                cpml_h_face<T>(a.face[f], i, i, j, k, dEz_x, dEy_x, v0, v1);
+                my += v0;
+                mz += v1;
+            }
+        }
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            int f = a.face_by_dim[2][side];
+            if (f >= 0 && k >= a.face[f].lo[2] && k <= a.face[f].hi[2]) {
+                double v0, v1;  // psi0 -> Mx from dEy_z, psi1 -> My from dEx_z
+                cpml_h_face<T>(a.face[f], k, i, j, k, dEy_z, dEx_z, v0, v1);
+                mx += v0;
+                my += v1;
+            }
+        }
+    }
+    if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
+    const double kdx = a.kdH[0][i], kdy = a.kdH[1][j], kdz = a.kdH[2][k];
+    T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
+    Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z / kdz - dEz_y / kdy) + mx));
+    By[c] = (T)((double)By[c] + a.dt * ((dEz_x / kdx - dEx_z / kdz) + my));
+    Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+}
+
+template <typename T>
+void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    int nbx = (g.n[2] + 63) / 64, nby = (g.n[1] + 3) / 4;
+    dim3 blk(64, 4);
+    dim3 grd(nbx, nby, g.n[0]);
+    bool cp = a.nfaces > 0;
+#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP>), grd, blk, 0, s, a, nbx, nby)
+#define MPX_FE_NP(NL, CP)            \
+    switch (np) {                    \
+    case 0: MPX_FE(0, NL, CP); break; \
+    case 1: MPX_FE(1, NL, CP); break; \
+    case 2: MPX_FE(2, NL, CP); break; \
+    default: MPX_FE(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cp) { MPX_FE_NP(true, true) } else { MPX_FE_NP(true, false) }
+    } else {
+        if (cp) { MPX_FE_NP(false, true) } else { MPX_FE_NP(false, false) }
+    }
+#undef MPX_FE_NP
+#undef MPX_FE
+}
+
+template <typename T>
+void launch_fused_B(const StepArgs<T> &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    int nk = g.n[2] - a.blo[2] + 1, nj = g.n[1] - a.blo[1] + 1, ni = g.n[0] - a.blo[0] + 1;
+    int nbx = (nk + 63) / 64, nby = (nj + 3) / 4;
+    dim3 blk(64, 4);
+    dim3 grd(nbx, nby, ni);
+    if (a.nfaces > 0)
+        hipLaunchKernelGGL((k_fused_B<T, true>), grd, blk, 0, s, a, nbx, nby, ni);
+    else
+        hipLaunchKernelGGL((k_fused_B<T, false>), grd, blk, 0, s, a, nbx, nby, ni);
+}
+
+template void launch_fused_E<double>(const StepArgs<double> &, int, bool, hipStream_t);
+template void launch_fused_E<float>(const StepArgs<float> &, int, bool, hipStream_t);
+template void launch_fused_B<double>(const StepArgs<double> &, hipStream_t);
+template void launch_fused_B<float>(const StepArgs<float> &, hipStream_t);
+
+// ------------------------------------------------------------------------------------
+// fdtd-plain, fdtd-nl, fdtd-disp (periodic only; no currents)
+// ------------------------------------------------------------------------------------
+
+// E loops of FDTD_Plain / FDTD_Nonlinear / FDTD_Dispersion over [1, N+1]^3
+// (src/fdtd_plain.cpp:54-75, src/fdtd_nl.cpp:55-98, src/fdtd_disp.cpp:113-204)
+__global__ __launch_bounds__(256) void k_simple_stepD(SimpleArgs a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x + 1;
+    int j = blockIdx.y * 4 + threadIdx.y + 1;
+    int i = blockIdx.z + 1;
+    const GridDims &g = a.g;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    size_t c = g.at(i, j, k);
+    double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+    const double *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
+    double dx = a.dx, dy = a.dy, dz = a.dz, dt = a.dt;
+    double cx = (Bz[c] - Bz[g.at(i, j - 1, k)]) / dy - (By[c] - By[g.at(i, j, k - 1)]) / dz;
+    double cy = (Bx[c] - Bx[g.at(i, j, k - 1)]) / dz - (Bz[c] - Bz[g.at(i - 1, j, k)]) / dx;
+    double cz = (By[c] - By[g.at(i - 1, j, k)]) / dx - (Bx[c] - Bx[g.at(i, j - 1, k)]) / dy;
+    double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    if (a.kind == 0) {
+        Ex[c] = ex + dt * cx;
+        Ey[c] = ey + dt * cy;
+        Ez[c] = ez + dt * cz;
+    } else if (a.kind == 1) {
+        double E2 = ex * ex + ey * ey + ez * ez;
+        double Dx = (a.eps + a.chi * E2) * ex + dt * cx;
+        double Dy = (a.eps + a.chi * E2) * ey + dt * cy;
+        double Dz = (a.eps + a.chi * E2) * ez + dt * cz;
+        double E = sqrt(E2);
+        double D = sqrt(Dx * Dx + Dy * Dy + Dz * Dz);
+        double Eold;
+        int it = 0;
+        do {
+            Eold = E;
+            E2 = E * E;
+            E = (2 * a.chi * E2 * E + D) / (3 * a.chi * E2 + a.eps);
+        } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
+        Ex[c] = E * Dx / D;
+        Ey[c] = E * Dy / D;
+        Ez[c] = E * Dz / D;
+    } else {
+        double e[3] = {ex, ey, ez};
+        double P[3] = {0, 0, 0}, Pp[3] = {0, 0, 0};
+        double dt2 = dt * dt;
+        for (int n = 0; n < 3; ++n) {
+            double aa = dt2 * a.om2[n];
+            double b = aa * a.eps2[n];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double p = a.P[n][d][c], pp = a.Pp[n][d][c];
+                double pn = (2 - aa) * p - pp + b * e[d];
+                a.Pp[n][d][c] = p;
+                a.P[n][d][c] = pn;
+                P[d] += pn;
+                Pp[d] += p;
+            }
+        }
+        Ex[c] = ex + Pp[0] - P[0] + dt * cx;
+        Ey[c] = ey + Pp[1] - P[1] + dt * cy;
+        Ez[c] = ez + Pp[2] - P[2] + dt * cz;
+    }
+}
+
+// stepB shared by the three solvers, including the /dx of both Bz terms
+// (src/fdtd_plain.cpp:97-118)
+__global__ __launch_bounds__(256) void k_simple_stepB(SimpleArgs a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    int i = blockIdx.z;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    size_t c = g.at(i, j, k);
+    const double *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
+    double dx = a.dx, dy = a.dy, dz = a.dz, dt = a.dt;
+    double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    a.B[0][c] = a.B[0][c] + dt * ((Ey[g.at(i, j, k + 1)] - ey) / dz - (Ez[g.at(i, j + 1, k)] - ez) / dy);
+    a.B[1][c] = a.B[1][c] + dt * ((Ez[g.at(i + 1, j, k)] - ez) / dx - (Ex[g.at(i, j, k + 1)] - ex) / dz);
+    a.B[2][c] = a.B[2][c] + dt * ((Ex[g.at(i, j + 1, k)] - ex) / dx - (Ey[g.at(i + 1, j, k)] - ey) / dx);
+}
+
+void launch_simple_stepD(const SimpleArgs &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[0] + 1);
+    hipLaunchKernelGGL(k_simple_stepD, grd, blk, 0, s, a);
+}
+
+void launch_simple_stepB(const SimpleArgs &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[0] + 1);
+    hipLaunchKernelGGL(k_simple_stepB, grd, blk, 0, s, a);
+}
+
+// ------------------------------------------------------------------------------------
+// diagnostics: probes and energy
+// ------------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_probe(ProbeArgs<T> a, double *row)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    row[p] = (double)a.f[a.comp[p]][a.idx[p]];
+}
+
+template <typename T>
+void launch_probe(const ProbeArgs<T> &a, double *row, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_probe<T>, dim3((a.n + 63) / 64), dim3(64), 0, s, a, row);
+}
+template void launch_probe<double>(const ProbeArgs<double> &, double *, hipStream_t);
+template void launch_probe<float>(const ProbeArgs<float> &, double *, hipStream_t);
+
+// FieldEnergyDiag::calculate, src/fielddiag.cpp:255-286: per-block partial sums, then one
+// block adds the partials in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_energy_partial(GridDims g, const T *e0, const T *e1, const T *e2,
+                                                        const T *b0, const T *b1, const T *b2, double *part)
+{
+    __shared__ double red[256];
+    int k = blockIdx.x * 64 + threadIdx.x + 1;
+    int j = blockIdx.y * 4 + threadIdx.y + 1;
+    int i = blockIdx.z + 1;
+    double v = 0;
+    if (k <= g.n[2] && j <= g.n[1]) {
+        size_t c = g.at(i, j, k);
+        double ex = e0[c], ey = e1[c], ez = e2[c], bx = b0[c], by = b1[c], bz = b2[c];
+        v = ex * ex + ey * ey + ez * ez + bx * bx + by * by + bz * bz;
+    }
+    int t = threadIdx.y * 64 + threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(const double *part, size_t n, double *out)
+{
+    __shared__ double red[256];
+    double v = 0;
+    for (size_t q = threadIdx.x; q < n; q += 256) v += part[q];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+size_t energy_partials(const GridDims &g)
+{
+    return (size_t)((g.n[2] + 63) / 64) * ((g.n[1] + 3) / 4) * g.n[0];
+}
+
+template <typename T>
+void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grd((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, g.n[0]);
+    hipLaunchKernelGGL(k_energy_partial<T>, grd, blk, 0, s, g, e[0], e[1], e[2], b[0], b[1], b[2], part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, energy_partials(g), out);
+}
+template void launch_energy<double>(const GridDims &, double *const[3], double *const[3], double *, double *,
+                                    hipStream_t);
+template void launch_energy<float>(const GridDims &, float *const[3], float *const[3], double *, double *,
+                                   hipStream_t);
+
+}  // namespace mpx
+
+namespace mpx {
+
+// Re Efunc(0,0,0,0) with Phase = TShift = ZShift = 0: the amplitude normalisation of
+// ShortPulseInjectSourceFunc::setParam, src/shortpulseinject.cpp:101-112
+__global__ void k_pulse_norm(PulseParams P, FaddeevaTables ft, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = pulse_efunc(P, ft, 0, 0, 0, 0).re;
+}
+
+double pulse_norm_device(const PulseParams &P, const FaddeevaTables &ft, hipStream_t s)
+{
+    double *d = nullptr, h = 0;
+    if (hipMalloc(&d, sizeof(double)) != hipSuccess) return NAN;
+    hipLaunchKernelGGL(k_pulse_norm, dim3(1), dim3(64), 0, s, P, ft, d);
+    if (hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) h = NAN;
+    if (hipStreamSynchronize(s) != hipSuccess) h = NAN;
+    (void)hipFree(d);
+    return h;
+}
+
+}  // namespace mpx

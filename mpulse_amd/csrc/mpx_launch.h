@@ -1,0 +1,62 @@
+// Host-side launchers of the gfx950 kernels (defined in mpx_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mpx_types.h"
+
+namespace mpx {
+
+constexpr int kMaxSumViews = kMaxFaces + kMaxSheets;
+
+// one current's layer as seen by sumCurrents: its box and up to three component arrays
+struct SumView {
+    int lo[3], hi[3];
+    int n1, n2;
+    const double *comp[3];
+};
+struct SumList {
+    int n;
+    SumView v[kMaxSumViews];
+};
+
+// fdtd-plain (kind 0), fdtd-nl (kind 1), fdtd-disp (kind 2)
+struct SimpleArgs {
+    GridDims g;
+    int kind;
+    double *E[3];
+    double *B[3];
+    double *P[3][3];
+    double *Pp[3][3];
+    double dx, dy, dz, dt, eps, chi;
+    double eps2[3], om2[3];
+};
+
+template <typename T>
+struct ProbeArgs {
+    int n;
+    const T *f[6];
+    const int *comp;
+    const size_t *idx;
+};
+
+void launch_sheet_eval(const SheetEval &a, const FaddeevaTables &ft, hipStream_t s);
+void launch_ref_cpml(const StepArgs<double> &a, int fi, bool isH, hipStream_t s);
+void launch_ref_sum(const GridDims &g, const SumList &L, double *const T[3], hipStream_t s);
+template <typename T>
+void launch_wrap(const GridDims &g, T *const f[3], int dim, hipStream_t s);
+void launch_ref_stepD(const StepArgs<double> &a, int np, bool nonlin, hipStream_t s);
+void launch_ref_stepB(const StepArgs<double> &a, hipStream_t s);
+void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s);
+template <typename T>
+void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s);
+template <typename T>
+void launch_fused_B(const StepArgs<T> &a, hipStream_t s);
+void launch_simple_stepD(const SimpleArgs &a, hipStream_t s);
+void launch_simple_stepB(const SimpleArgs &a, hipStream_t s);
+template <typename T>
+void launch_probe(const ProbeArgs<T> &a, double *row, hipStream_t s);
+size_t energy_partials(const GridDims &g);
+template <typename T>
+void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s);
+
+}  // namespace mpx

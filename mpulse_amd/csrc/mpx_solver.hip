@@ -1,0 +1,1423 @@
+// Host side of the C ABI declared in include/mpulse_hip.h: builds the device descriptors
+// from an mpx_desc the way the reference builds its currents and storage, owns the device
+// state and enqueues the kernels of one time step on the solver's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpulse_hip.h"
+#include "mpx_launch.h"
+
+using mpx::CpmlFace;
+using mpx::GridDims;
+using mpx::StepArgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define MPX_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(MPX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+#define MPX_LAUNCHED(what)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = hipGetLastError();                                                      \
+        if (e_ != hipSuccess) return fail(MPX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Direction enum of src/mpulse.h:28
+enum { NORTH = 0, SOUTH = 1, WEST = 2, EAST = 3, UP = 4, DOWN = 5 };
+const int kFaceOrder[6] = {NORTH, SOUTH, EAST, WEST, UP, DOWN};  // src/cpml_border.cpp:16-51
+
+int dim_of(int dir) { return (dir == EAST || dir == WEST) ? 0 : ((dir == NORTH || dir == SOUTH) ? 1 : 2); }
+bool is_high(int dir) { return dir == EAST || dir == NORTH || dir == UP; }
+
+enum CurType { CUR_CPML_E = 1, CUR_CPML_H, CUR_INC_E, CUR_INC_H };
+
+struct Layer {
+    std::string name;
+    int dir;
+    int lo[3], hi[3];
+    void *comp[3];  // device arrays (nullptr where the component is never written)
+    size_t count() const
+    {
+        return (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+    }
+};
+
+struct Current {
+    int type;
+    int dir;
+    int factory;
+    int layer = -1;   // index into layers (first of the three component layers)
+    int face = -1;    // CPML: index into the side's face list
+    // source evaluation
+    mpx::SheetEval ev{};
+};
+
+struct Timer {
+    bool on = false;
+    std::vector<int> cls;
+    std::vector<hipEvent_t> ev0, ev1;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t get()
+    {
+        if (used < pool.size()) return pool[used++];
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        pool.push_back(e);
+        ++used;
+        return e;
+    }
+};
+
+const char *kClassNames[] = {"sheet_eval", "cpml_E", "sum_J", "stepD_ref", "wrap_E", "cpml_H", "sum_M",
+                             "stepB_ref", "wrap_B", "fused_E", "fused_B", "simple_E", "simple_B", "probe"};
+enum {
+    K_SHEET = 0, K_CPML_E, K_SUM_J, K_STEPD_REF, K_WRAP_E, K_CPML_H, K_SUM_M, K_STEPB_REF, K_WRAP_B,
+    K_FUSED_E, K_FUSED_B, K_SIMPLE_E, K_SIMPLE_B, K_PROBE, K_NCLASSES
+};
+
+}  // namespace
+
+struct mpx_solver {
+    mpx_desc d{};
+    int solver = 0, prec = 0, path = 0;
+    bool plrc = false, nonlin = false;
+    size_t esz = 8;
+    GridDims g{};
+    size_t cells = 0;  // padded element count of one grid
+    hipStream_t stream = nullptr;
+    int periodic[3] = {1, 1, 1};
+    double delta[3] = {0, 0, 0};
+
+    std::vector<void *> allocs;
+    void *E[3] = {nullptr, nullptr, nullptr};
+    void *B[3] = {nullptr, nullptr, nullptr};
+    double2 *psi[3][3] = {};
+    int np = 0;
+    int pole_ref[3] = {-1, -1, -1};  // active pole slot -> reference pole index
+    std::vector<double> kE[3], kH[3];
+    double *d_kdE[3] = {}, *d_kdH[3] = {};
+
+    std::vector<Layer> layers;
+    std::vector<Current> ecur, hcur;
+    std::vector<CpmlFace> efaces, hfaces;
+    int sheets_first = 0;
+
+    double *Jt[3] = {}, *Mt[3] = {};
+    bool haveJ = false, haveM = false;
+    double *P[3][3] = {}, *Pp[3][3] = {};
+
+    mpx::PoleConsts pc{};
+    double sumChi0 = 0, sumXi0 = 0, denom = 1, numer = 1;
+    mpx::FaddeevaTables ft{};
+
+    int nprobe = 0, probe_cap = 0, probe_rec = 0;
+    int *d_probe_comp = nullptr;
+    size_t *d_probe_idx = nullptr;
+    double *d_probe_hist = nullptr;
+
+    double *d_part = nullptr, *d_energy = nullptr;
+    int time = 0;
+    Timer timer;
+
+    ~mpx_solver()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : timer.pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    T *dalloc(size_t count, int &rc)
+    {
+        void *p = nullptr;
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            rc = fail(MPX_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+            return nullptr;
+        }
+        allocs.push_back(p);
+        if (hipMemsetAsync(p, 0, count * sizeof(T), stream) != hipSuccess) {
+            rc = fail(MPX_ERR_HIP, "hipMemsetAsync failed");
+            return nullptr;
+        }
+        return (T *)p;
+    }
+
+    void tick(int cls)
+    {
+        if (!timer.on) return;
+        timer.cls.push_back(cls);
+        hipEvent_t e = timer.get();
+        (void)hipEventRecord(e, stream);
+        timer.ev0.push_back(e);
+    }
+    void tock()
+    {
+        if (!timer.on) return;
+        hipEvent_t e = timer.get();
+        (void)hipEventRecord(e, stream);
+        timer.ev1.push_back(e);
+    }
+};
+
+// synchronous host-to-device copy ordered after the allocations' clears on the solver stream
+static int upload(mpx_solver *s, void *dst, const void *src, size_t bytes)
+{
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess) return 1;
+    return hipStreamSynchronize(s->stream) != hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------
+// set-up helpers mirroring the reference's Storage / current construction
+// ---------------------------------------------------------------------------------------
+
+// Storage::getBorderExtent, src/storage.cpp:208-318 (single domain, bordercells = 0)
+static bool border_extent(const mpx_solver *s, int dir, int thickness, int distance, int blow[3], int bhigh[3])
+{
+    int low[3] = {0, 0, 0}, high[3], glow[3] = {0, 0, 0}, ghigh[3];
+    for (int d = 0; d < 3; ++d) {
+        high[d] = s->g.n[d] + 1;
+        ghigh[d] = high[d];
+        blow[d] = low[d];
+        bhigh[d] = high[d];
+    }
+    switch (dir) {
+    case WEST:
+        if (low[0] < glow[0] + thickness + distance) {
+            bhigh[0] = glow[0] + thickness - 1 + distance;
+            blow[0] = glow[0] + distance;
+            return true;
+        }
+        return false;
+    case SOUTH:
+        if (low[1] < glow[1] + thickness + distance) {
+            bhigh[1] = glow[1] + thickness - 1 + distance;
+            blow[1] = glow[1] + distance;
+            return true;
+        }
+        return false;
+    case DOWN:
+        if ((low[2] < glow[2] + thickness + distance) && (high[2] >= glow[2] + thickness)) {
+            bhigh[2] = glow[2] + thickness - 1 + distance;
+            blow[2] = glow[2] + distance;
+            return true;
+        }
+        return false;
+    case EAST:
+        if (high[0] > ghigh[0] - thickness - distance) {
+            blow[0] = ghigh[0] - thickness + 1 - distance;
+            bhigh[0] = ghigh[0] - distance;
+            return true;
+        }
+        return false;
+    case NORTH:
+        if (high[1] > ghigh[1] - thickness - distance) {
+            blow[1] = ghigh[1] - thickness + 1 - distance;
+            bhigh[1] = ghigh[1] - distance;
+            return true;
+        }
+        return false;
+    case UP:
+        if (high[2] > ghigh[2] - thickness - distance) {
+            blow[2] = ghigh[2] - thickness + 1 - distance;
+            bhigh[2] = ghigh[2] - distance;
+            return true;
+        }
+        return false;
+    }
+    return false;
+}
+
+// Storage::addBorderLayer, src/storage.cpp:106-142: one grid per (name, direction); a second
+// request for the same pair returns the first grid.  Returns the layer index or -1.
+static int add_border_layer(mpx_solver *s, const std::string &name, int dir, int thickness, int distance)
+{
+    for (size_t q = 0; q < s->layers.size(); ++q)
+        if (s->layers[q].dir == dir && s->layers[q].name == name) return (int)q;
+    Layer L;
+    L.name = name;
+    L.dir = dir;
+    L.comp[0] = L.comp[1] = L.comp[2] = nullptr;
+    if (!border_extent(s, dir, thickness, distance, L.lo, L.hi)) return -1;
+    s->layers.push_back(L);
+    return (int)s->layers.size() - 1;
+}
+
+// CPMLBorder::initCoefficients, src/cpml_border.cpp:58-140
+static void cpml_kappa(mpx_solver *s, int thickness, double kappaMax)
+{
+    for (int dim = 0; dim < 3; ++dim) {
+        std::vector<double> &KE = s->kE[dim], &KH = s->kH[dim];
+        int low = 0, high = s->g.n[dim] + 1, glow = 0, ghigh = high;
+        if (low < glow + thickness) {
+            for (double &v : KE) v = 1.0;
+            for (double &v : KH) v = 1.0;
+            for (int i = 0; i <= thickness; ++i) {
+                double x = 1 - double(i) / double(thickness);
+                double x3 = x * x * x;
+                if (low + i <= high) KE[low + i] = 1 + (kappaMax - 1) * x3;
+            }
+            for (int i = 0; i < thickness; ++i) {
+                double x = 1 - (double(i) + 0.5) / double(thickness);
+                double x3 = x * x * x;
+                if (low + i <= high) KH[low + i] = 1 + (kappaMax - 1) * x3;
+            }
+        }
+        if (high > ghigh - thickness) {
+            for (int i = 0; i <= thickness; ++i) {
+                double x = 1 - double(i) / double(thickness);
+                double x3 = x * x * x;
+                if (high - i >= low) KH[high - i] = 1 + (kappaMax - 1) * x3;
+            }
+            for (int i = 0; i < thickness; ++i) {
+                double x = 1 - (double(i) + 0.5) / double(thickness);
+                double x3 = x * x * x;
+                if (high - i >= low) KE[high - i] = 1 + (kappaMax - 1) * x3;
+            }
+        }
+    }
+}
+
+// CPMLBorderOneD::initCoefficients, src/cpml_border.cpp:179-244
+static void cpml1d_kappa(mpx_solver *s, int thickness, double kappaMax)
+{
+    for (int dim = 0; dim < 2; ++dim) {
+        for (double &v : s->kE[dim]) v = 1.0;
+        for (double &v : s->kH[dim]) v = 1.0;
+    }
+    std::vector<double> &KE = s->kE[2], &KH = s->kH[2];
+    int low = 0, high = s->g.n[2] + 1, glow = 0, ghigh = high;
+    if (low < glow + thickness) {
+        for (double &v : KE) v = 1.0;
+        for (double &v : KH) v = 1.0;
+        for (int i = 0; i < thickness; ++i) {
+            double x = 1 - double(i) / double(thickness);
+            double x3 = x * x * x;
+            if (low + i + 1 <= high) KE[low + i + 1] = 1 + (kappaMax - 1) * x3;
+        }
+        for (int i = 0; i < thickness; ++i) {
+            double x = 1 - (double(i) + 0.5) / double(thickness);
+            double x3 = x * x * x;
+            if (low + i <= high) KH[low + i] = 1 + (kappaMax - 1) * x3;
+        }
+    }
+    if (high > ghigh - thickness) {
+        for (int i = 0; i < thickness; ++i) {
+            double x = 1 - (double(i)) / double(thickness);
+            double x3 = x * x * x;
+            if (high - i - 1 >= low) KE[high - i - 1] = 1 + (kappaMax - 1) * x3;
+        }
+        for (int i = 0; i < thickness; ++i) {
+            double x = 1 - double(i + 0.5) / double(thickness);
+            double x3 = x * x * x;
+            if (high - i - 1 >= low) KH[high - i - 1] = 1 + (kappaMax - 1) * x3;
+        }
+    }
+}
+
+// CPMLBorderCurrent::makeCoeff, src/cpml_border.cpp:287-339
+static void cpml_coeff(const mpx_factory &f, int dir, bool isH, int lowk, int highk, double dt,
+                       std::vector<double> &b, std::vector<double> &c)
+{
+    bool reverse = !is_high(dir);
+    b.assign(highk - lowk + 1, 0.0);
+    c.assign(highk - lowk + 1, 0.0);
+    double offset = isH ? 0.5 : 0.0;
+    double eps = 1;  // CPMLBorder passes eps = 1, src/cpml_border.cpp:17-52
+    for (int k = 0; k < f.d; ++k) {
+        double x = 1 - (double(k) - offset) / double(f.d);
+        double x3 = x * x * x;
+        int pos = reverse ? (lowk + k) : (highk - k);
+        double sigma = x3 * f.sigma_max;
+        double kappa = 1 + (f.kappa_max - 1) * x3;
+        double a = f.a_max * (1 - x);
+        double bb = exp(-(sigma / kappa + a) * dt / eps);
+        double cc = sigma * (bb - 1) / (kappa * (sigma + kappa * a));
+        if (pos - lowk >= 0 && pos - lowk < (int)b.size()) {
+            b[pos - lowk] = bb;
+            c[pos - lowk] = cc;
+        }
+    }
+}
+
+// pole constants, src/fdtd_plrc.t:181-216
+static void pole_constants(const double LEps[3], const double LDelta[3], const double LOm2[3], double dt,
+                           std::complex<double> Crec[3], std::complex<double> dchi0[3],
+                           std::complex<double> dxi0[3], double &sumChi0, double &sumXi0)
+{
+    std::complex<double> I(0, 1), sumChi = 0.0, sumXi = 0.0;
+    for (int n = 0; n < 3; ++n) {
+        double beta = std::sqrt(LOm2[n] - LDelta[n] * LDelta[n]);
+        double gamma = LEps[n] * LOm2[n] / beta;
+        std::complex<double> phasor = LDelta[n] - I * beta;
+        Crec[n] = std::exp(-phasor * dt);
+        std::complex<double> chi0 = (I * gamma * (Crec[n] - 1.0) / phasor);
+        std::complex<double> xi0 = I * gamma * (Crec[n] * (phasor * dt + 1.0) - 1.0) / (phasor * phasor * dt);
+        dchi0[n] = chi0 * (1.0 - Crec[n]);
+        dxi0[n] = xi0 * (1.0 - Crec[n]);
+        sumChi += chi0;
+        sumXi += xi0;
+    }
+    sumChi0 = std::real(sumChi);
+    sumXi0 = std::real(sumXi);
+}
+
+namespace mpx {
+double pulse_norm_device(const PulseParams &P, const FaddeevaTables &ft, hipStream_t s);
+}
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+
+extern "C" {
+
+const char *mpx_last_error(void) { return g_err.c_str(); }
+int mpx_abi_version(void) { return MPX_ABI_VERSION; }
+
+void mpx_factory_defaults(mpx_factory *f, int kind)
+{
+    std::memset(f, 0, sizeof(*f));
+    f->kind = kind;
+    if (kind == MPX_FACTORY_CPML || kind == MPX_FACTORY_CPML_1D) {
+        f->d = 8;  // src/cpml_border.cpp:146-149
+        f->kappa_max = 15;
+        f->a_max = 0.25;
+        f->sigma_max = 3;
+    } else {
+        f->d = 15;  // src/incsource.cpp:84
+        f->length = 1.;  // src/shortpulseinject.cpp:38-47
+        f->width = 1.;
+        f->om0 = 2 * M_PI;
+        f->amp = 1;
+        f->eps = 1;
+        f->k[2] = 1;  // src/sources.cpp:62-80
+        f->ramp = 0.5;
+        f->gwidth = 10;  // src/sources.cpp:237-239
+        f->goffset = 40;
+        if (kind == MPX_FACTORY_PLANE_WAVE || kind == MPX_FACTORY_PLANE_GAUSS) f->amp = 0;
+    }
+}
+
+void mpx_desc_defaults(mpx_desc *d)
+{
+    std::memset(d, 0, sizeof(*d));
+    d->abi_version = MPX_ABI_VERSION;
+    d->nx = d->ny = d->nz = 128;  // src/globals.cpp:23-33
+    d->dx = d->dy = d->dz = 0.1;
+    d->dt = 1e-2;
+    d->solver = MPX_SOLVER_PLRC_NONLIN;
+    d->boundary = MPX_BOUNDARY_PERIODIC;
+    d->precision = MPX_PREC_FP64;
+    d->path = MPX_PATH_FUSED;
+    d->eps = 1.0;  // src/fdtd_plrc.t:417
+    d->chi = 0.1;  // src/fdtd_plrc.cpp:463
+    for (int n = 0; n < 3; ++n) d->l_om2[n] = 1.0;  // src/fdtd_plrc.t:427-429
+}
+
+static int build(mpx_solver *s, const mpx_desc *desc);
+
+int mpx_create(const mpx_desc *desc, mpx_solver **out)
+{
+    if (!desc || !out) return fail(MPX_ERR_INVALID, "null descriptor or output pointer");
+    *out = nullptr;
+    if (desc->abi_version != MPX_ABI_VERSION) return fail(MPX_ERR_INVALID, "descriptor ABI version mismatch");
+    if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return fail(MPX_ERR_INVALID, "grid sizes must be positive");
+    if (!(desc->dt > 0) || !(desc->dx > 0) || !(desc->dy > 0) || !(desc->dz > 0))
+        return fail(MPX_ERR_INVALID, "grid spacing and dt must be positive");
+    if (desc->solver < MPX_SOLVER_PLRC_NONLIN || desc->solver > MPX_SOLVER_DISP)
+        return fail(MPX_ERR_INVALID, "unknown solver");
+    if (desc->boundary != MPX_BOUNDARY_PERIODIC && desc->boundary != MPX_BOUNDARY_XY_PERIODIC)
+        return fail(MPX_ERR_INVALID, "unknown boundary");
+    if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK)
+        return fail(MPX_ERR_INVALID, "unknown precision");
+    if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE)
+        return fail(MPX_ERR_INVALID, "unknown kernel path");
+    if (desc->n_factories < 0 || desc->n_factories > MPX_MAX_FACTORIES)
+        return fail(MPX_ERR_INVALID, "too many current factories");
+    mpx_solver *s = new (std::nothrow) mpx_solver();
+    if (!s) return fail(MPX_ERR_NOMEM, "host allocation failed");
+    int rc = build(s, desc);
+    if (rc != MPX_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return MPX_OK;
+}
+
+int mpx_destroy(mpx_solver *s)
+{
+    delete s;
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+static int build(mpx_solver *s, const mpx_desc *desc)
+{
+    s->d = *desc;
+    s->solver = desc->solver;
+    s->prec = desc->precision;
+    s->path = desc->path;
+    s->plrc = (s->solver == MPX_SOLVER_PLRC_NONLIN || s->solver == MPX_SOLVER_PLRC_LIN);
+    s->nonlin = (s->solver == MPX_SOLVER_PLRC_NONLIN);
+    s->esz = (s->prec == MPX_PREC_FP32_BULK) ? 4 : 8;
+    s->delta[0] = desc->dx;
+    s->delta[1] = desc->dy;
+    s->delta[2] = desc->dz;
+    s->periodic[2] = (desc->boundary == MPX_BOUNDARY_PERIODIC) ? 1 : 0;
+
+    if (!s->plrc && desc->n_factories > 0)
+        // FieldSolver::addCurrent default, src/fieldsolver.h:15-23
+        return fail(MPX_ERR_INVALID, "Additional currents cannot be used when using this Field Solver");
+    if (!s->plrc && s->prec != MPX_PREC_FP64)
+        return fail(MPX_ERR_UNSUPPORTED, "fdtd-plain/nl/disp run in float64 only");
+    if (s->plrc && (desc->l_om2[0] == 0 || desc->l_om2[1] == 0 || desc->l_om2[2] == 0))
+        return fail(MPX_ERR_INVALID, "FDTD_PLRCSolver: Omega should not be zero!");  // src/fdtd_plrc.t:20-24
+    if (s->plrc && s->path == MPX_PATH_REFERENCE && s->prec != MPX_PREC_FP64)
+        return fail(MPX_ERR_UNSUPPORTED, "the reference-structured path runs in float64 only");
+    for (int n = 0; n < 3; ++n)
+        if (s->plrc && !(desc->l_om2[n] > desc->l_delta[n] * desc->l_delta[n]))
+            return fail(MPX_ERR_INVALID, "pole " + std::to_string(n + 1) + ": L*Om2 must exceed L*delta^2");
+
+    // storage geometry: rows of (nz + 2) padded so that k = 1 starts a 32-element segment
+    const int align = 32;
+    GridDims &g = s->g;
+    g.n[0] = desc->nx;
+    g.n[1] = desc->ny;
+    g.n[2] = desc->nz;
+    g.py = desc->ny + 2;
+    g.z0 = align - 1;
+    g.pz = align + ((desc->nz + 1 + align - 1) / align) * align;
+    g.sx = (long long)g.py * g.pz;
+    s->cells = (size_t)(desc->nx + 2) * (size_t)g.sx;
+
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(MPX_ERR_HIP, "hipStreamCreate failed");
+
+    // Faddeeva node tables: n = (k h)^2, e = exp(-n)
+    const double hs[2] = {0.5, 0.53};
+    for (int t = 0; t < 2; ++t)
+        for (int q = 0; q < 12; ++q) {
+            double kh = hs[t] * (double)(q + 1);
+            s->ft.n[t][q] = kh * kh;
+            s->ft.e[t][q] = std::exp(-s->ft.n[t][q]);
+        }
+
+    int rc = MPX_OK;
+    for (int c = 0; c < 3; ++c) {
+        if (s->esz == 8) {
+            s->E[c] = s->dalloc<double>(s->cells, rc);
+            s->B[c] = s->dalloc<double>(s->cells, rc);
+        } else {
+            s->E[c] = s->dalloc<float>(s->cells, rc);
+            s->B[c] = s->dalloc<float>(s->cells, rc);
+        }
+        if (rc) return rc;
+    }
+
+    if (!s->plrc) {
+        if (s->solver == MPX_SOLVER_DISP)
+            for (int n = 0; n < 3; ++n)
+                for (int c = 0; c < 3; ++c) {
+                    s->P[n][c] = s->dalloc<double>(s->cells, rc);
+                    s->Pp[n][c] = s->dalloc<double>(s->cells, rc);
+                    if (rc) return rc;
+                }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            s->kE[d].assign(g.n[d] + 2, 1.0);
+            s->kH[d].assign(g.n[d] + 2, 1.0);
+        }
+        // currentFactories->initCurrents in setup-file order, src/fdtd_plrc.t:39-45
+        int ncpml = 0;
+        for (int fi = 0; fi < desc->n_factories; ++fi) {
+            const mpx_factory &f = desc->factories[fi];
+            if (f.kind == MPX_FACTORY_CPML || f.kind == MPX_FACTORY_CPML_1D) {
+                if (++ncpml > 1) return fail(MPX_ERR_UNSUPPORTED, "at most one CPML factory is supported");
+                if (f.d < 1) return fail(MPX_ERR_INVALID, "CPML thickness d must be >= 1");
+                bool one_d = f.kind == MPX_FACTORY_CPML_1D;
+                const int faces1d[2] = {UP, DOWN};
+                int nf = one_d ? 2 : 6;
+                for (int h = 0; h < 2; ++h)
+                    for (int q = 0; q < nf; ++q) {
+                        Current c;
+                        c.type = h ? CUR_CPML_H : CUR_CPML_E;
+                        c.dir = one_d ? faces1d[q] : kFaceOrder[q];
+                        c.factory = fi;
+                        (h ? s->hcur : s->ecur).push_back(c);
+                    }
+                if (one_d)
+                    cpml1d_kappa(s, f.d, f.kappa_max);
+                else
+                    cpml_kappa(s, f.d, f.kappa_max);
+            } else if (f.kind == MPX_FACTORY_SHORT_PULSE || f.kind == MPX_FACTORY_PLANE_WAVE ||
+                       f.kind == MPX_FACTORY_PLANE_GAUSS) {
+                if (ncpml == 0) s->sheets_first = 1;
+                // needCurrent(down) only: src/shortpulseinject.cpp:28-31, src/sources.cpp:8-11
+                for (int h = 0; h < 2; ++h) {
+                    Current c;
+                    c.type = h ? CUR_INC_H : CUR_INC_E;
+                    c.dir = DOWN;
+                    c.factory = fi;
+                    (h ? s->hcur : s->ecur).push_back(c);
+                }
+            } else {
+                return fail(MPX_ERR_UNSUPPORTED, "unsupported current factory kind " + std::to_string(f.kind));
+            }
+        }
+
+        // coreInitStorage, src/fdtd_plrc.cpp:13-86: layers, coefficients, invalid currents dropped
+        auto init_side = [&](std::vector<Current> &list, bool isH) -> int {
+            std::vector<Current> kept;
+            for (Current c : list) {
+                const mpx_factory &f = desc->factories[c.factory];
+                if (c.type == CUR_CPML_E || c.type == CUR_CPML_H) {
+                    int distance = isH ? (is_high(c.dir) ? 1 : 0) : 1;
+                    const char *nm[3] = {isH ? "CPMLPsiBx" : "CPMLPsiEx", isH ? "CPMLPsiBy" : "CPMLPsiEy",
+                                         isH ? "CPMLPsiBz" : "CPMLPsiEz"};
+                    int l0 = add_border_layer(s, nm[0], c.dir, f.d, distance);
+                    int l1 = add_border_layer(s, nm[1], c.dir, f.d, distance);
+                    int l2 = add_border_layer(s, nm[2], c.dir, f.d, distance);
+                    if (l0 < 0 || l1 < 0 || l2 < 0) continue;  // CurrentInvalidPredicate
+                    c.layer = l0;
+                    int dim = dim_of(c.dir);
+                    CpmlFace F{};
+                    F.dim = dim;
+                    for (int d = 0; d < 3; ++d) {
+                        F.lo[d] = s->layers[l0].lo[d];
+                        F.hi[d] = s->layers[l0].hi[d];
+                    }
+                    F.n1 = F.hi[1] - F.lo[1] + 1;
+                    F.n2 = F.hi[2] - F.lo[2] + 1;
+                    // face map, src/cpml_border.cpp:357-386 / 456-485
+                    if (dim == 0) { F.c0 = 1; F.c1 = 2; F.f0 = 1; F.f1 = 2; }
+                    else if (dim == 1) { F.c0 = 2; F.c1 = 0; F.f0 = 2; F.f1 = 0; }
+                    else { F.c0 = 0; F.c1 = 1; F.f0 = 0; F.f1 = 1; }
+                    F.dxn = s->delta[dim];
+                    size_t cnt = s->layers[l0].count();
+                    int lc[3] = {l0, l1, l2};
+                    for (int w = 0; w < 2; ++w) {
+                        void *p = (s->esz == 8) ? (void *)s->dalloc<double>(cnt, rc) : (void *)s->dalloc<float>(cnt, rc);
+                        if (rc) return rc;
+                        int comp = w == 0 ? F.c0 : F.c1;
+                        s->layers[lc[comp]].comp[0] = p;  // each component layer owns one array
+                        if (w == 0) F.psi0 = p; else F.psi1 = p;
+                    }
+                    std::vector<double> bb, cc;
+                    cpml_coeff(f, c.dir, isH, F.lo[dim], F.hi[dim], desc->dt, bb, cc);
+                    double *db = s->dalloc<double>(bb.size(), rc);
+                    double *dc = s->dalloc<double>(cc.size(), rc);
+                    if (rc) return rc;
+                    if (upload(s, db, bb.data(), bb.size() * 8) || upload(s, dc, cc.data(), cc.size() * 8))
+                        return fail(MPX_ERR_HIP, "coefficient upload failed");
+                    F.b = db;
+                    F.c = dc;
+                    std::vector<CpmlFace> &faces = isH ? s->hfaces : s->efaces;
+                    c.face = (int)faces.size();
+                    faces.push_back(F);
+                } else {
+                    // IncidentSource{E,H}Current::initStorage, src/incsource.t:15-31, 87-103
+                    int distance = f.d;
+                    if (isH && !is_high(c.dir)) distance -= 1;
+                    const char *nm[3] = {isH ? "IncidentHJx" : "IncidentEJx", isH ? "IncidentHJy" : "IncidentEJy",
+                                         isH ? "IncidentHJz" : "IncidentEJz"};
+                    int l0 = add_border_layer(s, nm[0], c.dir, 1, distance);
+                    int l1 = add_border_layer(s, nm[1], c.dir, 1, distance);
+                    int l2 = add_border_layer(s, nm[2], c.dir, 1, distance);
+                    if (l0 < 0 || l1 < 0 || l2 < 0) continue;
+                    c.layer = l0;
+                    int lc[3] = {l0, l1, l2};
+                    for (int w = 0; w < 3; ++w)
+                        if (!s->layers[lc[w]].comp[0]) {
+                            s->layers[lc[w]].comp[0] = s->dalloc<double>(s->layers[lc[w]].count(), rc);
+                            if (rc) return rc;
+                        }
+                    mpx::SheetEval &ev = c.ev;
+                    ev.isH = isH ? 1 : 0;
+                    // src/incsource.t:54 (E: offset -1) and :126 (H: +1 for east/north/up, else 0)
+                    ev.off = isH ? (is_high(c.dir) ? 1 : 0) : -1;
+                    ev.t1 = 0;  // down face: transverse1 = x, transverse2 = y, src/incsource.cpp:123-131
+                    ev.t2 = 1;
+                    ev.DX = desc->dz;
+                    ev.nx2 = g.n[0] + 2;
+                    ev.ny2 = g.n[1] + 2;
+                    ev.sheet.kp = s->layers[l0].lo[2];
+                    for (int w = 0; w < 3; ++w) ev.sheet.J[w] = (double *)s->layers[lc[w]].comp[0];
+                    if (f.kind == MPX_FACTORY_SHORT_PULSE) {
+                        ev.kind = 1;
+                        // ShortPulseInjectSourceFunc::setParam, src/shortpulseinject.cpp:56-113
+                        mpx::PulseParams &P = ev.pulse;
+                        P.DX = desc->dx;
+                        P.DY = desc->dy;
+                        P.DZ = desc->dz;
+                        P.DT = desc->dt;
+                        P.centrex = 0.5 * double(g.n[0] + 1 + 0);
+                        P.centrey = 0.5 * double(g.n[1] + 1 + 0);
+                        P.centrez = 0.5 * double(g.n[2] + 1 + 0);
+                        P.length = f.length;
+                        P.om0 = f.om0;
+                        if (!(f.eps > 0) || !(f.length != 0) || !(f.width > 0))
+                            return fail(MPX_ERR_INVALID, "short_pulse_inject: eps, width must be > 0 and length != 0");
+                        P.lightspeed = std::sqrt(1 / f.eps);
+                        P.ZRl = 0.5 * f.om0 * f.width * f.width / P.lightspeed;
+                        P.Phase = 0.0;
+                        P.TShift = 0.0;
+                        P.ZShift = 0.0;
+                        P.amp = 1.0;
+                        double Exmax = mpx::pulse_norm_device(P, s->ft, s->stream);
+                        P.amp = f.amp / Exmax;
+                        P.Phase = 2 * M_PI * f.phase;
+                        P.TShift = f.tshift;
+                        P.ZShift = f.zshift;
+                    } else {
+                        ev.kind = 2;
+                        // PlaneWaveSource/PlaneGaussSource::make*Current, src/sources.cpp:14-56, 185-223
+                        mpx::PlaneParams &W = ev.plane;
+                        W.gauss = f.kind == MPX_FACTORY_PLANE_GAUSS;
+                        double kx = f.k[0], ky = f.k[1], kz = f.k[2];
+                        double Hx = f.h[0], Hy = f.h[1], Hz = f.h[2];
+                        double Ev[3] = {ky * Hz - kz * Hy, kz * Hx - kx * Hz, kx * Hy - ky * Hx};
+                        double bmag = std::sqrt(Hx * Hx + Hy * Hy + Hz * Hz);
+                        double factor = -bmag / std::sqrt(Ev[0] * Ev[0] + Ev[1] * Ev[1] + Ev[2] * Ev[2]);
+                        for (int w = 0; w < 3; ++w) {
+                            Ev[w] *= factor / std::sqrt(f.eps);
+                            W.k[w] = f.k[w];
+                            W.H[w] = f.h[w];
+                            W.E[w] = Ev[w];
+                            W.H_bg[w] = W.gauss ? 0 : f.h_bg[w];
+                            W.E_bg[w] = W.gauss ? 0 : f.e_bg[w];
+                        }
+                        W.ramp = f.ramp;
+                        W.dt = desc->dt;
+                        W.om = std::sqrt(W.k[0] * W.k[0] + W.k[1] * W.k[1] + W.k[2] * W.k[2]) / std::sqrt(f.eps);
+                        W.dx = desc->dx;
+                        W.dy = desc->dy;
+                        W.dz = desc->dz;
+                        W.width = f.gwidth;
+                        W.offset = f.goffset;
+                    }
+                }
+                kept.push_back(c);
+            }
+            list.swap(kept);
+            return MPX_OK;
+        };
+        if ((rc = init_side(s->ecur, false)) != MPX_OK) return rc;
+        if ((rc = init_side(s->hcur, true)) != MPX_OK) return rc;
+        if ((int)s->efaces.size() > mpx::kMaxFaces || (int)s->hfaces.size() > mpx::kMaxFaces)
+            return fail(MPX_ERR_UNSUPPORTED, "too many CPML faces");
+        int nse = 0, nsh = 0;
+        for (auto &c : s->ecur) nse += c.type == CUR_INC_E;
+        for (auto &c : s->hcur) nsh += c.type == CUR_INC_H;
+        if (nse > mpx::kMaxSheets || nsh > mpx::kMaxSheets) return fail(MPX_ERR_UNSUPPORTED, "too many source sheets");
+        s->haveJ = !s->ecur.empty();
+        s->haveM = !s->hcur.empty();
+
+        // kappa*delta lines, as plrcStepD/B form them per cell (src/fdtd_plrc.cpp:313-315, 435-437)
+        for (int d = 0; d < 3; ++d) {
+            std::vector<double> kdE(g.n[d] + 2), kdH(g.n[d] + 2);
+            for (int q = 0; q < g.n[d] + 2; ++q) {
+                kdE[q] = s->kE[d][q] * s->delta[d];
+                kdH[q] = s->kH[d][q] * s->delta[d];
+            }
+            s->d_kdE[d] = s->dalloc<double>(kdE.size(), rc);
+            s->d_kdH[d] = s->dalloc<double>(kdH.size(), rc);
+            if (rc) return rc;
+            if (upload(s, s->d_kdE[d], kdE.data(), kdE.size() * 8) || upload(s, s->d_kdH[d], kdH.data(), kdH.size() * 8))
+                return fail(MPX_ERR_HIP, "kappa upload failed");
+        }
+
+        // active poles: Delta eps = 0 gives gamma = 0, so the recursion stays identically zero
+        std::complex<double> Crec[3], dchi0[3], dxi0[3];
+        pole_constants(desc->l_eps, desc->l_delta, desc->l_om2, desc->dt, Crec, dchi0, dxi0, s->sumChi0, s->sumXi0);
+        s->np = 0;
+        for (int n = 0; n < 3; ++n)
+            if (desc->l_eps[n] != 0.0) {
+                int a = s->np++;
+                s->pole_ref[a] = n;
+                std::complex<double> D = dchi0[n] - dxi0[n];
+                s->pc.dd[a] = make_double2(D.real(), D.imag());
+                s->pc.dxi0[a] = make_double2(dxi0[n].real(), dxi0[n].imag());
+                s->pc.crec[a] = make_double2(Crec[n].real(), Crec[n].imag());
+                for (int c = 0; c < 3; ++c) {
+                    s->psi[a][c] = s->dalloc<double2>(s->cells, rc);
+                    if (rc) return rc;
+                }
+            }
+        double sigma = 0.5 * 0.0 * desc->dt;  // no OptField writes Sigma on this path
+        if (s->nonlin)
+            s->denom = desc->eps + s->sumChi0 - s->sumXi0 + sigma;
+        else
+            s->denom = desc->eps + (s->sumChi0 - s->sumXi0) + sigma;
+        s->numer = desc->eps - s->sumXi0 - sigma;
+
+        if (s->path == MPX_PATH_REFERENCE) {
+            for (int c = 0; c < 3; ++c) {
+                if (s->haveJ) s->Jt[c] = s->dalloc<double>(s->cells, rc);
+                if (s->haveM) s->Mt[c] = s->dalloc<double>(s->cells, rc);
+                if (rc) return rc;
+            }
+        }
+    }
+    s->d_part = s->dalloc<double>(mpx::energy_partials(g), rc);
+    s->d_energy = s->dalloc<double>(1, rc);
+    if (rc) return rc;
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    s->time = 0;
+    return MPX_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------------------
+
+template <typename T>
+static StepArgs<T> step_args(mpx_solver *s, bool hside)
+{
+    StepArgs<T> a{};
+    a.g = s->g;
+    for (int d = 0; d < 3; ++d) {
+        a.periodic[d] = s->periodic[d];
+        a.kdE[d] = s->d_kdE[d];
+        a.kdH[d] = s->d_kdH[d];
+        a.E[d] = (T *)s->E[d];
+        a.B[d] = (T *)s->B[d];
+        a.blo[d] = s->periodic[d] ? 1 : 0;
+    }
+    for (int n = 0; n < 3; ++n)
+        for (int c = 0; c < 3; ++c) a.psi[n][c] = s->psi[n][c];
+    a.dt = s->d.dt;
+    a.eps = s->d.eps;
+    a.chi = s->d.chi;
+    a.denom = s->denom;
+    a.numer = s->numer;
+    a.pc = s->pc;
+    const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
+    a.nfaces = (int)faces.size();
+    for (int d = 0; d < 3; ++d) a.face_by_dim[d][0] = a.face_by_dim[d][1] = -1;
+    const std::vector<Current> &list = hside ? s->hcur : s->ecur;
+    for (size_t q = 0; q < faces.size(); ++q) a.face[q] = faces[q];
+    for (const Current &c : list)
+        if (c.face >= 0) {
+            int dim = dim_of(c.dir);
+            a.face_by_dim[dim][is_high(c.dir) ? 0 : 1] = c.face;
+        }
+    a.nsheets = 0;
+    for (const Current &c : list)
+        if (c.type == CUR_INC_E || c.type == CUR_INC_H) a.sheet[a.nsheets++] = c.ev.sheet;
+    a.sheets_first = s->sheets_first;
+    a.haveJ = 0;
+    return a;
+}
+
+static mpx::SumList sum_list(mpx_solver *s, bool hside)
+{
+    mpx::SumList L{};
+    const std::vector<Current> &list = hside ? s->hcur : s->ecur;
+    const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
+    for (const Current &c : list) {
+        mpx::SumView &v = L.v[L.n++];
+        const Layer &Ly = s->layers[c.layer];
+        for (int d = 0; d < 3; ++d) {
+            v.lo[d] = Ly.lo[d];
+            v.hi[d] = Ly.hi[d];
+            v.comp[d] = nullptr;
+        }
+        v.n1 = v.hi[1] - v.lo[1] + 1;
+        v.n2 = v.hi[2] - v.lo[2] + 1;
+        if (c.face >= 0) {
+            const CpmlFace &F = faces[c.face];
+            v.comp[F.c0] = (const double *)F.psi0;
+            v.comp[F.c1] = (const double *)F.psi1;
+        } else {
+            for (int d = 0; d < 3; ++d) v.comp[d] = c.ev.sheet.J[d];
+        }
+    }
+    return L;
+}
+
+static int eval_sheets(mpx_solver *s, std::vector<Current> &list, int type)
+{
+    for (Current &c : list)
+        if (c.type == type) {
+            c.ev.time = s->time;
+            s->tick(K_SHEET);
+            mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
+            s->tock();
+            MPX_LAUNCHED("sheet evaluation");
+        }
+    return MPX_OK;
+}
+
+template <typename T>
+static int wrap_group(mpx_solver *s, void *const f[3], int cls)
+{
+    T *ff[3] = {(T *)f[0], (T *)f[1], (T *)f[2]};
+    for (int dim = 0; dim < 3; ++dim) {
+        if (!s->periodic[dim]) continue;  // SingleXYPeriodicBoundary::exchangeZ is empty
+        s->tick(cls);
+        mpx::launch_wrap<T>(s->g, ff, dim, s->stream);
+        s->tock();
+        MPX_LAUNCHED("ghost wrap");
+    }
+    return MPX_OK;
+}
+
+static int record_probes(mpx_solver *s)
+{
+    if (s->nprobe == 0 || s->probe_rec >= s->probe_cap) return MPX_OK;
+    double *row = s->d_probe_hist + (size_t)s->probe_rec * s->nprobe;
+    s->tick(K_PROBE);
+    if (s->esz == 8) {
+        mpx::ProbeArgs<double> p{};
+        p.n = s->nprobe;
+        for (int c = 0; c < 3; ++c) {
+            p.f[c] = (const double *)s->E[c];
+            p.f[3 + c] = (const double *)s->B[c];
+        }
+        p.comp = s->d_probe_comp;
+        p.idx = s->d_probe_idx;
+        mpx::launch_probe<double>(p, row, s->stream);
+    } else {
+        mpx::ProbeArgs<float> p{};
+        p.n = s->nprobe;
+        for (int c = 0; c < 3; ++c) {
+            p.f[c] = (const float *)s->E[c];
+            p.f[3 + c] = (const float *)s->B[c];
+        }
+        p.comp = s->d_probe_comp;
+        p.idx = s->d_probe_idx;
+        mpx::launch_probe<float>(p, row, s->stream);
+    }
+    s->tock();
+    MPX_LAUNCHED("probe");
+    s->probe_rec++;
+    return MPX_OK;
+}
+
+static mpx::SimpleArgs simple_args(mpx_solver *s)
+{
+    mpx::SimpleArgs a{};
+    a.g = s->g;
+    a.kind = s->solver == MPX_SOLVER_PLAIN ? 0 : (s->solver == MPX_SOLVER_NL ? 1 : 2);
+    for (int c = 0; c < 3; ++c) {
+        a.E[c] = (double *)s->E[c];
+        a.B[c] = (double *)s->B[c];
+        for (int n = 0; n < 3; ++n) {
+            a.P[n][c] = s->P[n][c];
+            a.Pp[n][c] = s->Pp[n][c];
+        }
+    }
+    a.dx = s->d.dx;
+    a.dy = s->d.dy;
+    a.dz = s->d.dz;
+    a.dt = s->d.dt;
+    a.eps = s->d.eps;
+    a.chi = s->d.chi;
+    for (int n = 0; n < 3; ++n) {
+        a.eps2[n] = s->d.d_eps2[n];
+        a.om2[n] = s->d.d_om2[n];
+    }
+    return a;
+}
+
+// one full step: FDTD_PLRCSolver::stepScheme (src/fdtd_plrc.t:127-168) or the plain/nl/disp
+// stepScheme (src/fdtd_plain.cpp:31-35)
+static int step_once(mpx_solver *s)
+{
+    int rc;
+    if (!s->plrc) {
+        mpx::SimpleArgs a = simple_args(s);
+        s->tick(K_SIMPLE_E);
+        mpx::launch_simple_stepD(a, s->stream);
+        s->tock();
+        MPX_LAUNCHED("simple stepD");
+        if ((rc = wrap_group<double>(s, s->E, K_WRAP_E))) return rc;
+        s->tick(K_SIMPLE_B);
+        mpx::launch_simple_stepB(a, s->stream);
+        s->tock();
+        MPX_LAUNCHED("simple stepB");
+        if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
+    } else if (s->path == MPX_PATH_REFERENCE) {
+        StepArgs<double> ae = step_args<double>(s, false);
+        // E currents in list order
+        for (Current &c : s->ecur) {
+            if (c.type == CUR_CPML_E) {
+                s->tick(K_CPML_E);
+                mpx::launch_ref_cpml(ae, c.face, false, s->stream);
+                s->tock();
+                MPX_LAUNCHED("cpml E");
+            } else {
+                c.ev.time = s->time;
+                s->tick(K_SHEET);
+                mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
+                s->tock();
+                MPX_LAUNCHED("sheet E");
+            }
+        }
+        if (s->haveJ) {
+            mpx::SumList L = sum_list(s, false);
+            s->tick(K_SUM_J);
+            mpx::launch_ref_sum(s->g, L, s->Jt, s->stream);
+            s->tock();
+            MPX_LAUNCHED("sum J");
+            ae.haveJ = 1;
+            for (int c = 0; c < 3; ++c) ae.Jt[c] = s->Jt[c];
+        }
+        s->tick(K_STEPD_REF);
+        mpx::launch_ref_stepD(ae, s->np, s->nonlin, s->stream);
+        s->tock();
+        MPX_LAUNCHED("stepD");
+        if ((rc = wrap_group<double>(s, s->E, K_WRAP_E))) return rc;
+        StepArgs<double> ah = step_args<double>(s, true);
+        for (Current &c : s->hcur) {
+            if (c.type == CUR_CPML_H) {
+                s->tick(K_CPML_H);
+                mpx::launch_ref_cpml(ah, c.face, true, s->stream);
+                s->tock();
+                MPX_LAUNCHED("cpml H");
+            } else {
+                c.ev.time = s->time;
+                s->tick(K_SHEET);
+                mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
+                s->tock();
+                MPX_LAUNCHED("sheet H");
+            }
+        }
+        if (s->haveM) {
+            mpx::SumList L = sum_list(s, true);
+            s->tick(K_SUM_M);
+            mpx::launch_ref_sum(s->g, L, s->Mt, s->stream);
+            s->tock();
+            MPX_LAUNCHED("sum M");
+            ah.haveJ = 1;
+            for (int c = 0; c < 3; ++c) ah.Jt[c] = s->Mt[c];
+        }
+        s->tick(K_STEPB_REF);
+        mpx::launch_ref_stepB(ah, s->stream);
+        s->tock();
+        MPX_LAUNCHED("stepB");
+        if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
+    } else {
+        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
+        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
+        if (s->esz == 8) {
+            StepArgs<double> ae = step_args<double>(s, false);
+            s->tick(K_FUSED_E);
+            mpx::launch_fused_E<double>(ae, s->np, s->nonlin, s->stream);
+            s->tock();
+            MPX_LAUNCHED("fused E");
+            StepArgs<double> ah = step_args<double>(s, true);
+            s->tick(K_FUSED_B);
+            mpx::launch_fused_B<double>(ah, s->stream);
+            s->tock();
+            MPX_LAUNCHED("fused B");
+        } else {
+            StepArgs<float> ae = step_args<float>(s, false);
+            s->tick(K_FUSED_E);
+            mpx::launch_fused_E<float>(ae, s->np, s->nonlin, s->stream);
+            s->tock();
+            MPX_LAUNCHED("fused E");
+            StepArgs<float> ah = step_args<float>(s, true);
+            s->tick(K_FUSED_B);
+            mpx::launch_fused_B<float>(ah, s->stream);
+            s->tock();
+            MPX_LAUNCHED("fused B");
+        }
+    }
+    s->time++;
+    return record_probes(s);
+}
+
+// ghost planes of E and B as the reference stores them (the fused path wraps by index)
+static int materialise_ghosts(mpx_solver *s)
+{
+    if (s->plrc && s->path == MPX_PATH_FUSED) {
+        int rc;
+        if (s->esz == 8) {
+            if ((rc = wrap_group<double>(s, s->E, K_WRAP_E))) return rc;
+            if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
+        } else {
+            if ((rc = wrap_group<float>(s, s->E, K_WRAP_E))) return rc;
+            if ((rc = wrap_group<float>(s, s->B, K_WRAP_B))) return rc;
+        }
+    }
+    return MPX_OK;
+}
+
+extern "C" {
+
+int mpx_step(mpx_solver *s, int nsteps)
+{
+    if (!s || nsteps < 0) return fail(MPX_ERR_INVALID, "invalid solver or step count");
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = step_once(s);
+        if (rc) return rc;
+    }
+    return MPX_OK;
+}
+
+// FieldSolver::stepSchemeInit: src/fdtd_plrc.t:90-124 (PLRC), src/fdtd_plain.cpp:26-29 (others)
+int mpx_step_init(mpx_solver *s)
+{
+    if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    if (s->esz != 8) return fail(MPX_ERR_UNSUPPORTED, "stepSchemeInit is provided in float64 only");
+    int rc;
+    if (!s->plrc) {
+        mpx::SimpleArgs a = simple_args(s);
+        a.dt = 0.5 * s->d.dt;
+        mpx::launch_simple_stepB(a, s->stream);
+        MPX_LAUNCHED("simple stepB");
+        return wrap_group<double>(s, s->B, K_WRAP_B);
+    }
+    if ((rc = materialise_ghosts(s))) return rc;
+    StepArgs<double> ae = step_args<double>(s, false);
+    mpx::launch_init_accumulator(ae, s->np, s->stream);
+    MPX_LAUNCHED("initAccumulator");
+    // stepB(0.5*dt) with the magnetic currents as they stand
+    StepArgs<double> ah = step_args<double>(s, true);
+    ah.dt = 0.5 * s->d.dt;
+    if (s->haveM) {
+        for (int c = 0; c < 3; ++c)
+            if (!s->Mt[c]) {
+                s->Mt[c] = s->dalloc<double>(s->cells, rc);
+                if (rc) return rc;
+            }
+        mpx::SumList L = sum_list(s, true);
+        mpx::launch_ref_sum(s->g, L, s->Mt, s->stream);
+        MPX_LAUNCHED("sum M");
+        ah.haveJ = 1;
+        for (int c = 0; c < 3; ++c) ah.Jt[c] = s->Mt[c];
+    }
+    mpx::launch_ref_stepB(ah, s->stream);
+    MPX_LAUNCHED("stepB");
+    if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
+    // currents' stepSchemeInit: CPML E is empty, incident E/H and CPML H run one stepScheme
+    if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
+    for (Current &c : s->hcur) {
+        if (c.type == CUR_CPML_H) {
+            mpx::launch_ref_cpml(ah, c.face, true, s->stream);
+            MPX_LAUNCHED("cpml H");
+        } else {
+            c.ev.time = s->time;
+            mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
+            MPX_LAUNCHED("sheet H");
+        }
+    }
+    return MPX_OK;
+}
+
+int mpx_get_time(mpx_solver *s, int *t)
+{
+    if (!s || !t) return fail(MPX_ERR_INVALID, "null argument");
+    *t = s->time;
+    return MPX_OK;
+}
+
+int mpx_set_time(mpx_solver *s, int t)
+{
+    if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    s->time = t;
+    return MPX_OK;
+}
+
+int mpx_sync(mpx_solver *s)
+{
+    if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+// resolve a full-storage grid name; *elem = element size; nullptr + MPX_OK means "all zero"
+static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, bool *isPsi, int *psiPart)
+{
+    *ptr = nullptr;
+    *elem = s->esz;
+    *isPsi = false;
+    static const char *eb[6] = {"Ex", "Ey", "Ez", "Bx", "By", "Bz"};
+    for (int q = 0; q < 6; ++q)
+        if (!std::strcmp(name, eb[q])) {
+            *ptr = q < 3 ? s->E[q] : s->B[q - 3];
+            return MPX_OK;
+        }
+    const char comp[3] = {'x', 'y', 'z'};
+    for (int n = 0; n < 3; ++n)
+        for (int c = 0; c < 3; ++c) {
+            char nr[16], ni[16], np_[16], npp[16];
+            std::snprintf(nr, sizeof nr, "PsiR%d%c", n + 1, comp[c]);
+            std::snprintf(ni, sizeof ni, "PsiI%d%c", n + 1, comp[c]);
+            std::snprintf(np_, sizeof np_, "P%d%c", n + 1, comp[c]);
+            std::snprintf(npp, sizeof npp, "P%d%cp", n + 1, comp[c]);
+            bool r = !std::strcmp(name, nr), i = !std::strcmp(name, ni);
+            if (r || i) {
+                if (!s->plrc) return MPX_ERR_NAME;
+                *isPsi = true;
+                *psiPart = r ? 0 : 1;
+                *elem = 8;
+                for (int a = 0; a < s->np; ++a)
+                    if (s->pole_ref[a] == n) *ptr = s->psi[a][c];
+                return MPX_OK;
+            }
+            if (!std::strcmp(name, np_) || !std::strcmp(name, npp)) {
+                if (s->solver != MPX_SOLVER_DISP) return MPX_ERR_NAME;
+                *elem = 8;
+                *ptr = !std::strcmp(name, np_) ? s->P[n][c] : s->Pp[n][c];
+                return MPX_OK;
+            }
+        }
+    static const char *jm[6] = {"JxTotal", "JyTotal", "JzTotal", "MxTotal", "MyTotal", "MzTotal"};
+    for (int q = 0; q < 6; ++q)
+        if (!std::strcmp(name, jm[q])) {
+            double *p = q < 3 ? s->Jt[q] : s->Mt[q - 3];
+            if (!p) return MPX_ERR_NAME;
+            *elem = 8;
+            *ptr = p;
+            return MPX_OK;
+        }
+    if (!std::strcmp(name, "Sigma") && s->plrc) {
+        *elem = 8;
+        return MPX_OK;  // always zero: no OptField on this path
+    }
+    return MPX_ERR_NAME;
+}
+
+extern "C" {
+
+int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
+{
+    if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
+    const GridDims &g = s->g;
+    size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+    if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
+    void *ptr;
+    size_t elem;
+    bool isPsi;
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
+    if (rc) return fail(rc, std::string("unknown grid ") + name);
+    if (!ptr) {
+        std::memset(host, 0, count * sizeof(double));
+        return MPX_OK;
+    }
+    if ((rc = materialise_ghosts(s))) return rc;
+    size_t stride = isPsi ? 16 : elem;
+    std::vector<unsigned char> buf(s->cells * stride);
+    MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < nx2; ++i)
+        for (size_t j = 0; j < ny2; ++j)
+            for (size_t k = 0; k < nz2; ++k) {
+                size_t src = g.at((int)i, (int)j, (int)k);
+                size_t dst = (i * ny2 + j) * nz2 + k;
+                if (isPsi)
+                    host[dst] = ((const double *)buf.data())[2 * src + part];
+                else if (elem == 8)
+                    host[dst] = ((const double *)buf.data())[src];
+                else
+                    host[dst] = ((const float *)buf.data())[src];
+            }
+    return MPX_OK;
+}
+
+int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count)
+{
+    if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
+    const GridDims &g = s->g;
+    size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+    if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
+    void *ptr;
+    size_t elem;
+    bool isPsi;
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
+    if (rc) return fail(rc, std::string("unknown grid ") + name);
+    if (!ptr) return fail(MPX_ERR_UNSUPPORTED, std::string("grid ") + name + " is not stored (inactive pole or Sigma)");
+    size_t stride = isPsi ? 16 : elem;
+    std::vector<unsigned char> buf(s->cells * stride);
+    MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < nx2; ++i)
+        for (size_t j = 0; j < ny2; ++j)
+            for (size_t k = 0; k < nz2; ++k) {
+                size_t dst = g.at((int)i, (int)j, (int)k);
+                size_t src = (i * ny2 + j) * nz2 + k;
+                if (isPsi)
+                    ((double *)buf.data())[2 * dst + part] = host[src];
+                else if (elem == 8)
+                    ((double *)buf.data())[dst] = host[src];
+                else
+                    ((float *)buf.data())[dst] = (float)host[src];
+            }
+    MPX_HIP(hipMemcpyAsync(ptr, buf.data(), buf.size(), hipMemcpyHostToDevice, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    return MPX_OK;
+}
+
+int mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_t count, int lo[3], int hi[3])
+{
+    if (!s || !name) return fail(MPX_ERR_INVALID, "null argument");
+    for (const Layer &L : s->layers)
+        if (L.dir == dir && L.name == name) {
+            if (lo && hi)
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = L.lo[d];
+                    hi[d] = L.hi[d];
+                }
+            if (!host) return MPX_OK;
+            if (count != L.count()) return fail(MPX_ERR_INVALID, "layer buffer size mismatch");
+            if (!L.comp[0]) {
+                std::memset(host, 0, count * sizeof(double));
+                return MPX_OK;
+            }
+            bool sheet = L.name.rfind("Incident", 0) == 0;
+            size_t elem = sheet ? 8 : s->esz;
+            std::vector<unsigned char> buf(count * elem);
+            MPX_HIP(hipMemcpyAsync(buf.data(), L.comp[0], buf.size(), hipMemcpyDeviceToHost, s->stream));
+            MPX_HIP(hipStreamSynchronize(s->stream));
+            for (size_t q = 0; q < count; ++q)
+                host[q] = elem == 8 ? ((const double *)buf.data())[q] : ((const float *)buf.data())[q];
+            return MPX_OK;
+        }
+    return fail(MPX_ERR_NAME, std::string("no layer ") + name + " on direction " + std::to_string(dir));
+}
+
+int mpx_set_probes(mpx_solver *s, int nprobes, const int *ijk, const int *comp, int capacity)
+{
+    if (!s || nprobes < 0 || capacity < 0 || (nprobes > 0 && (!ijk || !comp)))
+        return fail(MPX_ERR_INVALID, "invalid probe arguments");
+    std::vector<size_t> idx(nprobes);
+    for (int p = 0; p < nprobes; ++p) {
+        for (int d = 0; d < 3; ++d)
+            if (ijk[3 * p + d] < 1 || ijk[3 * p + d] > s->g.n[d])
+                return fail(MPX_ERR_INVALID, "probe cell outside the interior 1..N");
+        if (comp[p] < 0 || comp[p] > 5) return fail(MPX_ERR_INVALID, "probe component must be 0..5");
+        idx[p] = s->g.at(ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2]);
+    }
+    int rc = MPX_OK;
+    s->nprobe = 0;
+    s->probe_rec = 0;
+    s->probe_cap = capacity;
+    if (nprobes == 0) return MPX_OK;
+    s->d_probe_comp = s->dalloc<int>(nprobes, rc);
+    s->d_probe_idx = s->dalloc<size_t>(nprobes, rc);
+    s->d_probe_hist = s->dalloc<double>((size_t)nprobes * (capacity > 0 ? capacity : 1), rc);
+    if (rc) return rc;
+    if (upload(s, s->d_probe_comp, comp, nprobes * sizeof(int)) ||
+        upload(s, s->d_probe_idx, idx.data(), nprobes * sizeof(size_t)))
+        return fail(MPX_ERR_HIP, "probe upload failed");
+    s->nprobe = nprobes;
+    return MPX_OK;
+}
+
+int mpx_get_probes(mpx_solver *s, double *host, int *nrecorded)
+{
+    if (!s || !nrecorded) return fail(MPX_ERR_INVALID, "null argument");
+    *nrecorded = s->probe_rec;
+    if (host && s->nprobe && s->probe_rec) {
+        MPX_HIP(hipMemcpyAsync(host, s->d_probe_hist, (size_t)s->probe_rec * s->nprobe * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream));
+        MPX_HIP(hipStreamSynchronize(s->stream));
+    }
+    return MPX_OK;
+}
+
+int mpx_energy(mpx_solver *s, double *energy)
+{
+    if (!s || !energy) return fail(MPX_ERR_INVALID, "null argument");
+    if (s->esz == 8) {
+        double *e[3] = {(double *)s->E[0], (double *)s->E[1], (double *)s->E[2]};
+        double *b[3] = {(double *)s->B[0], (double *)s->B[1], (double *)s->B[2]};
+        mpx::launch_energy<double>(s->g, e, b, s->d_part, s->d_energy, s->stream);
+    } else {
+        float *e[3] = {(float *)s->E[0], (float *)s->E[1], (float *)s->E[2]};
+        float *b[3] = {(float *)s->B[0], (float *)s->B[1], (float *)s->B[2]};
+        mpx::launch_energy<float>(s->g, e, b, s->d_part, s->d_energy, s->stream);
+    }
+    MPX_LAUNCHED("energy");
+    double v = 0;
+    MPX_HIP(hipMemcpyAsync(&v, s->d_energy, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    *energy = v * (s->d.dx * s->d.dy * s->d.dz);
+    return MPX_OK;
+}
+
+int mpx_time_steps(mpx_solver *s, int nsteps, mpx_timing *out)
+{
+    if (!s || !out || nsteps < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
+    std::memset(out, 0, sizeof(*out));
+    hipEvent_t t0, t1;
+    MPX_HIP(hipEventCreate(&t0));
+    MPX_HIP(hipEventCreate(&t1));
+    s->timer.on = true;
+    s->timer.used = 0;
+    s->timer.cls.clear();
+    s->timer.ev0.clear();
+    s->timer.ev1.clear();
+    (void)hipEventRecord(t0, s->stream);
+    int rc = mpx_step(s, nsteps);
+    (void)hipEventRecord(t1, s->stream);
+    s->timer.on = false;
+    if (rc == MPX_OK) {
+        if (hipEventSynchronize(t1) != hipSuccess) rc = fail(MPX_ERR_HIP, "event sync failed");
+    }
+    if (rc == MPX_OK) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        out->step_ms = ms / nsteps;
+        double sum[K_NCLASSES] = {0};
+        int cnt[K_NCLASSES] = {0};
+        for (size_t q = 0; q < s->timer.cls.size(); ++q) {
+            float m = 0;
+            (void)hipEventElapsedTime(&m, s->timer.ev0[q], s->timer.ev1[q]);
+            sum[s->timer.cls[q]] += m;
+            cnt[s->timer.cls[q]]++;
+        }
+        for (int c = 0; c < K_NCLASSES; ++c)
+            if (cnt[c]) {
+                int q = out->n_classes++;
+                std::snprintf(out->name[q], sizeof out->name[q], "%s", kClassNames[c]);
+                out->mean_ms[q] = sum[c] / cnt[c];
+                out->launches[q] = cnt[c];
+            }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return rc;
+}
+
+int mpx_state_bytes(mpx_solver *s, size_t *bytes)
+{
+    if (!s || !bytes) return fail(MPX_ERR_INVALID, "null argument");
+    size_t b = 0;
+    for (void *p : s->allocs) (void)p;
+    b += 6 * s->cells * s->esz;
+    b += (size_t)s->np * 3 * s->cells * 16;
+    *bytes = b;
+    return MPX_OK;
+}
+
+}  // extern "C"

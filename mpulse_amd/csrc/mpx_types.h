@@ -1,0 +1,111 @@
+// Plain-old-data descriptors shared by the host solver and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "mpx_math.h"
+
+namespace mpx {
+
+constexpr int kMaxSheets = 8;   // incident-source current sheets per side
+constexpr int kMaxFaces = 6;    // CPML face currents per side
+constexpr int kNewtonCap = 200; // iteration bound of the device Kerr Newton loops
+
+// Storage of one (nx+2) x (ny+2) x (nz+2) grid in HBM: z fastest, the z row padded to pz
+// elements and shifted by z0 so that interior cell k = 1 starts an aligned segment.
+struct GridDims {
+    int n[3];      // interior cells per dimension
+    int py;        // ny + 2
+    int pz;        // padded z row
+    int z0;        // element offset of k = 0 inside a row
+    long long sx;  // py * pz
+    __host__ __device__ __forceinline__ size_t at(int i, int j, int k) const
+    {
+        return (size_t)i * (size_t)sx + (size_t)j * (size_t)pz + (size_t)(k + z0);
+    }
+};
+
+// One CPML border current (a face of CPMLBorder{E,H}Current): its layer box, its two live
+// auxiliaries, and the b/c coefficient lines indexed by the normal coordinate.
+struct CpmlFace {
+    int dim;          // normal dimension
+    int lo[3], hi[3]; // layer box (inclusive), as Storage::getBorderExtent builds it
+    int n1, n2;       // box extents of dims 1 and 2
+    void *psi0;       // component psi0 (see the face map of CPMLBorder*Current::initStorage)
+    void *psi1;
+    int c0, c1;       // current components fed by psi0 / psi1
+    int f0, f1;       // field components read for psi0 (f1 difference) / psi1 (f0 difference)
+    const double *b;  // [normal - lo[dim]]
+    const double *c;
+    double dxn;       // grid spacing of the normal dimension
+    __host__ __device__ __forceinline__ size_t at(int i, int j, int k) const
+    {
+        return ((size_t)(i - lo[0]) * (size_t)n1 + (size_t)(j - lo[1])) * (size_t)n2 + (size_t)(k - lo[2]);
+    }
+    __host__ __device__ __forceinline__ bool contains(int i, int j, int k) const
+    {
+        return i >= lo[0] && i <= hi[0] && j >= lo[1] && j <= hi[1] && k >= lo[2] && k <= hi[2];
+    }
+};
+
+// One incident-source current sheet: a z plane kp of the down face, three float64
+// component planes of (nx+2) x (ny+2), index i*(ny+2)+j.
+struct Sheet {
+    int kp;
+    double *J[3];
+};
+
+// Source-function parameters for the sheet evaluation kernel.
+struct PlaneParams {
+    int gauss;
+    double k[3], E[3], H[3], E_bg[3], H_bg[3];
+    double ramp, dt, om, dx, dy, dz, width, offset;
+};
+
+struct SheetEval {
+    int kind;        // 1 short pulse, 2 plane wave / gauss
+    int isH;         // 0: E current (samples H), 1: H current (samples E)
+    int off;         // offset of the sampled cell along z (IncidentSource*Current::stepScheme)
+    int t1, t2;      // transverse components
+    double DX;       // normal spacing
+    int time;
+    int nx2, ny2;    // nx+2, ny+2
+    Sheet sheet;
+    PulseParams pulse;
+    PlaneParams plane;
+};
+
+// PLRC pole constants (PLRCData, src/fdtd_plrc.h:71-78) of the active poles, precomputed on the
+// host with the formulas of src/fdtd_plrc.t:198-216.
+struct PoleConsts {
+    double2 dd[3];    // dchi0 - dxi0
+    double2 dxi0[3];
+    double2 crec[3];
+};
+
+template <typename T>
+struct StepArgs {
+    GridDims g;
+    int periodic[3];
+    T *E[3];
+    T *B[3];
+    double2 *psi[3][3];  // [active pole][component]
+    const double *kdE[3];  // kappaE * delta per dimension (lines 0..n+1)
+    const double *kdH[3];
+    double dt;
+    double eps, chi;
+    double denom, numer;  // eps + sumChi0 - sumXi0 (+sigma), eps - sumXi0 (-sigma), sigma = 0
+    PoleConsts pc;
+    int nfaces;
+    CpmlFace face[kMaxFaces];   // E-side faces for the E kernel, H-side faces for the B kernel
+    int face_by_dim[3][2];      // per dimension: [0] high-side face (N/E/U, first in the list), [1] low side, or -1
+    int nsheets;
+    Sheet sheet[kMaxSheets];
+    int sheets_first;           // a source factory precedes the CPML factory in the list
+    // reference path only: current totals
+    const T *Jt[3];
+    int haveJ;
+    int blo[3];                 // first updated B index per dimension (fused path)
+};
+
+}  // namespace mpx
