@@ -1,0 +1,168 @@
+"""Throughput benchmark of the MI355X time-stepping core.
+
+Workload: 512^3 Yee grid, one Lorentz pole (deps 1.1, Om = 2 omega_L) + Kerr (chi3 2e-22 m^2/V^2),
+16-cell CPML on every face, 20 fs / 60-cell-waist short pulse at 5e9 V/m, FP32-bulk storage
+(E, B and CPML auxiliaries float32; PLRC accumulators and the Kerr Newton solve float64).
+A step is one full Yee step (E side then B side) of the whole grid; value = cell-updates/s
+= Nx*Ny*Nz*steps / time, with all state resident in HBM.  The float64 fused path and the
+float64 reference-structured path are timed beside it, and the CPU oracle (OpenMP) is timed
+on a bounded sample of the same workload.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s at 512³ PLRC+Kerr, FP32-bulk vs FP64, % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(desc, cpml_d, n_poles):
+    """Algorithmic HBM bytes per launch of the fused E and B kernels: every state word the
+    kernel owns is read once and written once, neighbour re-reads are free, static tables are
+    ignored (B is read-only in the E kernel and E read-only in the B kernel)."""
+    nx, ny, nz = desc.nx, desc.ny, desc.nz
+    cells = nx * ny * nz
+    wf = 4 if desc.precision == 1 else 8      # E, B, CPML psi word
+    e_kernel = cells * (2 * 3 * wf + 3 * wf + 2 * 6 * 8 * n_poles)
+    b_kernel = cells * (2 * 3 * wf + 3 * wf)
+    # CPML face-layer cells (interior part), two live auxiliaries read + written per side
+    layer = 2 * cpml_d * (nx * ny + ny * nz + nx * nz)
+    e_kernel += layer * 2 * 2 * wf
+    b_kernel += layer * 2 * 2 * wf
+    return e_kernel, b_kernel
+
+
+def run_gpu(desc, steps, warmup):
+    import torch
+
+    from mpulse_amd import abi
+    s = abi.Solver(desc)
+    s.step(warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall_ms = (t1 - t0) * 1e3 / steps
+    # per-kernel durations with HIP events on the solver's own stream
+    prof = s.time_steps(min(steps, 50))
+    s.close()
+    torch.cuda.empty_cache()
+    return wall_ms, prof
+
+
+def cpu_baseline(n=512, n_sample_steps=3):
+    """Oracle (C, OpenMP over x planes) on the same 512^3 workload for a few steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+
+    from mpulse_amd import configs
+    d, _ = configs.config_roofline(precision=0, n=n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    o = Oracle(d, omp=True)
+    o.step(1)  # first touch of the 34 GB state
+    t0 = time.perf_counter()
+    o.step(n_sample_steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    cells = d.nx * d.ny * d.nz
+    return {"value": cells * n_sample_steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement, float64, OpenMP {threads} threads) on the full {n}^3 workload, "
+                      f"{n_sample_steps} steps after 1 untimed step, {dt:.1f} s"}
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from mpulse_amd import abi, configs
+    torch.cuda.set_device(0)
+    n = args.n
+    results = {}
+    forms = [("fp32_bulk_fused", abi.PREC_FP32_BULK, abi.PATH_FUSED)]
+    if not args.no_fp64:
+        forms += [("fp64_fused", abi.PREC_FP64, abi.PATH_FUSED), ("fp64_reference", abi.PREC_FP64, abi.PATH_REFERENCE)]
+    cells = n ** 3
+    for name, prec, path in forms:
+        d, meta = configs.config_roofline(precision=prec, path=path, n=n)
+        wall_ms, prof = run_gpu(d, args.steps, args.warmup)
+        results[name] = {"ms_per_step": wall_ms, "cell_updates_per_s": cells / (wall_ms * 1e-3),
+                         "kernels_ms": {k: v[0] for k, v in prof["kernels"].items()}}
+        print(f"# {name}: {wall_ms:.3f} ms/step, {cells / (wall_ms * 1e-3) / 1e9:.2f} G cell-updates/s, "
+              f"kernels {prof['kernels']}", file=sys.stderr, flush=True)
+
+    main_res = results["fp32_bulk_fused"]
+    d, meta = configs.config_roofline(precision=abi.PREC_FP32_BULK, n=n)
+    eb, bb = kernel_bytes(d, meta["d_cpml"], 1)
+    km = main_res["kernels_ms"]
+    e_ms = km.get("fused_E")
+    achieved = eb / (e_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"fused_E_fp32_{n}_traffic.json")
+    step_bytes = eb + bb
+    out = {
+        "metric": METRIC,
+        "value": main_res["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-bulk (f32 E/B/CPML storage, f64 PLRC psi and Kerr solve, f64 arithmetic)",
+        "data": "synthetic (zero initial fields, analytic short-pulse source)",
+        "config": {"workload": f"{n}^3 PLRC+Kerr, 1 Lorentz pole, 16-cell CPML, 20 fs short pulse, FP32-bulk",
+                   "grid": [n, n, n], "cpml": 16, "poles": 1, "kerr": True, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "fused_E", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "algorithmic_bytes_per_launch": eb, "mean_launch_ms": e_ms},
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                          "achieved_GBs": step_bytes / (main_res["ms_per_step"] * 1e-3) / 1e9,
+                          "frac": step_bytes / (main_res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "forms": results,
+    }
+    if "fp64_fused" in results:
+        out["fp32_over_fp64_fused"] = main_res["cell_updates_per_s"] / results["fp64_fused"]["cell_updates_per_s"]
+        out["fp32_over_fp64_reference_path"] = (main_res["cell_updates_per_s"]
+                                                / results["fp64_reference"]["cell_updates_per_s"])
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n)
+        except Exception as exc:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": str(exc)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
