@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--forms", default="", help="comma list of fp32_bulk_fused, fp64_fused, fp64_reference")
+    ap.add_argument("--copy-probe", action="store_true", help="also time a 4 GiB device copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -108,9 +110,13 @@ def main():
     torch.cuda.set_device(0)
     n = args.n
     results = {}
-    forms = [("fp32_bulk_fused", abi.PREC_FP32_BULK, abi.PATH_FUSED)]
-    if not args.no_fp64:
-        forms += [("fp64_fused", abi.PREC_FP64, abi.PATH_FUSED), ("fp64_reference", abi.PREC_FP64, abi.PATH_REFERENCE)]
+    all_forms = [("fp32_bulk_fused", abi.PREC_FP32_BULK, abi.PATH_FUSED), ("fp64_fused", abi.PREC_FP64, abi.PATH_FUSED),
+                 ("fp64_reference", abi.PREC_FP64, abi.PATH_REFERENCE)]
+    if args.forms:
+        want = set(args.forms.split(","))
+        forms = [f for f in all_forms if f[0] in want or f[0] == "fp32_bulk_fused"]
+    else:
+        forms = all_forms[:1] if args.no_fp64 else all_forms
     cells = n ** 3
     for name, prec, path in forms:
         d, meta = configs.config_roofline(precision=prec, path=path, n=n)
@@ -152,8 +158,13 @@ def main():
                           "frac": step_bytes / (main_res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "forms": results,
     }
+    if args.copy_probe:
+        nb = 4 << 30
+        ms = abi.copy_probe(nb, 20)
+        out["copy_probe"] = {"bytes_moved": 2 * nb, "ms": ms, "GBs": 2 * nb / (ms * 1e-3) / 1e9}
     if "fp64_fused" in results:
         out["fp32_over_fp64_fused"] = main_res["cell_updates_per_s"] / results["fp64_fused"]["cell_updates_per_s"]
+    if "fp64_reference" in results:
         out["fp32_over_fp64_reference_path"] = (main_res["cell_updates_per_s"]
                                                 / results["fp64_reference"]["cell_updates_per_s"])
     if not args.no_cpu:

@@ -67,6 +67,7 @@ extern "C" {
 /* kernel structure */
 #define MPX_PATH_FUSED     0  /* one fused E-side and one fused B-side kernel per step */
 #define MPX_PATH_REFERENCE 1  /* one kernel per reference loop: currents, totals, update, wrap */
+#define MPX_PATH_SWEEP     2  /* one kernel per step (E and B in a single x sweep; periodic boundary) */
 
 /* CurrentFactory registry keys, src/fdtd_plrc.t:431-469 */
 #define MPX_FACTORY_CPML          1  /* "cpml_border"          src/cpml_border.cpp:14-140   */
@@ -176,6 +177,18 @@ int  mpx_sync(mpx_solver *s);
 int  mpx_time_steps(mpx_solver *s, int nsteps, mpx_timing *out);
 /* Device-side byte size of the resident state. */
 int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
+
+/* Launch-shape parameters: "march_e" / "march_b" = x planes walked by one thread in the
+   E-side / B-side fused kernel (1, 2, 4); "sweep_seg" = x planes per block of the sweep kernel;
+   "sweep_variant" = its thread-block shape (0: 64x8, 1: 64x8 with two rows per thread,
+   2: 64x16).  No counterpart in the reference: the hook of
+   the tile autotuner. */
+int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
+
+/* Streaming copy of `bytes` between two fresh device buffers, `reps` times; mean ms per copy.
+   No counterpart in the reference: a measurement aid that calibrates HBM byte counters and
+   gives the practical copy bandwidth beside the stencil kernels. */
+int  mpx_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 const char *mpx_last_error(void);
 int  mpx_abi_version(void);

@@ -19,7 +19,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0,
 SOLVER_PLRC_NONLIN, SOLVER_PLRC_LIN, SOLVER_PLAIN, SOLVER_NL, SOLVER_DISP = 0, 1, 2, 3, 4
 BOUNDARY_PERIODIC, BOUNDARY_XY_PERIODIC = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
-PATH_FUSED, PATH_REFERENCE = 0, 1
+PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
 
 # Direction enum of the reference (src/mpulse.h:28)
@@ -137,6 +137,8 @@ def load():
         "mpx_sync": (C.c_int, [P]),
         "mpx_time_steps": (C.c_int, [P, C.c_int, C.POINTER(Timing)]),
         "mpx_state_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
+        "mpx_copy_probe": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+        "mpx_set_tuning": (C.c_int, [P, C.c_char_p, C.c_int]),
         "mpx_last_error": (C.c_char_p, []),
         "mpx_abi_version": (C.c_int, []),
         "mpx_desc_defaults": (None, [C.POINTER(Desc)]),
@@ -250,7 +252,20 @@ class Solver:
             res["kernels"][t.name[q].value.decode()] = (t.mean_ms[q], t.launches[q])
         return res
 
+    def tune(self, key, value):
+        self._check(self.lib.mpx_set_tuning(self.h, key.encode(), int(value)))
+
     def state_bytes(self):
         b = C.c_size_t()
         self._check(self.lib.mpx_state_bytes(self.h, C.byref(b)))
         return b.value
+
+
+def copy_probe(nbytes, reps=20):
+    """Mean ms of one device-to-device streaming copy of nbytes (read + write)."""
+    lib = load()
+    ms = C.c_double()
+    rc = lib.mpx_copy_probe(nbytes, reps, C.byref(ms))
+    if rc != OK:
+        raise MpxError(rc, lib.mpx_last_error().decode())
+    return ms.value

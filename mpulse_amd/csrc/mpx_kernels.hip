@@ -235,17 +235,18 @@ template void launch_wrap<float>(const GridDims &, float *const[3], int, hipStre
 // FDTD_PLRCLinCore::plrcStepD    src/fdtd_plrc.cpp:94-261.
 // d* are the B differences in the reference's order, kd* the kappa*delta factors.
 // ------------------------------------------------------------------------------------
-template <int NP, bool NONLIN>
-__device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
-                                              double dBx_z, double dBz_x, double dBy_x, double dBx_y,
-                                              double kdx, double kdy, double kdz, double jx, double jy, double jz,
-                                              double2 *const psi[3][3], size_t c, const PoleConsts &pc,
-                                              double dt, double denom, double numer, double chi)
+template <int NP, bool NONLIN, typename PIN, typename POUT>
+__device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
+                                                 double dBx_z, double dBz_x, double dBy_x, double dBx_y,
+                                                 double kdx, double kdy, double kdz, double jx, double jy,
+                                                 double jz, const PIN &psi_in, const POUT &psi, size_t c,
+                                                 bool store, const PoleConsts &pc, double dt, double denom,
+                                                 double numer, double chi)
 {
     double Psix = 0, Psiy = 0, Psiz = 0;
 #pragma unroll
     for (int n = 0; n < NP; ++n) {
-        double2 px = psi[n][0][c], py = psi[n][1][c], pz = psi[n][2][c];
+        double2 px = psi_in[n][0][c], py = psi_in[n][1][c], pz = psi_in[n][2][c];
         const double2 D = pc.dd[n], X = pc.dxi0[n], C = pc.crec[n];
         double pxr = D.x * ex + px.x, pxi = D.y * ex + px.y;
         double pyr = D.x * ey + py.x, pyi = D.y * ey + py.y;
@@ -253,9 +254,11 @@ __device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez
         Psix += pxr;
         Psiy += pyr;
         Psiz += pzr;
-        psi[n][0][c] = make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr));
-        psi[n][1][c] = make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr));
-        psi[n][2][c] = make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr));
+        if (store) {
+            psi[n][0][c] = make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr));
+            psi[n][1][c] = make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr));
+            psi[n][2][c] = make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr));
+        }
     }
     if (NONLIN) {
         double E2 = ex * ex + ey * ey + ez * ez;
@@ -292,6 +295,17 @@ __device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez
         ey = eyn;
         ez = ezn;
     }
+}
+
+template <int NP, bool NONLIN>
+__device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
+                                              double dBx_z, double dBz_x, double dBy_x, double dBx_y,
+                                              double kdx, double kdy, double kdz, double jx, double jy, double jz,
+                                              double2 *const psi[3][3], size_t c, const PoleConsts &pc,
+                                              double dt, double denom, double numer, double chi)
+{
+    plrc_e_update_pp<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kdx, kdy, kdz, jx, jy, jz, psi,
+                                 psi, c, true, pc, dt, denom, numer, chi);
 }
 
 // FDTD_PLRCSolver::stepD loop, src/fdtd_plrc.t:221-233, reading the current totals
@@ -420,40 +434,36 @@ void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s)
 // fused path
 // ------------------------------------------------------------------------------------
 
-// CPML E auxiliaries of one face (CPMLBorderECurrent::stepScheme, src/cpml_border.cpp:398-425)
-// updated in the E kernel; returns the two current contributions.  The face map is
-// x: (Jy, Jz; Bz, By), y: (Jz, Jx; Bx, Bz), z: (Jx, Jy; By, Bx) (src/cpml_border.cpp:357-386).
-template <typename T>
-__device__ __forceinline__ void cpml_e_face(const CpmlFace &f, int coord, int i, int j, int k, double d1,
-                                            double d0, double &v0, double &v1)
+// CPML auxiliaries of one face updated inside the fused kernels.  The face map is
+// x: (psi0 -> Jy from dBz, psi1 -> Jz from dBy), y: (psi0 -> Jz from dBx, psi1 -> Jx from dBz),
+// z: (psi0 -> Jx from dBy, psi1 -> Jy from dBx) (src/cpml_border.cpp:357-386, 456-485).
+// E side: CPMLBorderECurrent::stepScheme, src/cpml_border.cpp:398-425 (psi0 gets -c d1/dx);
+// H side: CPMLBorderHCurrent::stepScheme, src/cpml_border.cpp:499-527 (psi0 gets +c d1/dx).
+template <typename T, bool HSIDE>
+__device__ __forceinline__ void cpml_face(const CpmlFace &f, int coord, int l, double d1, double d0, double &v0,
+                                          double &v1)
 {
     int q = coord - f.lo[f.dim];
-    size_t l = f.at(i, j, k);
     T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
     double bq = f.b[q], cq = f.c[q];
-    T n0 = (T)(bq * (double)p0[l] - cq * d1 / f.dxn);
-    T n1 = (T)(bq * (double)p1[l] + cq * d0 / f.dxn);
+    T n0, n1;
+    if (HSIDE) {
+        n0 = (T)(bq * (double)p0[l] + cq * d1 / f.dxn);
+        n1 = (T)(bq * (double)p1[l] - cq * d0 / f.dxn);
+    } else {
+        n0 = (T)(bq * (double)p0[l] - cq * d1 / f.dxn);
+        n1 = (T)(bq * (double)p1[l] + cq * d0 / f.dxn);
+    }
     p0[l] = n0;
     p1[l] = n1;
     v0 = (double)n0;
     v1 = (double)n1;
 }
 
-// CPMLBorderHCurrent::stepScheme, src/cpml_border.cpp:499-527, for the B kernel
-template <typename T>
-__device__ __forceinline__ void cpml_h_face(const CpmlFace &f, int coord, int i, int j, int k, double d1,
-                                            double d0, double &v0, double &v1)
+// 32-bit layer index of cell (i, j, k) in a face box
+__device__ __forceinline__ int layer_index(const CpmlFace &f, int i, int j, int k)
 {
-    int q = coord - f.lo[f.dim];
-    size_t l = f.at(i, j, k);
-    T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
-    double bq = f.b[q], cq = f.c[q];
-    T n0 = (T)(bq * (double)p0[l] + cq * d1 / f.dxn);
-    T n1 = (T)(bq * (double)p1[l] - cq * d0 / f.dxn);
-    p0[l] = n0;
-    p1[l] = n1;
-    v0 = (double)n0;
-    v1 = (double)n1;
+    return ((i - f.lo[0]) * f.n1 + (j - f.lo[1])) * f.n2 + (k - f.lo[2]);
 }
 
 // add the current-sheet layers that contain cell (i, j, k), in list order
@@ -463,7 +473,7 @@ __device__ __forceinline__ void add_sheets(const StepArgs<T> &a, int i, int j, i
 {
     for (int s = 0; s < a.nsheets; ++s)
         if (k == a.sheet[s].kp) {
-            size_t q = (size_t)i * a.g.py + j;
+            int q = i * a.g.py + j;
             x += a.sheet[s].J[0][q];
             y += a.sheet[s].J[1][q];
             z += a.sheet[s].J[2][q];
@@ -485,74 +495,104 @@ __device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int
     bz = t / nby;
 }
 
-template <typename T, int NP, bool NONLIN, bool CPML>
-__global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby)
+// The per-thread CPML faces that do not change along x: returns the face index for the
+// high side ([0]) and low side ([1]) of dimension dim at coordinate coord, or -1.
+__device__ __forceinline__ void faces_at(const int fbd[3][2], const CpmlFace *faces, int dim, int coord, int out[2])
 {
-    int tbx, tby, tbz;
-    xcd_tile(nbx, nby, a.g.n[0], tbx, tby, tbz);
-    const int k = tbx * 64 + threadIdx.x + 1;
-    const int j = tby * 4 + threadIdx.y + 1;
-    const int i = tbz + 1;
-    const GridDims &g = a.g;
-    if (k > g.n[2] || j > g.n[1]) return;
-    const int im = (i == 1 && a.periodic[0]) ? g.n[0] : i - 1;
-    const int jm = (j == 1 && a.periodic[1]) ? g.n[1] : j - 1;
-    const int km = (k == 1 && a.periodic[2]) ? g.n[2] : k - 1;
-    const size_t c = g.at(i, j, k);
-    const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
-    const double bx = Bx[c], by = By[c], bz = Bz[c];
-    const double dBz_y = bz - (double)Bz[g.at(i, jm, k)];
-    const double dBy_z = by - (double)By[g.at(i, j, km)];
-    const double dBx_z = bx - (double)Bx[g.at(i, j, km)];
-    const double dBz_x = bz - (double)Bz[g.at(im, j, k)];
-    const double dBy_x = by - (double)By[g.at(im, j, k)];
-    const double dBx_y = bx - (double)Bx[g.at(i, jm, k)];
-    double jx = 0.0, jy = 0.0, jz = 0.0;
-    if (a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
-    if (CPML) {
-        // current totals in the reference's list order N,S (y), E,W (x), U,D (z); within a
-        // dimension the high face comes first in that list (face_by_dim[d][0]).
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[1][side];
-            if (f >= 0 && j >= a.face[f].lo[1] && j <= a.face[f].hi[1]) {
-                double v0, v1;  // psi0 -> Jz from dBx_y, psi1 -> Jx from dBz_y
-                cpml_e_face<T>(a.face[f], j, i, j, k, dBx_y, dBz_y, v0, v1);
-                jz += v0;
-                jx += v1;
-            }
-        }
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[0][side];
-            if (f >= 0 && i >= a.face[f].lo[0] && i <= a.face[f].hi[0]) {
-                double v0, v1;  // psi0 -> Jy from dBz_x, psi1 -> Jz from dBy_x
-                cpml_e_face<T>(a.face[f], i, i, j, k, dBz_x, dBy_x, v0, v1);
-                jy += v0;
-                jz += v1;
-            }
-        }
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[2][side];
-            if (f >= 0 && k >= a.face[f].lo[2] && k <= a.face[f].hi[2]) {
-                double v0, v1;  // psi0 -> Jx from dBy_z, psi1 -> Jy from dBx_z
-                cpml_e_face<T>(a.face[f], k, i, j, k, dBy_z, dBx_z, v0, v1);
-                jx += v0;
-                jy += v1;
-            }
-        }
+    for (int s = 0; s < 2; ++s) {
+        int f = fbd[dim][s];
+        out[s] = (f >= 0 && coord >= faces[f].lo[dim] && coord <= faces[f].hi[dim]) ? f : -1;
     }
-    if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
-    double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
-    plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
-                              a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
-    a.E[0][c] = (T)ex;
-    a.E[1][c] = (T)ey;
-    a.E[2][c] = (T)ez;
 }
 
-template <typename T, bool CPML>
+// Fused E side of one step.  A thread owns one (j, k) column segment of L cells along x and
+// walks it plane by plane, keeping B of the previous plane in registers for the x difference.
+template <typename T, int NP, bool NONLIN, bool CPML, int L>
+__global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby, int nbz)
+{
+    int tbx, tby, tbz;
+    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    const GridDims &g = a.g;
+    const int k = tbx * 64 + threadIdx.x + 1;
+    const int j = tby * 4 + threadIdx.y + 1;
+    const int i0 = tbz * L + 1;
+    if (k > g.n[2] || j > g.n[1] || i0 > g.n[0]) return;
+    const int i1 = min(i0 + L - 1, g.n[0]);
+    const int sx = (int)g.sx, pz = g.pz;
+    const int ojm = (j == 1 && a.periodic[1]) ? (g.n[1] - 1) * pz : -pz;
+    const int okm = (k == 1 && a.periodic[2]) ? (g.n[2] - 1) : -1;
+    int c = i0 * sx + j * pz + k + g.z0;
+    const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
+    const int cim = (i0 == 1 && a.periodic[0]) ? c + (g.n[0] - 1) * sx : c - sx;
+    double bz_im = (double)Bz[cim], by_im = (double)By[cim];
+    int fy[2] = {-1, -1}, fz[2] = {-1, -1};
+    if (CPML) {
+        faces_at(a.face_by_dim, a.face, 1, j, fy);
+        faces_at(a.face_by_dim, a.face, 2, k, fz);
+    }
+    const double kdy = a.kdE[1][j], kdz = a.kdE[2][k];
+#pragma unroll
+    for (int u = 0; u < L; ++u, c += sx) {
+        const int i = i0 + u;
+        if (i > i1) break;
+        const double bx = Bx[c], by = By[c], bz = Bz[c];
+        const double dBz_y = bz - (double)Bz[c + ojm];
+        const double dBx_y = bx - (double)Bx[c + ojm];
+        const double dBy_z = by - (double)By[c + okm];
+        const double dBx_z = bx - (double)Bx[c + okm];
+        const double dBz_x = bz - bz_im;
+        const double dBy_x = by - by_im;
+        bz_im = bz;
+        by_im = by;
+        double jx = 0.0, jy = 0.0, jz = 0.0;
+        if (a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+        if (CPML) {
+            // current totals in the reference's list order N,S (y), E,W (x), U,D (z); within a
+            // dimension the high face comes first in that list
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fy[s] >= 0) {
+                    const CpmlFace &f = a.face[fy[s]];
+                    double v0, v1;
+                    cpml_face<T, false>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
+                    jz += v0;
+                    jx += v1;
+                }
+            int fx[2];
+            faces_at(a.face_by_dim, a.face, 0, i, fx);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fx[s] >= 0) {
+                    const CpmlFace &f = a.face[fx[s]];
+                    double v0, v1;
+                    cpml_face<T, false>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
+                    jy += v0;
+                    jz += v1;
+                }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fz[s] >= 0) {
+                    const CpmlFace &f = a.face[fz[s]];
+                    double v0, v1;
+                    cpml_face<T, false>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
+                    jx += v0;
+                    jy += v1;
+                }
+        }
+        if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+        double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
+        plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], kdy, kdz, jx,
+                                  jy, jz, a.psi, (size_t)c, a.pc, a.dt, a.denom, a.numer, a.chi);
+        a.E[0][c] = (T)ex;
+        a.E[1][c] = (T)ey;
+        a.E[2][c] = (T)ez;
+    }
+}
+
+// Fused B side of one step: a thread walks L planes along x keeping E of the next plane in
+// registers as the centre of the following iteration.
+template <typename T, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
@@ -560,71 +600,87 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
     const GridDims &g = a.g;
     const int k = tbx * 64 + threadIdx.x + a.blo[2];
     const int j = tby * 4 + threadIdx.y + a.blo[1];
-    const int i = tbz + a.blo[0];
-    if (k > g.n[2] || j > g.n[1]) return;
-    const int ip = (i == g.n[0] && a.periodic[0]) ? 1 : i + 1;
-    const int jp = (j == g.n[1] && a.periodic[1]) ? 1 : j + 1;
-    const int kp = (k == g.n[2] && a.periodic[2]) ? 1 : k + 1;
-    const size_t c = g.at(i, j, k);
+    const int i0 = tbz * L + a.blo[0];
+    if (k > g.n[2] || j > g.n[1] || i0 > g.n[0]) return;
+    const int i1 = min(i0 + L - 1, g.n[0]);
+    const int sx = (int)g.sx, pz = g.pz;
+    const int ojp = (j == g.n[1] && a.periodic[1]) ? -(g.n[1] - 1) * pz : pz;
+    const int okp = (k == g.n[2] && a.periodic[2]) ? -(g.n[2] - 1) : 1;
+    int c = i0 * sx + j * pz + k + g.z0;
     const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
-    const double ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    const double dEy_z = (double)Ey[g.at(i, j, kp)] - ey;
-    const double dEz_y = (double)Ez[g.at(i, jp, k)] - ez;
-    const double dEz_x = (double)Ez[g.at(ip, j, k)] - ez;
-    const double dEx_z = (double)Ex[g.at(i, j, kp)] - ex;
-    const double dEx_y = (double)Ex[g.at(i, jp, k)] - ex;
-    const double dEy_x = (double)Ey[g.at(ip, j, k)] - ey;
-    double mx = 0.0, my = 0.0, mz = 0.0;
-    if (a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
-    if (CPML) {
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[1][side];
-            if (f >= 0 && j >= a.face[f].lo[1] && j <= a.face[f].hi[1]) {
-                double v0, v1;  // psi0 -> Mz from dEx_y, psi1 -> Mx from dEz_y
-                cpml_h_face<T>(a.face[f], j, i, j, k, dEx_y, dEz_y, v0, v1);
-                mz += v0;
-                mx += v1;
-            }
-        }
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[0][side];
-            if (f >= 0 && i >= a.face[f].lo[0] && i <= a.face[f].hi[0]) {
-                double v0, v1;  // psi0 -> My from dEz_x, psi1 -> Mz from dEy_x
-                cpml_h_face<T>(a.face[f], i, i, j, k, dEz_x, dEy_x, v0, v1);
-                my += v0;
-                mz += v1;
-            }
-        }
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            int f = a.face_by_dim[2][side];
-            if (f >= 0 && k >= a.face[f].lo[2] && k <= a.face[f].hi[2]) {
-                double v0, v1;  // psi0 -> Mx from dEy_z, psi1 -> My from dEx_z
-                cpml_h_face<T>(a.face[f], k, i, j, k, dEy_z, dEx_z, v0, v1);
-                mx += v0;
-                my += v1;
-            }
-        }
-    }
-    if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
-    const double kdx = a.kdH[0][i], kdy = a.kdH[1][j], kdz = a.kdH[2][k];
     T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
-    Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z / kdz - dEz_y / kdy) + mx));
-    By[c] = (T)((double)By[c] + a.dt * ((dEz_x / kdx - dEx_z / kdz) + my));
-    Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+    double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    int fy[2] = {-1, -1}, fz[2] = {-1, -1};
+    if (CPML) {
+        faces_at(a.face_by_dim, a.face, 1, j, fy);
+        faces_at(a.face_by_dim, a.face, 2, k, fz);
+    }
+    const double kdy = a.kdH[1][j], kdz = a.kdH[2][k];
+#pragma unroll
+    for (int u = 0; u < L; ++u, c += sx) {
+        const int i = i0 + u;
+        if (i > i1) break;
+        const int cip = (i == g.n[0] && a.periodic[0]) ? c - (g.n[0] - 1) * sx : c + sx;
+        const double ex1 = Ex[cip], ey1 = Ey[cip], ez1 = Ez[cip];
+        const double dEy_z = (double)Ey[c + okp] - ey;
+        const double dEx_z = (double)Ex[c + okp] - ex;
+        const double dEz_y = (double)Ez[c + ojp] - ez;
+        const double dEx_y = (double)Ex[c + ojp] - ex;
+        const double dEz_x = ez1 - ez;
+        const double dEy_x = ey1 - ey;
+        double mx = 0.0, my = 0.0, mz = 0.0;
+        if (a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
+        if (CPML) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fy[s] >= 0) {
+                    const CpmlFace &f = a.face[fy[s]];
+                    double v0, v1;  // psi0 -> Mz from dEx_y, psi1 -> Mx from dEz_y
+                    cpml_face<T, true>(f, j, layer_index(f, i, j, k), dEx_y, dEz_y, v0, v1);
+                    mz += v0;
+                    mx += v1;
+                }
+            int fx[2];
+            faces_at(a.face_by_dim, a.face, 0, i, fx);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fx[s] >= 0) {
+                    const CpmlFace &f = a.face[fx[s]];
+                    double v0, v1;  // psi0 -> My from dEz_x, psi1 -> Mz from dEy_x
+                    cpml_face<T, true>(f, i, layer_index(f, i, j, k), dEz_x, dEy_x, v0, v1);
+                    my += v0;
+                    mz += v1;
+                }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (fz[s] >= 0) {
+                    const CpmlFace &f = a.face[fz[s]];
+                    double v0, v1;  // psi0 -> Mx from dEy_z, psi1 -> My from dEx_z
+                    cpml_face<T, true>(f, k, layer_index(f, i, j, k), dEy_z, dEx_z, v0, v1);
+                    mx += v0;
+                    my += v1;
+                }
+        }
+        if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
+        const double kdx = a.kdH[0][i];
+        Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z / kdz - dEz_y / kdy) + mx));
+        By[c] = (T)((double)By[c] + a.dt * ((dEz_x / kdx - dEx_z / kdz) + my));
+        Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+        ex = ex1;
+        ey = ey1;
+        ez = ez1;
+    }
 }
 
-template <typename T>
-void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+template <typename T, int L>
+static void launch_fused_E_L(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
 {
     const GridDims &g = a.g;
-    int nbx = (g.n[2] + 63) / 64, nby = (g.n[1] + 3) / 4;
+    int nbx = (g.n[2] + 63) / 64, nby = (g.n[1] + 3) / 4, nbz = (g.n[0] + L - 1) / L;
     dim3 blk(64, 4);
-    dim3 grd(nbx, nby, g.n[0]);
+    dim3 grd(nbx, nby, nbz);
     bool cp = a.nfaces > 0;
-#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP>), grd, blk, 0, s, a, nbx, nby)
+#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP, L>), grd, blk, 0, s, a, nbx, nby, nbz)
 #define MPX_FE_NP(NL, CP)            \
     switch (np) {                    \
     case 0: MPX_FE(0, NL, CP); break; \
@@ -642,17 +698,37 @@ void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
 }
 
 template <typename T>
-void launch_fused_B(const StepArgs<T> &a, hipStream_t s)
+void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+{
+    switch (a.march) {
+    case 2: launch_fused_E_L<T, 2>(a, np, nonlin, s); break;
+    case 4: launch_fused_E_L<T, 4>(a, np, nonlin, s); break;
+    default: launch_fused_E_L<T, 1>(a, np, nonlin, s); break;
+    }
+}
+
+template <typename T, int L>
+static void launch_fused_B_L(const StepArgs<T> &a, hipStream_t s)
 {
     const GridDims &g = a.g;
     int nk = g.n[2] - a.blo[2] + 1, nj = g.n[1] - a.blo[1] + 1, ni = g.n[0] - a.blo[0] + 1;
-    int nbx = (nk + 63) / 64, nby = (nj + 3) / 4;
+    int nbx = (nk + 63) / 64, nby = (nj + 3) / 4, nbz = (ni + L - 1) / L;
     dim3 blk(64, 4);
-    dim3 grd(nbx, nby, ni);
+    dim3 grd(nbx, nby, nbz);
     if (a.nfaces > 0)
-        hipLaunchKernelGGL((k_fused_B<T, true>), grd, blk, 0, s, a, nbx, nby, ni);
+        hipLaunchKernelGGL((k_fused_B<T, true, L>), grd, blk, 0, s, a, nbx, nby, nbz);
     else
-        hipLaunchKernelGGL((k_fused_B<T, false>), grd, blk, 0, s, a, nbx, nby, ni);
+        hipLaunchKernelGGL((k_fused_B<T, false, L>), grd, blk, 0, s, a, nbx, nby, nbz);
+}
+
+template <typename T>
+void launch_fused_B(const StepArgs<T> &a, hipStream_t s)
+{
+    switch (a.march) {
+    case 2: launch_fused_B_L<T, 2>(a, s); break;
+    case 4: launch_fused_B_L<T, 4>(a, s); break;
+    default: launch_fused_B_L<T, 1>(a, s); break;
+    }
 }
 
 template void launch_fused_E<double>(const StepArgs<double> &, int, bool, hipStream_t);
@@ -858,5 +934,293 @@ double pulse_norm_device(const PulseParams &P, const FaddeevaTables &ft, hipStre
     (void)hipFree(d);
     return h;
 }
+
+}  // namespace mpx
+
+namespace mpx {
+
+// Plain streaming copy, one 16-byte vector per thread over a grid that covers the buffer
+// (the fastest copy shape measured on gfx950): the calibration kernel for HBM byte counters
+// and the practical bandwidth ceiling of the device.
+__global__ __launch_bounds__(256) void k_hbm_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n)
+{
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = src[q];
+}
+
+void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
+{
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_hbm_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const double2 *)src,
+                       (double2 *)dst, n);
+}
+
+}  // namespace mpx
+
+namespace mpx {
+
+// add the current-sheet layers of list `sh` that contain cell (i, j, k), in list order
+__device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py, int i, int j, int k, double &x,
+                                                 double &y, double &z)
+{
+    for (int s = 0; s < n; ++s)
+        if (k == sh[s].kp) {
+            int q = i * py + j;
+            x += sh[s].J[0][q];
+            y += sh[s].J[1][q];
+            z += sh[s].J[2][q];
+        }
+}
+
+// ------------------------------------------------------------------------------------
+// single-pass sweep path (periodic boundaries): E and B of a step in one kernel
+// ------------------------------------------------------------------------------------
+//
+// Block = 64 lanes along z x BY thread rows along y; each thread row handles RPT tile rows,
+// so a tile is 64 x TYR cells (TYR = BY * RPT).  Lane 63 and tile row TYR-1 are a one-cell
+// halo: E is computed there (identically to the owning block) but never stored, so the
+// B update of the owned 63 x (TYR-1) cells finds E_new at k+1 and j+1 inside the block.
+// The block walks the x planes i0..i1 of its segment: at plane p it computes E_new(p), puts
+// it in a triple-buffered LDS tile, synchronises once, and updates B at plane p-1 from
+// E_new(p-1) and E_new(p).  Every array the step reads is an "old" buffer and every array it
+// writes a "new" buffer (CPML H auxiliaries excepted: only owners touch them), so halo
+// readers never see another block's results.
+template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
+__global__ __launch_bounds__(64 * BY) void k_sweep(SweepArgs<T> a, int ntz, int nty, int nseg)
+{
+    constexpr int TYR = BY * RPT;
+    __shared__ T lds[3][3][TYR][64];
+    int tz, ty, tx;
+    xcd_tile(ntz, nty, nseg, tz, ty, tx);
+    const GridDims &g = a.g;
+    const int Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
+    const int sx = (int)g.sx, pz = g.pz;
+    const int lane = threadIdx.x;
+    const int k = 1 + 63 * tz + lane;
+    const bool kact = k <= Nz + 1;
+    // physical z index (periodic image of the halo lane); idle lanes get 1 so that every
+    // index they form stays inside the arrays, although they load nothing from the grids
+    const int kk = (!kact || k == Nz + 1) ? 1 : k;
+    const int okm = (kk == 1) ? (Nz - 1) : -1;
+    const int i0 = 1 + a.seg * tx;
+    const int i1 = min(i0 + a.seg - 1, Nx);
+    int jj[RPT], ojm[RPT], rowi[RPT];
+    bool eact[RPT], own[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        rowi[r] = threadIdx.y + BY * r;
+        int j = 1 + (TYR - 1) * ty + rowi[r];
+        eact[r] = kact && j <= Ny + 1;
+        own[r] = eact[r] && lane < 63 && k <= Nz && rowi[r] < TYR - 1 && j <= Ny;
+        jj[r] = (j == Ny + 1) ? 1 : j;
+        ojm[r] = (jj[r] == 1) ? (Ny - 1) * pz : -pz;
+        if (!eact[r]) jj[r] = 1;  // keeps idle threads' addresses in range; they load nothing
+    }
+    const T *Bxo = a.Bo[0], *Byo = a.Bo[1], *Bzo = a.Bo[2];
+    // B_old at plane i0-1 (the x neighbour of the first E plane)
+    const int im0 = (i0 == 1) ? Nx : i0 - 1;
+    double bprev[RPT][3], eprev[RPT][3];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        eprev[r][0] = eprev[r][1] = eprev[r][2] = 0.0;
+        bprev[r][0] = bprev[r][1] = bprev[r][2] = 0.0;
+        if (eact[r]) {
+            int c = im0 * sx + jj[r] * pz + kk + g.z0;
+            bprev[r][0] = Bxo[c];
+            bprev[r][1] = Byo[c];
+            bprev[r][2] = Bzo[c];
+        }
+    }
+    const double kdz_e = a.kdE[2][kk], kdz_h = a.kdH[2][kk];
+    for (int p = i0; p <= i1 + 1; ++p) {
+        const int pp = (p == Nx + 1) ? 1 : p;
+        const int buf = p % 3;
+        double ecur[RPT][3], bcur[RPT][3];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            ecur[r][0] = ecur[r][1] = ecur[r][2] = 0.0;
+            bcur[r][0] = bcur[r][1] = bcur[r][2] = 0.0;
+            if (!eact[r]) continue;
+            const int j = jj[r];
+            const int c = pp * sx + j * pz + kk + g.z0;
+            const double bx = Bxo[c], by = Byo[c], bz = Bzo[c];
+            const double dBz_y = bz - (double)Bzo[c + ojm[r]];
+            const double dBx_y = bx - (double)Bxo[c + ojm[r]];
+            const double dBy_z = by - (double)Byo[c + okm];
+            const double dBx_z = bx - (double)Bxo[c + okm];
+            const double dBz_x = bz - bprev[r][2];
+            const double dBy_x = by - bprev[r][1];
+            bcur[r][0] = bx;
+            bcur[r][1] = by;
+            bcur[r][2] = bz;
+            const bool store = own[r] && p <= i1;
+            double jx = 0.0, jy = 0.0, jz = 0.0;
+            if (a.sheets_first) add_sheet_planes(a.sheet, a.nsheets, g.py, pp, j, kk, jx, jy, jz);
+            if (CPML) {
+                // E auxiliaries, list order N,S (y), E,W (x), U,D (z), high face first
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int dim = d == 0 ? 1 : (d == 1 ? 0 : 2);
+                    const int coord = dim == 0 ? pp : (dim == 1 ? j : kk);
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int fi = a.efbd[dim][s];
+                        if (fi < 0) continue;
+                        const CpmlFace &f = a.eface[fi];
+                        if (coord < f.lo[dim] || coord > f.hi[dim]) continue;
+                        const int l = layer_index(f, pp, j, kk);
+                        const int q = coord - f.lo[dim];
+                        const T *o0 = (const T *)a.epsi_old[fi][0], *o1 = (const T *)a.epsi_old[fi][1];
+                        double d1 = dim == 0 ? dBz_x : (dim == 1 ? dBx_y : dBy_z);
+                        double d0 = dim == 0 ? dBy_x : (dim == 1 ? dBz_y : dBx_z);
+                        T n0 = (T)(f.b[q] * (double)o0[l] - f.c[q] * d1 / f.dxn);
+                        T n1 = (T)(f.b[q] * (double)o1[l] + f.c[q] * d0 / f.dxn);
+                        if (store) {
+                            ((T *)f.psi0)[l] = n0;
+                            ((T *)f.psi1)[l] = n1;
+                        }
+                        if (dim == 0) {
+                            jy += (double)n0;
+                            jz += (double)n1;
+                        } else if (dim == 1) {
+                            jz += (double)n0;
+                            jx += (double)n1;
+                        } else {
+                            jx += (double)n0;
+                            jy += (double)n1;
+                        }
+                    }
+                }
+            }
+            if (!a.sheets_first) add_sheet_planes(a.sheet, a.nsheets, g.py, pp, j, kk, jx, jy, jz);
+            double ex = (double)a.Eo[0][c], ey = (double)a.Eo[1][c], ez = (double)a.Eo[2][c];
+            plrc_e_update_pp<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][pp],
+                                         a.kdE[1][j], kdz_e, jx, jy, jz, a.psio, a.psin, c, store, a.pc, a.dt,
+                                         a.denom, a.numer, a.chi);
+            const T tx_ = (T)ex, ty_ = (T)ey, tz_ = (T)ez;
+            if (store) {
+                a.En[0][c] = tx_;
+                a.En[1][c] = ty_;
+                a.En[2][c] = tz_;
+            }
+            lds[buf][0][rowi[r]][lane] = tx_;
+            lds[buf][1][rowi[r]][lane] = ty_;
+            lds[buf][2][rowi[r]][lane] = tz_;
+            ecur[r][0] = (double)tx_;
+            ecur[r][1] = (double)ty_;
+            ecur[r][2] = (double)tz_;
+        }
+        __syncthreads();
+        if (p > i0) {
+            const int pb = p - 1;  // B plane, inside [i0, i1]
+            const int pbuf = pb % 3;
+#pragma unroll
+            for (int r = 0; r < RPT; ++r) {
+                if (!own[r]) continue;
+                const int j = jj[r];
+                const int c = pb * sx + j * pz + kk + g.z0;
+                const int row = rowi[r];
+                const double ex = eprev[r][0], ey = eprev[r][1], ez = eprev[r][2];
+                const double dEy_z = (double)lds[pbuf][1][row][lane + 1] - ey;
+                const double dEx_z = (double)lds[pbuf][0][row][lane + 1] - ex;
+                const double dEz_y = (double)lds[pbuf][2][row + 1][lane] - ez;
+                const double dEx_y = (double)lds[pbuf][0][row + 1][lane] - ex;
+                const double dEz_x = ecur[r][2] - ez;
+                const double dEy_x = ecur[r][1] - ey;
+                double mx = 0.0, my = 0.0, mz = 0.0;
+                if (a.sheets_first) add_sheet_planes(a.hsheet, a.nhsheets, g.py, pb, j, kk, mx, my, mz);
+                if (CPML) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int dim = d == 0 ? 1 : (d == 1 ? 0 : 2);
+                        const int coord = dim == 0 ? pb : (dim == 1 ? j : kk);
+#pragma unroll
+                        for (int s = 0; s < 2; ++s) {
+                            const int fi = a.hfbd[dim][s];
+                            if (fi < 0) continue;
+                            const CpmlFace &f = a.hface[fi];
+                            if (coord < f.lo[dim] || coord > f.hi[dim]) continue;
+                            const int l = layer_index(f, pb, j, kk);
+                            const int q = coord - f.lo[dim];
+                            T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
+                            double d1 = dim == 0 ? dEz_x : (dim == 1 ? dEx_y : dEy_z);
+                            double d0 = dim == 0 ? dEy_x : (dim == 1 ? dEz_y : dEx_z);
+                            T n0 = (T)(f.b[q] * (double)p0[l] + f.c[q] * d1 / f.dxn);
+                            T n1 = (T)(f.b[q] * (double)p1[l] - f.c[q] * d0 / f.dxn);
+                            p0[l] = n0;
+                            p1[l] = n1;
+                            if (dim == 0) {
+                                my += (double)n0;
+                                mz += (double)n1;
+                            } else if (dim == 1) {
+                                mz += (double)n0;
+                                mx += (double)n1;
+                            } else {
+                                mx += (double)n0;
+                                my += (double)n1;
+                            }
+                        }
+                    }
+                }
+                if (!a.sheets_first) add_sheet_planes(a.hsheet, a.nhsheets, g.py, pb, j, kk, mx, my, mz);
+                const double kdx = a.kdH[0][pb], kdy = a.kdH[1][j];
+                a.Bn[0][c] = (T)(bprev[r][0] + a.dt * ((dEy_z / kdz_h - dEz_y / kdy) + mx));
+                a.Bn[1][c] = (T)(bprev[r][1] + a.dt * ((dEz_x / kdx - dEx_z / kdz_h) + my));
+                a.Bn[2][c] = (T)(bprev[r][2] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                eprev[r][q] = ecur[r][q];
+                bprev[r][q] = bcur[r][q];
+            }
+    }
+}
+
+template <typename T, int NP, bool NONLIN, bool CPML>
+static void launch_sweep_np(const SweepArgs<T> &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    int ntz = (g.n[2] + 62) / 63;
+    int nseg = (g.n[0] + a.seg - 1) / a.seg;
+    if (a.variant == 1) {  // 64 x 8 threads, 2 rows each: 16-row tiles
+        int nty = (g.n[1] + 14) / 15;
+        hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 8, 2>), dim3(ntz, nty, nseg), dim3(64, 8), 0, s, a, ntz, nty,
+                           nseg);
+    } else if (a.variant == 2) {  // 64 x 16 threads, 1 row each
+        int nty = (g.n[1] + 14) / 15;
+        hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 16, 1>), dim3(ntz, nty, nseg), dim3(64, 16), 0, s, a, ntz,
+                           nty, nseg);
+    } else {  // 64 x 8 threads, 1 row each: 8-row tiles
+        int nty = (g.n[1] + 6) / 7;
+        hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 8, 1>), dim3(ntz, nty, nseg), dim3(64, 8), 0, s, a, ntz, nty,
+                           nseg);
+    }
+}
+
+template <typename T>
+void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+{
+    bool cp = a.nefaces > 0;
+#define MPX_SW(NPV, NL, CP) launch_sweep_np<T, NPV, NL, CP>(a, s)
+#define MPX_SW_NP(NL, CP)            \
+    switch (np) {                    \
+    case 0: MPX_SW(0, NL, CP); break; \
+    case 1: MPX_SW(1, NL, CP); break; \
+    case 2: MPX_SW(2, NL, CP); break; \
+    default: MPX_SW(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cp) { MPX_SW_NP(true, true) } else { MPX_SW_NP(true, false) }
+    } else {
+        if (cp) { MPX_SW_NP(false, true) } else { MPX_SW_NP(false, false) }
+    }
+#undef MPX_SW_NP
+#undef MPX_SW
+}
+template void launch_sweep<double>(const SweepArgs<double> &, int, bool, hipStream_t);
+template void launch_sweep<float>(const SweepArgs<float> &, int, bool, hipStream_t);
 
 }  // namespace mpx

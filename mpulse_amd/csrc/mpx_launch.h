@@ -51,11 +51,14 @@ template <typename T>
 void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s);
 template <typename T>
 void launch_fused_B(const StepArgs<T> &a, hipStream_t s);
+template <typename T>
+void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s);
 void launch_simple_stepD(const SimpleArgs &a, hipStream_t s);
 void launch_simple_stepB(const SimpleArgs &a, hipStream_t s);
 template <typename T>
 void launch_probe(const ProbeArgs<T> &a, double *row, hipStream_t s);
 size_t energy_partials(const GridDims &g);
+void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 template <typename T>
 void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s);
 

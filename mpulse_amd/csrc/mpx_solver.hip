@@ -88,10 +88,10 @@ struct Timer {
 };
 
 const char *kClassNames[] = {"sheet_eval", "cpml_E", "sum_J", "stepD_ref", "wrap_E", "cpml_H", "sum_M",
-                             "stepB_ref", "wrap_B", "fused_E", "fused_B", "simple_E", "simple_B", "probe"};
+                             "stepB_ref", "wrap_B", "fused_E", "fused_B", "simple_E", "simple_B", "probe", "sweep"};
 enum {
     K_SHEET = 0, K_CPML_E, K_SUM_J, K_STEPD_REF, K_WRAP_E, K_CPML_H, K_SUM_M, K_STEPB_REF, K_WRAP_B,
-    K_FUSED_E, K_FUSED_B, K_SIMPLE_E, K_SIMPLE_B, K_PROBE, K_NCLASSES
+    K_FUSED_E, K_FUSED_B, K_SIMPLE_E, K_SIMPLE_B, K_PROBE, K_SWEEP, K_NCLASSES
 };
 
 }  // namespace
@@ -136,6 +136,13 @@ struct mpx_solver {
 
     double *d_part = nullptr, *d_energy = nullptr;
     int time = 0;
+    int march_e = 1, march_b = 1;  // x planes per thread in the fused kernels (1, 2 or 4)
+    // sweep path: second set of buffers (swapped with the first after every step)
+    void *E2[3] = {}, *B2[3] = {};
+    double2 *psi2[3][3] = {};
+    std::vector<void *> epsi2[2];      // per E face: alternate psi0 / psi1
+    std::vector<int> eface_layer[2];   // per E face: layer holding psi0 / psi1
+    int sweep_seg = 32, sweep_variant = 2;
     Timer timer;
 
     ~mpx_solver()
@@ -451,7 +458,7 @@ int mpx_create(const mpx_desc *desc, mpx_solver **out)
         return fail(MPX_ERR_INVALID, "unknown boundary");
     if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK)
         return fail(MPX_ERR_INVALID, "unknown precision");
-    if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE)
+    if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE && desc->path != MPX_PATH_SWEEP)
         return fail(MPX_ERR_INVALID, "unknown kernel path");
     if (desc->n_factories < 0 || desc->n_factories > MPX_MAX_FACTORIES)
         return fail(MPX_ERR_INVALID, "too many current factories");
@@ -497,6 +504,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         return fail(MPX_ERR_INVALID, "FDTD_PLRCSolver: Omega should not be zero!");  // src/fdtd_plrc.t:20-24
     if (s->plrc && s->path == MPX_PATH_REFERENCE && s->prec != MPX_PREC_FP64)
         return fail(MPX_ERR_UNSUPPORTED, "the reference-structured path runs in float64 only");
+    if (s->path == MPX_PATH_SWEEP && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
+        return fail(MPX_ERR_UNSUPPORTED, "the sweep path needs a PLRC solver and the single_periodic boundary");
     for (int n = 0; n < 3; ++n)
         if (s->plrc && !(desc->l_om2[n] > desc->l_delta[n] * desc->l_delta[n]))
             return fail(MPX_ERR_INVALID, "pole " + std::to_string(n + 1) + ": L*Om2 must exceed L*delta^2");
@@ -512,6 +521,9 @@ static int build(mpx_solver *s, const mpx_desc *desc)
     g.pz = align + ((desc->nz + 1 + align - 1) / align) * align;
     g.sx = (long long)g.py * g.pz;
     s->cells = (size_t)(desc->nx + 2) * (size_t)g.sx;
+    // the fused kernels index grids with 32-bit offsets
+    if (s->plrc && s->path != MPX_PATH_REFERENCE && s->cells >= (size_t)0x7fffffff)
+        return fail(MPX_ERR_UNSUPPORTED, "grid too large for the fused kernels' 32-bit indexing");
 
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(MPX_ERR_HIP, "hipStreamCreate failed");
@@ -780,6 +792,32 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                 if (rc) return rc;
             }
         }
+        if (s->path == MPX_PATH_SWEEP) {
+            for (int c = 0; c < 3; ++c) {
+                if (s->esz == 8) {
+                    s->E2[c] = s->dalloc<double>(s->cells, rc);
+                    s->B2[c] = s->dalloc<double>(s->cells, rc);
+                } else {
+                    s->E2[c] = s->dalloc<float>(s->cells, rc);
+                    s->B2[c] = s->dalloc<float>(s->cells, rc);
+                }
+                for (int a = 0; a < s->np; ++a) s->psi2[a][c] = s->dalloc<double2>(s->cells, rc);
+                if (rc) return rc;
+            }
+            for (size_t f = 0; f < s->efaces.size(); ++f) {
+                const CpmlFace &F = s->efaces[f];
+                size_t cnt = (size_t)(F.hi[0] - F.lo[0] + 1) * F.n1 * F.n2;
+                for (int w = 0; w < 2; ++w) {
+                    void *p = (s->esz == 8) ? (void *)s->dalloc<double>(cnt, rc) : (void *)s->dalloc<float>(cnt, rc);
+                    if (rc) return rc;
+                    s->epsi2[w].push_back(p);
+                    int li = -1;
+                    for (size_t q = 0; q < s->layers.size(); ++q)
+                        if (s->layers[q].comp[0] == (w == 0 ? F.psi0 : F.psi1)) li = (int)q;
+                    s->eface_layer[w].push_back(li);
+                }
+            }
+        }
     }
     s->d_part = s->dalloc<double>(mpx::energy_partials(g), rc);
     s->d_energy = s->dalloc<double>(1, rc);
@@ -829,6 +867,7 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
         if (c.type == CUR_INC_E || c.type == CUR_INC_H) a.sheet[a.nsheets++] = c.ev.sheet;
     a.sheets_first = s->sheets_first;
     a.haveJ = 0;
+    a.march = hside ? s->march_b : s->march_e;
     return a;
 }
 
@@ -943,6 +982,70 @@ static mpx::SimpleArgs simple_args(mpx_solver *s)
     return a;
 }
 
+template <typename T>
+static mpx::SweepArgs<T> sweep_args(mpx_solver *s)
+{
+    mpx::SweepArgs<T> a{};
+    a.g = s->g;
+    for (int c = 0; c < 3; ++c) {
+        a.Eo[c] = (const T *)s->E[c];
+        a.En[c] = (T *)s->E2[c];
+        a.Bo[c] = (const T *)s->B[c];
+        a.Bn[c] = (T *)s->B2[c];
+        a.kdE[c] = s->d_kdE[c];
+        a.kdH[c] = s->d_kdH[c];
+        for (int n = 0; n < 3; ++n) {
+            a.psio[n][c] = s->psi[n][c];
+            a.psin[n][c] = s->psi2[n][c];
+        }
+    }
+    a.dt = s->d.dt;
+    a.chi = s->d.chi;
+    a.denom = s->denom;
+    a.numer = s->numer;
+    a.pc = s->pc;
+    a.nefaces = (int)s->efaces.size();
+    a.nhfaces = (int)s->hfaces.size();
+    for (int d = 0; d < 3; ++d)
+        for (int q = 0; q < 2; ++q) a.efbd[d][q] = a.hfbd[d][q] = -1;
+    for (size_t f = 0; f < s->efaces.size(); ++f) {
+        a.eface[f] = s->efaces[f];
+        a.eface[f].psi0 = s->epsi2[0][f];
+        a.eface[f].psi1 = s->epsi2[1][f];
+        a.epsi_old[f][0] = s->efaces[f].psi0;
+        a.epsi_old[f][1] = s->efaces[f].psi1;
+    }
+    for (size_t f = 0; f < s->hfaces.size(); ++f) a.hface[f] = s->hfaces[f];
+    for (const Current &c : s->ecur)
+        if (c.face >= 0) a.efbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    for (const Current &c : s->hcur)
+        if (c.face >= 0) a.hfbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    for (const Current &c : s->ecur)
+        if (c.type == CUR_INC_E) a.sheet[a.nsheets++] = c.ev.sheet;
+    for (const Current &c : s->hcur)
+        if (c.type == CUR_INC_H) a.hsheet[a.nhsheets++] = c.ev.sheet;
+    a.sheets_first = s->sheets_first;
+    a.seg = s->sweep_seg;
+    a.variant = s->sweep_variant;
+    return a;
+}
+
+// after a sweep step the "new" buffers hold the state: exchange the two sets
+static void swap_sweep_buffers(mpx_solver *s)
+{
+    for (int c = 0; c < 3; ++c) {
+        std::swap(s->E[c], s->E2[c]);
+        std::swap(s->B[c], s->B2[c]);
+        for (int n = 0; n < 3; ++n) std::swap(s->psi[n][c], s->psi2[n][c]);
+    }
+    for (size_t f = 0; f < s->efaces.size(); ++f) {
+        std::swap(s->efaces[f].psi0, s->epsi2[0][f]);
+        std::swap(s->efaces[f].psi1, s->epsi2[1][f]);
+        if (s->eface_layer[0][f] >= 0) s->layers[s->eface_layer[0][f]].comp[0] = s->efaces[f].psi0;
+        if (s->eface_layer[1][f] >= 0) s->layers[s->eface_layer[1][f]].comp[0] = s->efaces[f].psi1;
+    }
+}
+
 // one full step: FDTD_PLRCSolver::stepScheme (src/fdtd_plrc.t:127-168) or the plain/nl/disp
 // stepScheme (src/fdtd_plain.cpp:31-35)
 static int step_once(mpx_solver *s)
@@ -1020,6 +1123,17 @@ static int step_once(mpx_solver *s)
         s->tock();
         MPX_LAUNCHED("stepB");
         if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
+    } else if (s->path == MPX_PATH_SWEEP) {
+        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
+        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
+        s->tick(K_SWEEP);
+        if (s->esz == 8)
+            mpx::launch_sweep<double>(sweep_args<double>(s), s->np, s->nonlin, s->stream);
+        else
+            mpx::launch_sweep<float>(sweep_args<float>(s), s->np, s->nonlin, s->stream);
+        s->tock();
+        MPX_LAUNCHED("sweep");
+        swap_sweep_buffers(s);
     } else {
         if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
         if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
@@ -1054,7 +1168,7 @@ static int step_once(mpx_solver *s)
 // ghost planes of E and B as the reference stores them (the fused path wraps by index)
 static int materialise_ghosts(mpx_solver *s)
 {
-    if (s->plrc && s->path == MPX_PATH_FUSED) {
+    if (s->plrc && s->path != MPX_PATH_REFERENCE) {
         int rc;
         if (s->esz == 8) {
             if ((rc = wrap_group<double>(s, s->E, K_WRAP_E))) return rc;
@@ -1417,6 +1531,61 @@ int mpx_state_bytes(mpx_solver *s, size_t *bytes)
     b += 6 * s->cells * s->esz;
     b += (size_t)s->np * 3 * s->cells * 16;
     *bytes = b;
+    return MPX_OK;
+}
+
+// Streaming-copy probe: copies `bytes` between two fresh device buffers `reps` times and
+// reports the mean kernel time.  No reference counterpart: it calibrates HBM byte counters
+// and measures the practical copy bandwidth next to the stencil kernels.
+int mpx_copy_probe(size_t bytes, int reps, double *ms_per_copy)
+{
+    if (bytes < 16 || reps < 1 || !ms_per_copy) return fail(MPX_ERR_INVALID, "invalid copy probe arguments");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    MPX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        (void)hipStreamDestroy(st);
+        return fail(MPX_ERR_NOMEM, "copy probe allocation failed");
+    }
+    MPX_HIP(hipMemsetAsync(a, 0, bytes, st));
+    MPX_HIP(hipEventCreate(&e0));
+    MPX_HIP(hipEventCreate(&e1));
+    mpx::launch_hbm_copy(a, b, bytes, st);  // warm-up
+    MPX_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) mpx::launch_hbm_copy(a, b, bytes, st);
+    MPX_HIP(hipEventRecord(e1, st));
+    MPX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MPX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return MPX_OK;
+}
+
+// Launch-shape parameters of the fused kernels (autotuner hook; no reference counterpart).
+int mpx_set_tuning(mpx_solver *s, const char *key, int value)
+{
+    if (!s || !key) return fail(MPX_ERR_INVALID, "null argument");
+    bool march = !std::strcmp(key, "march_e") || !std::strcmp(key, "march_b");
+    if (march && value != 1 && value != 2 && value != 4) return fail(MPX_ERR_INVALID, "march must be 1, 2 or 4");
+    if (!std::strcmp(key, "march_e"))
+        s->march_e = value;
+    else if (!std::strcmp(key, "march_b"))
+        s->march_b = value;
+    else if (!std::strcmp(key, "sweep_seg")) {
+        if (value < 1 || value > s->g.n[0]) return fail(MPX_ERR_INVALID, "sweep_seg must be in 1..nx");
+        s->sweep_seg = value;
+    } else if (!std::strcmp(key, "sweep_variant")) {
+        if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");
+        s->sweep_variant = value;
+    } else
+        return fail(MPX_ERR_NAME, std::string("unknown tuning key ") + key);
     return MPX_OK;
 }
 

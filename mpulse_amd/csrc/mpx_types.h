@@ -106,6 +106,35 @@ struct StepArgs {
     const T *Jt[3];
     int haveJ;
     int blo[3];                 // first updated B index per dimension (fused path)
+    int march;                  // fused path: x planes walked by one thread
+};
+
+// Single-pass sweep path: "o" = buffers of step n (read), "n" = buffers of step n+1 (written).
+template <typename T>
+struct SweepArgs {
+    GridDims g;
+    const T *Eo[3];
+    T *En[3];
+    const T *Bo[3];
+    T *Bn[3];
+    const double2 *psio[3][3];
+    double2 *psin[3][3];
+    const double *kdE[3];
+    const double *kdH[3];
+    double dt, chi, denom, numer;
+    PoleConsts pc;
+    int nefaces, nhfaces;
+    CpmlFace eface[kMaxFaces];       // psi0/psi1 = new E auxiliaries
+    const void *epsi_old[kMaxFaces][2];
+    int efbd[3][2];
+    CpmlFace hface[kMaxFaces];       // updated in place (owner cells only)
+    int hfbd[3][2];
+    int nsheets, nhsheets;
+    Sheet sheet[kMaxSheets];
+    Sheet hsheet[kMaxSheets];
+    int sheets_first;
+    int seg;       // x planes per block
+    int variant;   // thread-block shape
 };
 
 }  // namespace mpx

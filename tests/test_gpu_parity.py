@@ -72,3 +72,43 @@ def test_config_small_probes(path):
     err = rel_l2(ph, rh)
     print("probes", err)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK])
+@pytest.mark.parametrize("variant,seg", [(0, 5), (1, 7), (2, 64)])
+def test_sweep_path_matches_fused(prec, variant, seg):
+    """The single-pass sweep kernel against the two-kernel fused path on the same state,
+    grid sizes that leave partial tiles in every dimension."""
+    d, meta = configs.plrc_kerr_pulse(37, 5, 10.0, 6, 1e9, 50, d_src=8, nz=70, precision=prec)
+    d.ny = 33
+    d.factories[1].tshift = 12.0
+    d2, _ = configs.plrc_kerr_pulse(37, 5, 10.0, 6, 1e9, 50, d_src=8, nz=70, precision=prec, path=abi.PATH_SWEEP)
+    d2.ny = 33
+    d2.factories[1].tshift = 12.0
+    print(f"case: sweep variant {variant} seg {seg} prec {prec}", flush=True)
+    with abi.Solver(d) as a, abi.Solver(d2) as b:
+        b.tune("sweep_variant", variant)
+        b.tune("sweep_seg", seg)
+        a.step(50)
+        b.step(50)
+        names = FIELDS + ("PsiR1x", "PsiI1z")
+        for n in names:
+            x, y = b.get(n), a.get(n)
+            err = rel_l2(x, y)
+            assert err < (1e-13 if prec == abi.PREC_FP64 else 1e-6), (n, err)
+        for name in ("CPMLPsiEy", "CPMLPsiBz"):
+            for dr in (abi.EAST, abi.SOUTH, abi.UP):
+                la, _, _ = a.get_layer(name, dr)
+                lb, _, _ = b.get_layer(name, dr)
+                ia, ib = la[1:-1, 1:-1, 1:-1], lb[1:-1, 1:-1, 1:-1]
+                if np.abs(ia).max() > 0:
+                    assert rel_l2(ib, ia) < (1e-13 if prec == abi.PREC_FP64 else 1e-6), (name, dr)
+
+
+def test_sweep_path_against_oracle():
+    d, meta = configs.plrc_kerr_pulse(32, 6, 10.0, 8, 1e9, 60, d_src=9, path=abi.PATH_SWEEP)
+    d.factories[1].tshift = 15.0
+    g, r, _, _, _ = run_pair(d, 60, FIELDS + ("PsiR1x", "PsiI1y"))
+    for n in g:
+        err = rel_l2(g[n], r[n])
+        assert err < 1e-12, (n, err)
