@@ -269,3 +269,85 @@ def copy_probe(nbytes, reps=20):
     if rc != OK:
         raise MpxError(rc, lib.mpx_last_error().decode())
     return ms.value
+
+
+# ---------------------------------------------------------------------------------------
+# C++ host layer (setup files, FieldSolver plugins): libmpulse_host.so
+# ---------------------------------------------------------------------------------------
+
+HOST_LIB_PATH = os.path.join(HERE, "libmpulse_host.so")
+_host = None
+
+
+def load_host():
+    global _host
+    if _host is not None:
+        return _host
+    load()  # the host layer links the HIP library
+    if not os.path.exists(HOST_LIB_PATH):
+        raise ImportError(f"{HOST_LIB_PATH} is missing: build it with __graft_entry__.build()")
+    h = C.CDLL(HOST_LIB_PATH)
+    P = C.c_void_p
+    h.mph_parse_setup.restype = C.c_int
+    h.mph_parse_setup.argtypes = [C.c_char_p, C.POINTER(Desc), C.POINTER(C.c_int)]
+    h.mph_create.restype = C.c_int
+    h.mph_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(P)]
+    h.mph_run.restype = C.c_int
+    h.mph_run.argtypes = [P, C.c_int]
+    h.mph_solver.restype = P
+    h.mph_solver.argtypes = [P]
+    h.mph_destroy.restype = C.c_int
+    h.mph_destroy.argtypes = [P]
+    h.mph_last_error.restype = C.c_char_p
+    h.mph_last_error.argtypes = []
+    _host = h
+    return h
+
+
+def parse_setup(text):
+    """Descriptor and T-total of an MPulse setup file (no GPU work)."""
+    h = load_host()
+    d = Desc()
+    steps = C.c_int()
+    rc = h.mph_parse_setup(text.encode(), C.byref(d), C.byref(steps))
+    if rc != OK:
+        raise MpxError(rc, h.mph_last_error().decode())
+    return d, steps.value
+
+
+class HostSimulation:
+    """A setup file driven through the C++ Process / FieldSimulation / FieldSolver layer."""
+
+    def __init__(self, text, precision=PREC_FP64, path=PATH_FUSED):
+        self.h = load_host()
+        self.desc, self.total = parse_setup(text)
+        sim = C.c_void_p()
+        rc = self.h.mph_create(text.encode(), precision, path, C.byref(sim))
+        if rc != OK:
+            raise MpxError(rc, self.h.mph_last_error().decode())
+        self.sim = sim
+        self.shape = (self.desc.nx + 2, self.desc.ny + 2, self.desc.nz + 2)
+
+    def run(self, steps=-1):
+        rc = self.h.mph_run(self.sim, steps)
+        if rc != OK:
+            raise MpxError(rc, self.h.mph_last_error().decode())
+
+    def get(self, name):
+        lib = load()
+        out = np.empty(self.shape, dtype=np.float64)
+        rc = lib.mpx_get_field(self.h.mph_solver(self.sim), name.encode(), _dptr(out), out.size)
+        if rc != OK:
+            raise MpxError(rc, lib.mpx_last_error().decode())
+        return out
+
+    def close(self):
+        if getattr(self, "sim", None):
+            self.h.mph_destroy(self.sim)
+            self.sim = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

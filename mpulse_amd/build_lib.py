@@ -35,7 +35,31 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_LIB = os.path.join(HERE, "libmpulse_host.so")
+HOST_EXE = os.path.join(HERE, "mpulse_hip_run")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
+LINK = ["-L", HERE, "-lmpulse_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+
+
+def build_host(verbose=False):
+    """The C++ host layer (setup front-end, FieldSolver plugins, driver) and its CLI."""
+    host = os.path.join(HERE, "host")
+    cmd = [CXX, *HOST_FLAGS, "-shared", os.path.join(host, "mpulse_host.cpp"), "-o", HOST_LIB + ".tmp", *LINK]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    cmd = [CXX, *HOST_FLAGS, os.path.join(host, "mpulse_main.cpp"), "-o", HOST_EXE + ".tmp", "-L", HERE,
+           "-lmpulse_host", *LINK]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_EXE + ".tmp", HOST_EXE)
 
 
 if __name__ == "__main__":

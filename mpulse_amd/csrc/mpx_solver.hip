@@ -1579,8 +1579,8 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
     else if (!std::strcmp(key, "march_b"))
         s->march_b = value;
     else if (!std::strcmp(key, "sweep_seg")) {
-        if (value < 1 || value > s->g.n[0]) return fail(MPX_ERR_INVALID, "sweep_seg must be in 1..nx");
-        s->sweep_seg = value;
+        if (value < 1) return fail(MPX_ERR_INVALID, "sweep_seg must be >= 1");
+        s->sweep_seg = value < s->g.n[0] ? value : s->g.n[0];
     } else if (!std::strcmp(key, "sweep_variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");
         s->sweep_variant = value;

@@ -1,0 +1,636 @@
+// C++ host layer over the C ABI: setup-file front-end, FieldSolver plugins that keep their
+// state on the GPU, and the FieldSimulation / Process driver.
+#include "mpulse_host.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+
+namespace mpulse_hip {
+
+// ---------------------------------------------------------------------------------------
+// Rebuildable: the token grammar of src/rebuild.cpp:15-116 and src/parameter.t:1-27
+// ---------------------------------------------------------------------------------------
+
+static void skip_comment_lines(std::istream &in, std::string &tok)
+{
+    while (tok == "//") {
+        std::string rest;
+        std::getline(in, rest);
+        if (!(in >> tok)) {
+            tok.clear();
+            return;
+        }
+    }
+}
+
+std::string Rebuildable::Rebuild(std::istream &in)
+{
+    ParamMap pm;
+    MakeParamMap(pm);
+    std::string tok;
+    if (!(in >> tok)) return "";
+    while (tok != "}" && !tok.empty()) {
+        skip_comment_lines(in, tok);
+        if (tok.empty() || tok == "}") break;
+        auto it = pm.find(tok);
+        if (it == pm.end()) {
+            std::string name = tok;
+            if (!(in >> tok)) {
+                tok.clear();
+                break;
+            }
+            if (tok == "{") {
+                warnings.push_back("Unknown Task " + name + ". Skipping... ");
+                int level = 0;
+                do {
+                    if (!(in >> tok)) {
+                        tok.clear();
+                        break;
+                    }
+                    if (tok == "{") level++;
+                    if (tok == "}") level--;
+                } while (tok != "}" || level >= 0);
+                if (tok.empty()) {
+                    warnings.push_back("Unexpected end of file.");
+                    break;
+                }
+                if (!(in >> tok)) tok.clear();
+            } else {
+                // the value is not consumed: it is read as the next key, as in the reference
+                warnings.push_back("Unknown Parameter " + name);
+            }
+            continue;
+        }
+        Param &p = it->second;
+        if (p.kind == Param::INT) {
+            in >> *p.ip;
+            if (!(in >> tok)) tok.clear();
+        } else if (p.kind == Param::DOUBLE) {
+            in >> *p.dp;
+            if (!(in >> tok)) tok.clear();
+        } else {
+            Rebuildable *child = p.make();
+            p.sink(p.owner, child);
+            if (!(in >> tok)) {
+                tok.clear();
+                break;
+            }
+            if (tok == "{") tok = child->Rebuild(in);  // braces after an object key are optional
+        }
+    }
+    if (tok == "}") {
+        if (!(in >> tok)) tok.clear();
+    }
+    skip_comment_lines(in, tok);
+    return tok;
+}
+
+template <class T>
+static Rebuildable *make_new()
+{
+    return new T();
+}
+
+template <class Owner, class Base, class T>
+static Param pobject(Owner *owner, void (*sink)(void *, Rebuildable *))
+{
+    Param q;
+    q.kind = Param::OBJECT;
+    q.make = &make_new<T>;
+    q.sink = sink;
+    q.owner = owner;
+    return q;
+}
+
+// ---------------------------------------------------------------------------------------
+// parameter maps (keys and defaults of the reference's MakeParamMap)
+// ---------------------------------------------------------------------------------------
+
+void Globals::MakeParamMap(ParamMap &pm)
+{
+    pm["grid-x"] = pint(&GridX, 128);
+    pm["grid-y"] = pint(&GridY, 128);
+    pm["grid-z"] = pint(&GridZ, 128);
+    pm["grid-dx"] = pdouble(&GridDX, 0.1);
+    pm["grid-dy"] = pdouble(&GridDY, 0.1);
+    pm["grid-dz"] = pdouble(&GridDZ, 0.1);
+    pm["dt"] = pdouble(&DT, 1e-2);
+    pm["T-total"] = pint(&TotalTime, 100000);
+}
+
+void CPMLBorder::MakeParamMap(ParamMap &pm)
+{
+    pm["d"] = pint(&thickness, 8);
+    pm["kappaMax"] = pdouble(&kappaMax, 15);
+    pm["aMax"] = pdouble(&aMax, 0.25);
+    pm["sigmaMax"] = pdouble(&sigmaMax, 3);
+}
+
+mpx_factory CPMLBorder::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, kind());
+    f.d = thickness;
+    f.kappa_max = kappaMax;
+    f.a_max = aMax;
+    f.sigma_max = sigmaMax;
+    return f;
+}
+
+void IncidentSource::MakeParamMap(ParamMap &pm) { pm["d"] = pint(&distance, 15); }
+
+void ShortPulseInject::MakeParamMap(ParamMap &pm)
+{
+    IncidentSource::MakeParamMap(pm);
+    pm["length"] = pdouble(&length, 1.);
+    pm["width"] = pdouble(&width, 1.);
+    pm["om0"] = pdouble(&om0, 2 * 3.14159265358979323846);
+    pm["TShift"] = pdouble(&TShift, 0);
+    pm["ZShift"] = pdouble(&ZShift, 0);
+    pm["Phase"] = pdouble(&Phase, 0);
+    pm["amp"] = pdouble(&amp, 1);
+    pm["eps"] = pdouble(&eps, 1);
+}
+
+mpx_factory ShortPulseInject::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_SHORT_PULSE);
+    f.d = distance;
+    f.length = length;
+    f.width = width;
+    f.om0 = om0;
+    f.tshift = TShift;
+    f.zshift = ZShift;
+    f.phase = Phase;
+    f.amp = amp;
+    f.eps = eps;
+    return f;
+}
+
+void PlaneWaveSource::MakeParamMap(ParamMap &pm)
+{
+    IncidentSource::MakeParamMap(pm);
+    pm["kx"] = pdouble(&kx, 0);
+    pm["ky"] = pdouble(&ky, 0);
+    pm["kz"] = pdouble(&kz, 1);
+    pm["Hx"] = pdouble(&Hx, 0);
+    pm["Hy"] = pdouble(&Hy, 0);
+    pm["Hz"] = pdouble(&Hz, 0);
+    pm["Hx_bg"] = pdouble(&Hx_bg, 0);
+    pm["Hy_bg"] = pdouble(&Hy_bg, 0);
+    pm["Hz_bg"] = pdouble(&Hz_bg, 0);
+    pm["Ex_bg"] = pdouble(&Ex_bg, 0);
+    pm["Ey_bg"] = pdouble(&Ey_bg, 0);
+    pm["Ez_bg"] = pdouble(&Ez_bg, 0);
+    pm["ramp"] = pdouble(&ramp, 0.5);
+    pm["eps"] = pdouble(&eps, 1);
+}
+
+mpx_factory PlaneWaveSource::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_PLANE_WAVE);
+    f.d = distance;
+    f.k[0] = kx;
+    f.k[1] = ky;
+    f.k[2] = kz;
+    f.h[0] = Hx;
+    f.h[1] = Hy;
+    f.h[2] = Hz;
+    f.h_bg[0] = Hx_bg;
+    f.h_bg[1] = Hy_bg;
+    f.h_bg[2] = Hz_bg;
+    f.e_bg[0] = Ex_bg;
+    f.e_bg[1] = Ey_bg;
+    f.e_bg[2] = Ez_bg;
+    f.ramp = ramp;
+    f.eps = eps;
+    return f;
+}
+
+void PlaneGaussSource::MakeParamMap(ParamMap &pm)
+{
+    IncidentSource::MakeParamMap(pm);
+    pm["kx"] = pdouble(&kx, 0);
+    pm["ky"] = pdouble(&ky, 0);
+    pm["kz"] = pdouble(&kz, 1);
+    pm["Hx"] = pdouble(&Hx, 0);
+    pm["Hy"] = pdouble(&Hy, 0);
+    pm["Hz"] = pdouble(&Hz, 0);
+    pm["width"] = pdouble(&width, 10);
+    pm["offset"] = pdouble(&offset, 40);
+    pm["eps"] = pdouble(&eps, 1);
+}
+
+mpx_factory PlaneGaussSource::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_PLANE_GAUSS);
+    f.d = distance;
+    f.k[0] = kx;
+    f.k[1] = ky;
+    f.k[2] = kz;
+    f.h[0] = Hx;
+    f.h[1] = Hy;
+    f.h[2] = Hz;
+    f.gwidth = width;
+    f.goffset = offset;
+    f.eps = eps;
+    return f;
+}
+
+// A factory kind the GPU solver does not provide: building it is a setup error.
+template <const char *Name>
+struct Unsupported : CurrentFactory {
+    Unsupported() { throw SetupError(std::string(Name) + " is not provided by the GPU solver"); }
+    mpx_factory describe() const override { return mpx_factory(); }
+
+  protected:
+    void MakeParamMap(ParamMap &) override {}
+};
+static const char kPlasma[] = "plasma";
+static const char kPlasmaCurrent[] = "plasma_current";
+static const char kSideInject[] = "side_inject";
+static const char kGaussInject[] = "gauss_inject";
+static const char kFocused[] = "focused_pulse_inject";
+
+static void push_factory(void *owner, Rebuildable *r)
+{
+    static_cast<FDTD_PLRCSolverHIP *>(owner)->currentFactories.push_back(static_cast<CurrentFactory *>(r));
+}
+
+void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
+{
+    if (nonlin_) pm["chi"] = pdouble(&chi, 0.1);  // src/fdtd_plrc.cpp:461-465
+    pm["eps"] = pdouble(&eps, 1.0);
+    pm["L1eps"] = pdouble(&LEps[0], 0.0);
+    pm["L2eps"] = pdouble(&LEps[1], 0.0);
+    pm["L3eps"] = pdouble(&LEps[2], 0.0);
+    pm["L1delta"] = pdouble(&LDelta[0], 0.0);
+    pm["L2delta"] = pdouble(&LDelta[1], 0.0);
+    pm["L3delta"] = pdouble(&LDelta[2], 0.0);
+    pm["L1Om2"] = pdouble(&LOm2[0], 1.0);
+    pm["L2Om2"] = pdouble(&LOm2[1], 1.0);
+    pm["L3Om2"] = pdouble(&LOm2[2], 1.0);
+    pm["cpml_border"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, CPMLBorder>(this, push_factory);
+    pm["cpml_border_1d"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, CPMLBorderOneD>(this, push_factory);
+    pm["short_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, ShortPulseInject>(this, push_factory);
+    pm["wave_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlaneWaveSource>(this, push_factory);
+    pm["plane_gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlaneGaussSource>(this, push_factory);
+    pm["plasma"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kPlasma>>(this, push_factory);
+    pm["plasma_current"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kPlasmaCurrent>>(this, push_factory);
+    pm["side_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kSideInject>>(this, push_factory);
+    pm["gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kGaussInject>>(this, push_factory);
+    pm["focused_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kFocused>>(this, push_factory);
+}
+
+FDTD_PLRCSolverHIP::~FDTD_PLRCSolverHIP()
+{
+    for (CurrentFactory *f : currentFactories) delete f;
+}
+
+void FDTD_PLRCSolverHIP::describe(mpx_desc &d) const
+{
+    d.solver = nonlin_ ? MPX_SOLVER_PLRC_NONLIN : MPX_SOLVER_PLRC_LIN;
+    d.eps = eps;
+    d.chi = nonlin_ ? chi : 0.0;
+    for (int n = 0; n < 3; ++n) {
+        d.l_eps[n] = LEps[n];
+        d.l_delta[n] = LDelta[n];
+        d.l_om2[n] = LOm2[n];
+    }
+    d.n_factories = 0;
+    for (CurrentFactory *f : currentFactories) {
+        if (d.n_factories >= MPX_MAX_FACTORIES) throw SetupError("too many current factories");
+        d.factories[d.n_factories++] = f->describe();
+    }
+}
+
+void FDTD_SimpleHIP::MakeParamMap(ParamMap &pm)
+{
+    pm["eps"] = pdouble(&eps, 1.0);
+    if (solver_kind_ != MPX_SOLVER_PLAIN) pm["chi"] = pdouble(&chi, 0.1);
+    if (solver_kind_ == MPX_SOLVER_DISP) {
+        pm["L1eps2"] = pdouble(&LEps2[0], 0.0);
+        pm["L2eps2"] = pdouble(&LEps2[1], 0.0);
+        pm["L3eps2"] = pdouble(&LEps2[2], 0.0);
+        pm["L1Om2"] = pdouble(&LOm2[0], 0.0);
+        pm["L2Om2"] = pdouble(&LOm2[1], 0.0);
+        pm["L3Om2"] = pdouble(&LOm2[2], 0.0);
+    }
+}
+
+void FDTD_SimpleHIP::describe(mpx_desc &d) const
+{
+    d.solver = solver_kind_;
+    d.eps = eps;
+    d.chi = chi;
+    for (int n = 0; n < 3; ++n) {
+        d.d_eps2[n] = LEps2[n];
+        d.d_om2[n] = LOm2[n];
+    }
+    d.n_factories = 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// FieldSolver
+// ---------------------------------------------------------------------------------------
+
+FieldSolver::~FieldSolver()
+{
+    if (solver_) mpx_destroy(solver_);
+}
+
+void FieldSolver::addCurrent(Current *)
+{
+    // src/fieldsolver.h:15-23
+    throw SetupError("Additional currents cannot be used when using this Field Solver");
+}
+
+void FieldSolver::addMagCurrent(Current *)
+{
+    throw SetupError("Additional magnetic currents cannot be used when using this Field Solver");
+}
+
+static mpx_desc full_desc(const FieldSolver &s, const Globals &g, const Boundary *b)
+{
+    mpx_desc d;
+    mpx_desc_defaults(&d);
+    d.nx = g.GridX;
+    d.ny = g.GridY;
+    d.nz = g.GridZ;
+    d.dx = g.GridDX;
+    d.dy = g.GridDY;
+    d.dz = g.GridDZ;
+    d.dt = g.DT;
+    d.boundary = b ? b->kind() : MPX_BOUNDARY_PERIODIC;
+    d.precision = s.precision;
+    d.path = s.path;
+    s.describe(d);
+    return d;
+}
+
+void FieldSolver::initStorage(Storage *)
+{
+    if (!globals) throw SetupError("No globals specified! Must exit!");
+    mpx_desc d = full_desc(*this, *globals, boundary);
+    if (solver_) {
+        mpx_destroy(solver_);
+        solver_ = nullptr;
+    }
+    if (mpx_create(&d, &solver_) != MPX_OK) throw SetupError(mpx_last_error());
+}
+
+void FieldSolver::stepSchemeInit(double)
+{
+    if (mpx_step_init(solver_) != MPX_OK) throw SetupError(mpx_last_error());
+}
+
+void FieldSolver::stepScheme(double)
+{
+    // the sources read the driver's time index (Process::time, src/incsource.t:46)
+    if (time && mpx_set_time(solver_, *time) != MPX_OK) throw SetupError(mpx_last_error());
+    if (mpx_step(solver_, 1) != MPX_OK) throw SetupError(mpx_last_error());
+}
+
+// ---------------------------------------------------------------------------------------
+// Storage (host view)
+// ---------------------------------------------------------------------------------------
+
+std::vector<double> Storage::getGrid(const std::string &name) const
+{
+    const mpx_desc *unused = nullptr;
+    (void)unused;
+    Globals *g = solver->globals;
+    std::vector<double> v((size_t)(g->GridX + 2) * (g->GridY + 2) * (g->GridZ + 2));
+    if (mpx_get_field(solver->handle(), name.c_str(), v.data(), v.size()) != MPX_OK) throw SetupError(mpx_last_error());
+    return v;
+}
+
+std::vector<double> Storage::getBorderLayer(const std::string &name, int dir, int lo[3], int hi[3]) const
+{
+    if (mpx_get_layer(solver->handle(), name.c_str(), dir, nullptr, 0, lo, hi) != MPX_OK)
+        throw SetupError(mpx_last_error());
+    std::vector<double> v((size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1));
+    if (mpx_get_layer(solver->handle(), name.c_str(), dir, v.data(), v.size(), lo, hi) != MPX_OK)
+        throw SetupError(mpx_last_error());
+    return v;
+}
+
+double Storage::energy() const
+{
+    double e = 0;
+    if (mpx_energy(solver->handle(), &e) != MPX_OK) throw SetupError(mpx_last_error());
+    return solver->boundary ? solver->boundary->SumReduce(e) : e;
+}
+
+// ---------------------------------------------------------------------------------------
+// FieldSimulation / Process
+// ---------------------------------------------------------------------------------------
+
+static void set_solver(void *owner, Rebuildable *r)
+{
+    auto *fs = static_cast<FieldSimulation *>(owner);
+    delete fs->solver;
+    fs->solver = static_cast<FieldSolver *>(r);
+}
+static void set_boundary(void *owner, Rebuildable *r)
+{
+    auto *fs = static_cast<FieldSimulation *>(owner);
+    delete fs->boundary;
+    fs->boundary = static_cast<Boundary *>(r);
+}
+
+template <bool NL>
+static Rebuildable *make_plrc()
+{
+    return new FDTD_PLRCSolverHIP(NL);
+}
+template <int K>
+static Rebuildable *make_simple()
+{
+    return new FDTD_SimpleHIP(K);
+}
+
+void FieldSimulation::MakeParamMap(ParamMap &pm)
+{
+    // solver, boundary registries of src/fieldsim.cpp:177-214
+    auto obj = [&](Rebuildable *(*mk)(), void (*sink)(void *, Rebuildable *)) {
+        Param q;
+        q.kind = Param::OBJECT;
+        q.make = mk;
+        q.sink = sink;
+        q.owner = this;
+        return q;
+    };
+    pm["fdtd-plain"] = obj(&make_simple<MPX_SOLVER_PLAIN>, set_solver);
+    pm["fdtd-nl"] = obj(&make_simple<MPX_SOLVER_NL>, set_solver);
+    pm["fdtd-disp"] = obj(&make_simple<MPX_SOLVER_DISP>, set_solver);
+    pm["fdtd-plrc-lin"] = obj(&make_plrc<false>, set_solver);
+    pm["fdtd-plrc-nonlin"] = obj(&make_plrc<true>, set_solver);
+    pm["single_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
+    pm["single_xy_periodic"] = obj(&make_new<SingleXYPeriodicBoundary>, set_boundary);
+    // the MPI boundaries wrap the same periodic domain; on one GPU the whole domain is local
+    pm["mpi_slice_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
+    pm["mpi_block_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
+}
+
+FieldSimulation::~FieldSimulation()
+{
+    delete solver;
+    delete boundary;
+}
+
+void FieldSimulation::init(Globals *g, int *time)
+{
+    // src/fieldsim.cpp:31-107 (initialisers and diagnostics are not part of this path)
+    if (!solver) throw SetupError("No field solver specified");
+    if (!boundary) boundary = new SinglePeriodicBoundary();
+    dt = g->DT;
+    solver->globals = g;
+    solver->boundary = boundary;
+    solver->time = time;
+    storage.reset(new Storage(solver));
+    solver->initStorage(storage.get());
+}
+
+void FieldSimulation::execute() { solver->stepScheme(dt); }
+
+static void set_globals(void *owner, Rebuildable *r)
+{
+    auto *p = static_cast<Process *>(owner);
+    delete p->globals;
+    p->globals = static_cast<Globals *>(r);
+}
+static void set_fieldsim(void *owner, Rebuildable *r)
+{
+    auto *p = static_cast<Process *>(owner);
+    delete p->fieldSim;
+    p->fieldSim = static_cast<FieldSimulation *>(r);
+}
+
+void Process::MakeParamMap(ParamMap &pm)
+{
+    Param g;
+    g.kind = Param::OBJECT;
+    g.make = &make_new<Globals>;
+    g.sink = set_globals;
+    g.owner = this;
+    pm["globals"] = g;
+    Param f = g;
+    f.make = &make_new<FieldSimulation>;
+    f.sink = set_fieldsim;
+    pm["fieldsim"] = f;
+}
+
+Process::~Process()
+{
+    delete fieldSim;
+    delete globals;
+}
+
+std::string Process::RebuildAll(std::istream &in)
+{
+    // src/process.cpp:45-59
+    std::string t = Rebuild(in);
+    if (!globals) throw SetupError("No globals specified! Must exit!");
+    if (!fieldSim) throw SetupError("No field simulation specified! Must exit!");
+    return t;
+}
+
+void Process::init() { fieldSim->init(globals, &time); }
+
+void Process::run(int steps, bool verbose)
+{
+    // src/process.cpp:17-31
+    int T = steps < 0 ? globals->TotalTime : time + steps;
+    for (; time < T; ++time) {
+        if (verbose) std::cout << "Cycle " << time << std::endl << std::flush;
+        fieldSim->execute();
+    }
+}
+
+}  // namespace mpulse_hip
+
+// ---------------------------------------------------------------------------------------
+// C entry points
+// ---------------------------------------------------------------------------------------
+
+using namespace mpulse_hip;
+
+struct mph_sim {
+    Process process;
+};
+
+static thread_local std::string g_mph_err;
+
+static int mph_fail(const std::string &m)
+{
+    g_mph_err = m;
+    return MPX_ERR_INVALID;
+}
+
+extern "C" {
+
+const char *mph_last_error(void) { return g_mph_err.c_str(); }
+
+int mph_parse_setup(const char *text, mpx_desc *out, int *total_steps)
+{
+    if (!text || !out) return mph_fail("null argument");
+    try {
+        Process p;
+        std::istringstream in(text);
+        p.RebuildAll(in);
+        if (!p.fieldSim->solver) return mph_fail("No field solver specified");
+        *out = full_desc(*p.fieldSim->solver, *p.globals, p.fieldSim->boundary);
+        if (total_steps) *total_steps = p.globals->TotalTime;
+        return MPX_OK;
+    } catch (const SetupError &e) {
+        return mph_fail(e.msg);
+    }
+}
+
+int mph_create(const char *text, int precision, int path, mph_sim **out)
+{
+    if (!text || !out) return mph_fail("null argument");
+    *out = nullptr;
+    mph_sim *sim = new mph_sim();
+    try {
+        std::istringstream in(text);
+        sim->process.RebuildAll(in);
+        if (!sim->process.fieldSim->solver) throw SetupError("No field solver specified");
+        sim->process.fieldSim->solver->precision = precision;
+        sim->process.fieldSim->solver->path = path;
+        sim->process.init();
+    } catch (const SetupError &e) {
+        delete sim;
+        return mph_fail(e.msg);
+    }
+    *out = sim;
+    return MPX_OK;
+}
+
+int mph_run(mph_sim *sim, int steps)
+{
+    if (!sim) return mph_fail("null simulation");
+    try {
+        sim->process.run(steps, false);
+    } catch (const SetupError &e) {
+        return mph_fail(e.msg);
+    }
+    return MPX_OK;
+}
+
+mpx_solver *mph_solver(mph_sim *sim) { return sim ? sim->process.fieldSim->solver->handle() : nullptr; }
+
+int mph_destroy(mph_sim *sim)
+{
+    delete sim;
+    return MPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+// C++ host layer over the C ABI of include/mpulse_hip.h.
+//
+// It mirrors the reference's plugin interfaces for the time-stepping path -- FieldSolver
+// (src/fieldsolver.h:10-37), CurrentFactory (src/currentfactory.h:10-15), IncidentSource
+// (src/incsource.h:16-31), Boundary (src/boundary.h:32-85), the FieldSimulation / Process
+// driver (src/fieldsim.cpp:31-112, src/process.cpp:12-31) and the setup-file grammar of
+// Rebuildable (src/rebuild.cpp:15-116) -- with the same registry keys, parameter keys and
+// defaults.  The solvers keep their state on the GPU; stepScheme calls mpx_step.
+#pragma once
+
+#include <iosfwd>
+#include <list>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mpulse_hip.h"
+
+namespace mpulse_hip {
+
+// ---------------------------------------------------------------------------------------
+// setup-file parameters: "key value" pairs and "name { ... }" blocks, "//" line comments
+// ---------------------------------------------------------------------------------------
+
+class Rebuildable;
+
+// Reads one value of a parameter, or builds one child object.
+struct Param {
+    enum Kind { INT, DOUBLE, OBJECT } kind;
+    int *ip = nullptr;
+    double *dp = nullptr;
+    // OBJECT: factory making the child; the child is handed to `sink`
+    Rebuildable *(*make)() = nullptr;
+    void (*sink)(void *owner, Rebuildable *child) = nullptr;
+    void *owner = nullptr;
+};
+using ParamMap = std::map<std::string, Param>;
+
+class Rebuildable {
+  public:
+    virtual ~Rebuildable() = default;
+    // Rebuildable::Rebuild semantics: read tokens until "}" or end of input; unknown keys are
+    // reported and skipped (blocks included); returns the next token after the block.
+    std::string Rebuild(std::istream &in);
+    std::vector<std::string> warnings;
+
+  protected:
+    virtual void MakeParamMap(ParamMap &pm) = 0;
+    static Param pint(int *p, int def)
+    {
+        *p = def;
+        Param q;
+        q.kind = Param::INT;
+        q.ip = p;
+        return q;
+    }
+    static Param pdouble(double *p, double def)
+    {
+        *p = def;
+        Param q;
+        q.kind = Param::DOUBLE;
+        q.dp = p;
+        return q;
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// globals, storage, boundary
+// ---------------------------------------------------------------------------------------
+
+class Globals : public Rebuildable {  // src/globals.cpp:21-48
+  public:
+    int GridX = 128, GridY = 128, GridZ = 128;
+    double GridDX = 0.1, GridDY = 0.1, GridDZ = 0.1, DT = 1e-2;
+    int TotalTime = 100000;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+class Boundary : public Rebuildable {  // src/boundary.h:32-85 (single GPU: identity reductions)
+  public:
+    virtual int kind() const = 0;  // MPX_BOUNDARY_*
+    double AvgReduce(double v) const { return v; }
+    double SumReduce(double v) const { return v; }
+    double MaxReduce(double v) const { return v; }
+    bool master() const { return true; }
+    int procnum() const { return 0; }
+    int getUniqueId() const { return 0; }
+
+  protected:
+    void MakeParamMap(ParamMap &) override {}
+};
+class SinglePeriodicBoundary : public Boundary {
+  public:
+    int kind() const override { return MPX_BOUNDARY_PERIODIC; }
+};
+class SingleXYPeriodicBoundary : public Boundary {
+  public:
+    int kind() const override { return MPX_BOUNDARY_XY_PERIODIC; }
+};
+
+class FieldSolver;
+
+// Host view of the state the solver keeps on the GPU (Storage, src/storage.h:15-116):
+// grids are fetched by reference name, z fastest over 0..N+1.
+class Storage {
+  public:
+    explicit Storage(FieldSolver *s) : solver(s) {}
+    std::vector<double> getGrid(const std::string &name) const;
+    std::vector<double> getBorderLayer(const std::string &name, int dir, int lo[3], int hi[3]) const;
+    double energy() const;
+    FieldSolver *solver;
+};
+
+// ---------------------------------------------------------------------------------------
+// current factories: translated into device descriptors
+// ---------------------------------------------------------------------------------------
+
+class CurrentFactory : public Rebuildable {  // src/currentfactory.h:10-15
+  public:
+    // the descriptor the GPU solver builds this factory's currents from
+    virtual mpx_factory describe() const = 0;
+};
+
+class CPMLBorder : public CurrentFactory {  // src/cpml_border.cpp:141-151
+  public:
+    int thickness = 8;
+    double kappaMax = 15, aMax = 0.25, sigmaMax = 3;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    virtual int kind() const { return MPX_FACTORY_CPML; }
+};
+class CPMLBorderOneD : public CPMLBorder {  // src/cpml_border.cpp:245-255
+  protected:
+    int kind() const override { return MPX_FACTORY_CPML_1D; }
+};
+
+class IncidentSource : public CurrentFactory {  // src/incsource.cpp:80-86
+  public:
+    int distance = 15;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+class ShortPulseInject : public IncidentSource {  // src/shortpulseinject.cpp:35-50
+  public:
+    double length = 1, width = 1, om0 = 6.283185307179586, TShift = 0, ZShift = 0, Phase = 0, amp = 1, eps = 1;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+class PlaneWaveSource : public IncidentSource {  // src/sources.cpp:58-82
+  public:
+    double kx = 0, ky = 0, kz = 1, Hx = 0, Hy = 0, Hz = 0, Hx_bg = 0, Hy_bg = 0, Hz_bg = 0;
+    double Ex_bg = 0, Ey_bg = 0, Ez_bg = 0, ramp = 0.5, eps = 1;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+class PlaneGaussSource : public IncidentSource {  // src/sources.cpp:225-242
+  public:
+    double kx = 0, ky = 0, kz = 1, Hx = 0, Hy = 0, Hz = 0, width = 10, offset = 40, eps = 1;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+// ---------------------------------------------------------------------------------------
+// field solvers
+// ---------------------------------------------------------------------------------------
+
+class Current;  // host-computed currents are not accepted by the GPU solvers
+
+class FieldSolver : public Rebuildable {  // src/fieldsolver.h:10-37
+  public:
+    ~FieldSolver() override;
+    virtual void initStorage(Storage *storage);
+    virtual void addCurrent(Current *);     // not supported: prints the reference's message, exits
+    virtual void addMagCurrent(Current *);  // idem
+    virtual void stepSchemeInit(double dt);
+    virtual void stepScheme(double dt);
+
+    // GPU execution choices (not setup-file keys of the reference)
+    int precision = MPX_PREC_FP64;
+    int path = MPX_PATH_FUSED;
+    mpx_solver *handle() const { return solver_; }
+    // fills the descriptor this solver hands to mpx_create
+    virtual void describe(mpx_desc &d) const = 0;
+
+    Globals *globals = nullptr;
+    Boundary *boundary = nullptr;
+    int *time = nullptr;  // Process::time
+
+  protected:
+    void MakeParamMap(ParamMap &) override {}
+    mpx_solver *solver_ = nullptr;
+};
+
+// FDTD_PLRCSolver<FDTD_PLRC{Lin,Nonlin}Core> (src/fdtd_plrc.t:411-473, src/fdtd_plrc.cpp:461-465)
+class FDTD_PLRCSolverHIP : public FieldSolver {
+  public:
+    explicit FDTD_PLRCSolverHIP(bool nonlin) : nonlin_(nonlin) {}
+    double eps = 1, chi = 0.1;
+    double LEps[3] = {0, 0, 0}, LDelta[3] = {0, 0, 0}, LOm2[3] = {1, 1, 1};
+    std::list<CurrentFactory *> currentFactories;
+    ~FDTD_PLRCSolverHIP() override;
+    void describe(mpx_desc &d) const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+
+  private:
+    bool nonlin_;
+};
+
+// FDTD_Plain / FDTD_Nonlinear / FDTD_Dispersion (src/fdtd_plain.h, src/fdtd_nl.cpp:146-153,
+// src/fdtd_disp.cpp:252-272)
+class FDTD_SimpleHIP : public FieldSolver {
+  public:
+    explicit FDTD_SimpleHIP(int solver) : solver_kind_(solver) {}
+    double eps = 1, chi = 0.1, LEps2[3] = {0, 0, 0}, LOm2[3] = {0, 0, 0};
+    void describe(mpx_desc &d) const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+
+  private:
+    int solver_kind_;
+};
+
+// ---------------------------------------------------------------------------------------
+// driver
+// ---------------------------------------------------------------------------------------
+
+class FieldSimulation : public Rebuildable {  // src/fieldsim.cpp:31-194
+  public:
+    ~FieldSimulation() override;
+    FieldSolver *solver = nullptr;
+    Boundary *boundary = nullptr;
+    std::unique_ptr<Storage> storage;
+    void init(Globals *g, int *time);
+    void execute();
+    double dt = 0;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+class Process : public Rebuildable {  // src/process.cpp:12-59
+  public:
+    ~Process() override;
+    Globals *globals = nullptr;
+    FieldSimulation *fieldSim = nullptr;
+    int time = 0;
+    std::string RebuildAll(std::istream &in);  // Process::Rebuild with its checks
+    void init();
+    void run(int steps = -1, bool verbose = true);  // steps < 0: T-total
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+// Fatal-error convention of the reference (print, exit(-1)) as an exception the C entry
+// points turn into a status code.
+struct SetupError : std::exception {
+    std::string msg;
+    explicit SetupError(std::string m) : msg(std::move(m)) {}
+    const char *what() const noexcept override { return msg.c_str(); }
+};
+
+}  // namespace mpulse_hip
+
+extern "C" {
+// Parse a setup text into the descriptor the GPU solver is created from (no GPU needed).
+int mph_parse_setup(const char *text, mpx_desc *out, int *total_steps);
+typedef struct mph_sim mph_sim;
+// Parse, create the GPU solver (FieldSimulation::init) and run `steps` (T-total if < 0).
+int mph_create(const char *text, int precision, int path, mph_sim **out);
+int mph_run(mph_sim *sim, int steps);
+mpx_solver *mph_solver(mph_sim *sim);
+int mph_destroy(mph_sim *sim);
+const char *mph_last_error(void);
+}

@@ -1,0 +1,79 @@
+"""The C++ host layer: setup-file grammar (src/rebuild.cpp:15-116) and registries
+(src/fieldsim.cpp:114-194, src/fdtd_plrc.t:411-473) produce the descriptor of the workload
+(CPU only), and a setup file runs through FieldSolver on the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+from mpulse_amd import abi, configs
+
+SETUP_SMALL = """
+// 64^3, one Lorentz pole + Kerr, 10-cell CPML, short pulse from z = 12
+globals { grid-x 64 grid-y 64 grid-z 64 grid-dx 0.3141592653589793 grid-dy 0.3141592653589793
+          grid-dz 0.3141592653589793 dt 0.09068996821171089 T-total 400 }
+fieldsim {
+  fdtd-plrc-nonlin { eps 1 chi 2e-4  L1eps 1.1 L1Om2 4 L1delta 0.005
+                     L2eps 0 L2Om2 1 L3eps 0 L3Om2 1
+                     cpml_border { d 10 kappaMax 15 aMax 0.25 sigmaMax 8.795 }
+                     short_pulse_inject { d 12 length 20.0 width 4.71238898038469 om0 1 amp 1
+                                          eps 2.4666666666666668 TShift 40 ZShift -6.440264939859076 } }
+  single_periodic
+  energy { interval 10 }
+}
+"""
+
+
+def test_setup_file_to_descriptor():
+    d, total = abi.parse_setup(SETUP_SMALL)
+    assert total == 400
+    assert (d.nx, d.ny, d.nz) == (64, 64, 64)
+    assert d.solver == abi.SOLVER_PLRC_NONLIN and d.boundary == abi.BOUNDARY_PERIODIC
+    assert d.chi == 2e-4 and d.l_eps[0] == 1.1 and d.l_om2[0] == 4 and d.l_delta[0] == 0.005
+    assert d.l_om2[1] == 1 and d.l_eps[1] == 0
+    assert d.n_factories == 2
+    cp, sp = d.factories[0], d.factories[1]
+    assert cp.kind == abi.FACTORY_CPML and cp.d == 10 and cp.sigma_max == 8.795
+    assert sp.kind == abi.FACTORY_SHORT_PULSE and sp.d == 12 and sp.tshift == 40 and sp.length == 20.0
+    ref, _ = configs.config_small()
+    assert abs(sp.zshift - ref.factories[1].zshift) < 1e-12
+
+
+def test_defaults_and_optional_braces():
+    """Keys left out take the reference's MakeParamMap defaults; an object key without braces
+    builds the object with defaults (src/parameter.t:14-27)."""
+    d, total = abi.parse_setup("globals { } fieldsim { fdtd-plrc-lin { cpml_border } single_xy_periodic }")
+    assert (d.nx, d.dx, d.dt, total) == (128, 0.1, 1e-2, 100000)
+    assert d.solver == abi.SOLVER_PLRC_LIN and d.boundary == abi.BOUNDARY_XY_PERIODIC
+    assert d.eps == 1.0 and list(d.l_om2) == [1.0, 1.0, 1.0]
+    f = d.factories[0]
+    assert d.n_factories == 1 and (f.d, f.kappa_max, f.a_max, f.sigma_max) == (8, 15, 0.25, 3)
+    d, _ = abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { short_pulse_inject { } } }")
+    f = d.factories[0]
+    assert (f.d, f.length, f.width, f.amp, f.eps) == (15, 1.0, 1.0, 1.0, 1.0)
+    assert abs(f.om0 - 2 * math.pi) < 1e-15
+    assert d.chi == 0.1
+
+
+def test_setup_errors_follow_the_reference():
+    with pytest.raises(abi.MpxError, match="No globals specified"):
+        abi.parse_setup("fieldsim { fdtd-plain { } }")
+    with pytest.raises(abi.MpxError, match="No field simulation specified"):
+        abi.parse_setup("globals { grid-x 8 }")
+    with pytest.raises(abi.MpxError, match="focused_pulse_inject is not provided"):
+        abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { focused_pulse_inject { } } }")
+
+
+@pytest.mark.gpu
+def test_setup_file_runs_like_the_descriptor_path():
+    """The same workload through the C++ FieldSolver layer and through mpx_create directly."""
+    text = SETUP_SMALL.replace("T-total 400", "T-total 40")
+    sim = abi.HostSimulation(text)
+    sim.run()
+    d, _ = abi.parse_setup(text)
+    with abi.Solver(d) as s:
+        s.step(40)
+        for n in ("Ex", "By", "Ez"):
+            a, b = sim.get(n), s.get(n)
+            assert np.array_equal(a, b), n
+    sim.close()
