@@ -75,6 +75,11 @@ extern "C" {
 #define MPX_FACTORY_SHORT_PULSE   3  /* "short_pulse_inject"   src/shortpulseinject.cpp      */
 #define MPX_FACTORY_PLANE_WAVE    4  /* "wave_inject"          src/sources.cpp:8-173         */
 #define MPX_FACTORY_PLANE_GAUSS   5  /* "plane_gauss_inject"   src/sources.cpp:179-331       */
+#define MPX_FACTORY_FOCUSED_PULSE 6  /* "focused_pulse_inject" keys of src/focusedpulseinject.cpp:38-56;
+                                        the field model is this library's own (the reference's does
+                                        not compile): x-polarised Gaussian pupil of waist "width",
+                                        non-paraxial angular spectrum at om0, Gaussian envelope of
+                                        duration "length", injected on the down face */
 
 typedef struct mpx_factory {
     int    kind;          /* MPX_FACTORY_* */

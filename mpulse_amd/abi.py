@@ -21,6 +21,7 @@ BOUNDARY_PERIODIC, BOUNDARY_XY_PERIODIC = 0, 1
 PREC_FP64, PREC_FP32_BULK = 0, 1
 PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
+FACTORY_FOCUSED_PULSE = 6
 
 # Direction enum of the reference (src/mpulse.h:28)
 NORTH, SOUTH, WEST, EAST, UP, DOWN = 0, 1, 2, 3, 4, 5

@@ -88,6 +88,25 @@ __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables 
             double posTime = a.time * P.DT - P.TShift;
             f[0] = P.amp * pulse_efunc(P, ft, posxh, posyo, poszo, posTime).re;
         }
+    } else if (a.kind == 3) {
+        // amp Re[env(t - z/v) U exp(-i(om0 t - phase))], env(s) = exp(-s^2/T^2)
+        const FocusedParams &F = a.focused;
+        size_t p = (size_t)i * a.ny2 + j;
+        double t = a.isH ? a.time * F.DT - F.TShift : (a.time - 0.5) * F.DT - F.TShift;
+        double z = a.isH ? F.zE : F.zB;
+        double s = t - z / F.v;
+        double env = F.amp * exp(-(s * s) / (F.T * F.T));
+        double sn, cs;
+        sincos(-(F.om0 * t - F.phase), &sn, &cs);
+        if (a.isH) {
+            double2 u = F.ex[p];
+            f[0] = env * (u.x * cs - u.y * sn);
+        } else {
+            double2 u = F.bx[p], w = F.by[p];
+            f[0] = env * (u.x * cs - u.y * sn);
+            f[1] = env * (w.x * cs - w.y * sn);
+        }
+        (void)z;
     } else {
         if (!a.isH)
             plane_getH(a.plane, i, j, z, a.time, f);

@@ -393,6 +393,60 @@ namespace mpx {
 double pulse_norm_device(const PulseParams &P, const FaddeevaTables &ft, hipStream_t s);
 }
 
+// Focused pulse model (the library's own; the reference's does not compile): an x-polarised
+// Gaussian pupil A(k) = exp(-(kx^2+ky^2) w0^2/4) at the carrier om0 with vector plane waves
+// E(k) = A (1, 0, -kx/kz), B(k) = k x E / om0, kz = sqrt(eps om0^2 - k_perp^2) (evanescent
+// waves dropped), summed over the periodic transverse grid and normalised to |E_x| = 1 at
+// the focus.  comp: 0 = E_x, 1 = B_x, 2 = B_y; (sx, sy) = Yee offsets of the component.
+static void focused_plane(const mpx_factory &f, int nx, int ny, double dx, double dy, double z, int comp, double sx,
+                          double sy, std::vector<double2> &out)
+{
+    typedef std::complex<double> cplx;
+    double k = std::sqrt(f.eps) * f.om0, w0 = f.width;
+    double cx = 0.5 * (nx + 1), cy = 0.5 * (ny + 1);
+    int nx2 = nx + 2, ny2 = ny + 2;
+    std::vector<double> kxs(nx), kys(ny);
+    for (int m = 0; m < nx; ++m) kxs[m] = 2 * M_PI * (m - nx / 2) / (nx * dx);
+    for (int m = 0; m < ny; ++m) kys[m] = 2 * M_PI * (m - ny / 2) / (ny * dy);
+    // spectrum G(kx, ky) at plane z, and the normalisation sum of A
+    std::vector<cplx> G((size_t)nx * ny);
+    double norm = 0;
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b) {
+            double kp2 = kxs[a] * kxs[a] + kys[b] * kys[b];
+            if (kp2 >= k * k) continue;
+            double kz = std::sqrt(k * k - kp2), A = std::exp(-kp2 * w0 * w0 / 4);
+            norm += A;
+            double g = comp == 0 ? A : (comp == 1 ? -kxs[a] * kys[b] * A / (kz * f.om0) : A * (kz * kz + kxs[a] * kxs[a]) / (kz * f.om0));
+            G[(size_t)a * ny + b] = g * std::exp(cplx(0, kz * z));
+        }
+    // separable synthesis with phasor tables: over ky, then over kx
+    std::vector<cplx> ey((size_t)ny * ny2), ex((size_t)nx * nx2);
+    for (int b = 0; b < ny; ++b)
+        for (int j = 0; j < ny2; ++j) ey[(size_t)b * ny2 + j] = std::exp(cplx(0, kys[b] * (j + sy - cy) * dy));
+    for (int a = 0; a < nx; ++a)
+        for (int i = 0; i < nx2; ++i) ex[(size_t)a * nx2 + i] = std::exp(cplx(0, kxs[a] * (i + sx - cx) * dx));
+    std::vector<cplx> tmp((size_t)nx * ny2);
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b) {
+            cplx g = G[(size_t)a * ny + b];
+            if (g == cplx(0, 0)) continue;
+            cplx *row = &tmp[(size_t)a * ny2];
+            const cplx *ph = &ey[(size_t)b * ny2];
+            for (int j = 0; j < ny2; ++j) row[j] += g * ph[j];
+        }
+    std::vector<cplx> res((size_t)nx2 * ny2);
+    for (int a = 0; a < nx; ++a)
+        for (int i = 0; i < nx2; ++i) {
+            cplx ph = ex[(size_t)a * nx2 + i];
+            const cplx *row = &tmp[(size_t)a * ny2];
+            cplx *dst = &res[(size_t)i * ny2];
+            for (int j = 0; j < ny2; ++j) dst[j] += ph * row[j];
+        }
+    out.resize((size_t)nx2 * ny2);
+    for (size_t q = 0; q < out.size(); ++q) out[q] = make_double2(res[q].real() / norm, res[q].imag() / norm);
+}
+
 // ---------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------
@@ -585,7 +639,7 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                 else
                     cpml_kappa(s, f.d, f.kappa_max);
             } else if (f.kind == MPX_FACTORY_SHORT_PULSE || f.kind == MPX_FACTORY_PLANE_WAVE ||
-                       f.kind == MPX_FACTORY_PLANE_GAUSS) {
+                       f.kind == MPX_FACTORY_PLANE_GAUSS || f.kind == MPX_FACTORY_FOCUSED_PULSE) {
                 if (ncpml == 0) s->sheets_first = 1;
                 // needCurrent(down) only: src/shortpulseinject.cpp:28-31, src/sources.cpp:8-11
                 for (int h = 0; h < 2; ++h) {
@@ -677,7 +731,36 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     ev.ny2 = g.n[1] + 2;
                     ev.sheet.kp = s->layers[l0].lo[2];
                     for (int w = 0; w < 3; ++w) ev.sheet.J[w] = (double *)s->layers[lc[w]].comp[0];
-                    if (f.kind == MPX_FACTORY_SHORT_PULSE) {
+                    if (f.kind == MPX_FACTORY_FOCUSED_PULSE) {
+                        ev.kind = 3;
+                        if (!(f.eps > 0) || !(f.width > 0) || !(f.length != 0) || !(f.om0 > 0))
+                            return fail(MPX_ERR_INVALID, "focused_pulse_inject: eps, width, om0 must be > 0, length != 0");
+                        mpx::FocusedParams &F = ev.focused;
+                        double cz = 0.5 * (g.n[2] + 1);
+                        int kk = f.d - 1;  // the plane both currents sample (down face)
+                        F.zE = (kk - cz) * desc->dz - f.zshift;
+                        F.zB = (kk - 0.5 - cz) * desc->dz - f.zshift;
+                        F.T = f.length;
+                        F.om0 = f.om0;
+                        F.v = std::sqrt(1 / f.eps);
+                        F.TShift = f.tshift;
+                        F.phase = 2 * M_PI * f.phase;
+                        F.amp = f.amp;
+                        F.DT = desc->dt;
+                        std::vector<double2> pl;
+                        double2 *dp[3];
+                        for (int w = 0; w < 3; ++w) {
+                            focused_plane(f, g.n[0], g.n[1], desc->dx, desc->dy, w == 0 ? F.zE : F.zB, w,
+                                          w == 1 ? 0.0 : 0.5, w == 1 ? 0.5 : 0.0, pl);
+                            dp[w] = s->dalloc<double2>(pl.size(), rc);
+                            if (rc) return rc;
+                            if (upload(s, dp[w], pl.data(), pl.size() * sizeof(double2)))
+                                return fail(MPX_ERR_HIP, "focused-pulse plane upload failed");
+                        }
+                        F.ex = dp[0];
+                        F.bx = dp[1];
+                        F.by = dp[2];
+                    } else if (f.kind == MPX_FACTORY_SHORT_PULSE) {
                         ev.kind = 1;
                         // ShortPulseInjectSourceFunc::setParam, src/shortpulseinject.cpp:56-113
                         mpx::PulseParams &P = ev.pulse;

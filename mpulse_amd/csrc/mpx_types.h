@@ -62,8 +62,15 @@ struct PlaneParams {
     double ramp, dt, om, dx, dy, dz, width, offset;
 };
 
+// Focused pulse: the transverse incident fields of one carrier-frequency angular spectrum,
+// precomputed on the two sampled planes ((nx+2) x (ny+2), index i*(ny+2)+j, (re, im)).
+struct FocusedParams {
+    const double2 *ex, *bx, *by;
+    double zE, zB, T, om0, v, TShift, phase, amp, DT;
+};
+
 struct SheetEval {
-    int kind;        // 1 short pulse, 2 plane wave / gauss
+    int kind;        // 1 short pulse, 2 plane wave / gauss, 3 focused pulse
     int isH;         // 0: E current (samples H), 1: H current (samples E)
     int off;         // offset of the sampled cell along z (IncidentSource*Current::stepScheme)
     int t1, t2;      // transverse components
@@ -73,6 +80,7 @@ struct SheetEval {
     Sheet sheet;
     PulseParams pulse;
     PlaneParams plane;
+    FocusedParams focused;
 };
 
 // PLRC pole constants (PLRCData, src/fdtd_plrc.h:71-78) of the active poles, precomputed on the

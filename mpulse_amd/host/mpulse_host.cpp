@@ -171,6 +171,37 @@ mpx_factory ShortPulseInject::describe() const
     return f;
 }
 
+void FocusedPulseInject::MakeParamMap(ParamMap &pm)
+{
+    IncidentSource::MakeParamMap(pm);
+    pm["length"] = pdouble(&length, 1.);
+    pm["width"] = pdouble(&width, 1.);
+    pm["om0"] = pdouble(&om0, 2 * 3.14159265358979323846);
+    pm["TShift"] = pdouble(&TShift, 0);
+    pm["ZShift"] = pdouble(&ZShift, 0);
+    pm["Phase"] = pdouble(&Phase, 0);
+    pm["OversampleX"] = pint(&OversampleX, 1);
+    pm["OversampleY"] = pint(&OversampleY, 1);
+    pm["amp"] = pdouble(&amp, 1);
+    pm["eps"] = pdouble(&eps, 1);
+}
+
+mpx_factory FocusedPulseInject::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_FOCUSED_PULSE);
+    f.d = distance;
+    f.length = length;
+    f.width = width;
+    f.om0 = om0;
+    f.tshift = TShift;
+    f.zshift = ZShift;
+    f.phase = Phase;
+    f.amp = amp;
+    f.eps = eps;
+    return f;
+}
+
 void PlaneWaveSource::MakeParamMap(ParamMap &pm)
 {
     IncidentSource::MakeParamMap(pm);
@@ -256,7 +287,6 @@ static const char kPlasma[] = "plasma";
 static const char kPlasmaCurrent[] = "plasma_current";
 static const char kSideInject[] = "side_inject";
 static const char kGaussInject[] = "gauss_inject";
-static const char kFocused[] = "focused_pulse_inject";
 
 static void push_factory(void *owner, Rebuildable *r)
 {
@@ -285,7 +315,7 @@ void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
     pm["plasma_current"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kPlasmaCurrent>>(this, push_factory);
     pm["side_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kSideInject>>(this, push_factory);
     pm["gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kGaussInject>>(this, push_factory);
-    pm["focused_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kFocused>>(this, push_factory);
+    pm["focused_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, FocusedPulseInject>(this, push_factory);
 }
 
 FDTD_PLRCSolverHIP::~FDTD_PLRCSolverHIP()

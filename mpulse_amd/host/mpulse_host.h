@@ -156,6 +156,19 @@ class ShortPulseInject : public IncidentSource {  // src/shortpulseinject.cpp:35
     void MakeParamMap(ParamMap &pm) override;
 };
 
+// focused_pulse_inject keys of src/focusedpulseinject.cpp:38-56; the field model is the
+// library's own (see include/mpulse_hip.h, MPX_FACTORY_FOCUSED_PULSE).  The oversampling keys
+// are accepted: the angular spectrum is summed on the domain's own periodic grid.
+class FocusedPulseInject : public IncidentSource {
+  public:
+    double length = 1, width = 1, om0 = 6.283185307179586, TShift = 0, ZShift = 0, Phase = 0, amp = 1, eps = 1;
+    int OversampleX = 1, OversampleY = 1;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
 class PlaneWaveSource : public IncidentSource {  // src/sources.cpp:58-82
   public:
     double kx = 0, ky = 0, kz = 1, Hx = 0, Hy = 0, Hz = 0, Hx_bg = 0, Hy_bg = 0, Hz_bg = 0;

@@ -526,11 +526,117 @@ static void plane_getE(const plane_t *W, int i, int j, int l, int time, double o
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* Focused pulse (focused_pulse_inject keys; the field model is the library's own design)  */
+/* ------------------------------------------------------------------------------------ */
+
+/* x-polarised Gaussian pupil A(k) = exp(-(kx^2+ky^2) w0^2/4) at the carrier om0, vector plane
+ * waves E(k) = A (1, 0, -kx/kz), B(k) = k x E / om0 (kz = sqrt(eps om0^2 - k_perp^2), evanescent
+ * waves dropped), summed on the periodic transverse grid of the domain; the field is
+ * amp Re[ env(t - z/v) U(x, y, z) exp(-i (om0 t - phase)) ] with env(s) = exp(-s^2/T^2),
+ * z measured from the focus, v = 1/sqrt(eps) and U normalised to |U| = 1 at the focus.
+ * The planes are sampled where the short-pulse source samples its fields (E_x at
+ * (i+1/2, j, k), B_x at (i, j+1/2, k-1/2), B_y at (i+1/2, j, k-1/2), z index d-1). */
+typedef struct {
+    int nx2, ny2;
+    double *ex, *bx, *by; /* interleaved re, im per (i, j) */
+    double zE, zB, T, om0, v, TShift, phase, amp, DT;
+} focused_t;
+
+static void focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy, double z, int comp,
+                          double sx, double sy, double *out)
+{
+    double n = sqrt(f->eps), k = n * f->om0, w0 = f->width;
+    double cx = 0.5 * (nx + 1), cy = 0.5 * (ny + 1);
+    int nx2 = nx + 2, ny2 = ny + 2;
+    /* inner sum over ky for every (kx, y_j) */
+    double complex *tmp = (double complex *)calloc((size_t)nx * ny2, sizeof(double complex));
+    for (int mx = 0; mx < nx; ++mx) {
+        double kx = 2 * M_PI * (mx - nx / 2) / (nx * dx);
+        for (int j = 0; j < ny2; ++j) {
+            double y = (j + sy - cy) * dy;
+            double complex acc = 0;
+            for (int my = 0; my < ny; ++my) {
+                double ky = 2 * M_PI * (my - ny / 2) / (ny * dy);
+                double kp2 = kx * kx + ky * ky;
+                if (kp2 >= k * k) continue;
+                double kz = sqrt(k * k - kp2);
+                double A = exp(-kp2 * w0 * w0 / 4);
+                double g = comp == 0 ? A : (comp == 1 ? -kx * ky * A / (kz * f->om0) : A * (kz * kz + kx * kx) / (kz * f->om0));
+                acc += g * cexp(CX(0, ky * y + kz * z));
+            }
+            tmp[(size_t)mx * ny2 + j] = acc;
+        }
+    }
+    double norm = 0;
+    for (int mx = 0; mx < nx; ++mx)
+        for (int my = 0; my < ny; ++my) {
+            double kx = 2 * M_PI * (mx - nx / 2) / (nx * dx), ky = 2 * M_PI * (my - ny / 2) / (ny * dy);
+            if (kx * kx + ky * ky < k * k) norm += exp(-(kx * kx + ky * ky) * w0 * w0 / 4);
+        }
+    for (int i = 0; i < nx2; ++i) {
+        double x = (i + sx - cx) * dx;
+        for (int j = 0; j < ny2; ++j) {
+            double complex acc = 0;
+            for (int mx = 0; mx < nx; ++mx) {
+                double kx = 2 * M_PI * (mx - nx / 2) / (nx * dx);
+                acc += tmp[(size_t)mx * ny2 + j] * cexp(CX(0, kx * x));
+            }
+            acc /= norm;
+            out[2 * ((size_t)i * ny2 + j)] = creal(acc);
+            out[2 * ((size_t)i * ny2 + j) + 1] = cimag(acc);
+        }
+    }
+    free(tmp);
+}
+
+static int focused_setup(focused_t *F, const mpx_factory *f, int nx, int ny, int nz, double dx, double dy,
+                         double dz, double dt)
+{
+    F->nx2 = nx + 2;
+    F->ny2 = ny + 2;
+    double cz = 0.5 * (nz + 1);
+    int kk = f->d - 1; /* the plane both currents sample (layers at d and d-1, down face) */
+    F->zE = (kk - cz) * dz - f->zshift;
+    F->zB = (kk - 0.5 - cz) * dz - f->zshift;
+    F->T = f->length;
+    F->om0 = f->om0;
+    F->v = sqrt(1 / f->eps);
+    F->TShift = f->tshift;
+    F->phase = 2 * M_PI * f->phase;
+    F->amp = f->amp;
+    F->DT = dt;
+    size_t cnt = 2 * (size_t)F->nx2 * F->ny2;
+    F->ex = (double *)malloc(cnt * sizeof(double));
+    F->bx = (double *)malloc(cnt * sizeof(double));
+    F->by = (double *)malloc(cnt * sizeof(double));
+    if (!F->ex || !F->bx || !F->by) return 0;
+    focused_plane(f, nx, ny, dx, dy, F->zE, 0, 0.5, 0.0, F->ex);
+    focused_plane(f, nx, ny, dx, dy, F->zB, 1, 0.0, 0.5, F->bx);
+    focused_plane(f, nx, ny, dx, dy, F->zB, 2, 0.5, 0.0, F->by);
+    return 1;
+}
+
+static double focused_value(const focused_t *F, const double *U, int i, int j, double t, double z)
+{
+    size_t q = 2 * ((size_t)i * F->ny2 + j);
+    double s = t - z / F->v;
+    double env = exp(-(s * s) / (F->T * F->T));
+    double ph = -(F->om0 * t - F->phase);
+    return F->amp * env * (U[q] * cos(ph) - U[q + 1] * sin(ph));
+}
+
+void or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy, double z, int comp, double sx,
+                      double sy, double *out)
+{
+    focused_plane(f, nx, ny, dx, dy, z, comp, sx, sy, out);
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* Currents (CPML borders and incident-source sheets)                                   */
 /* ------------------------------------------------------------------------------------ */
 
 enum { CUR_CPML_E = 1, CUR_CPML_H = 2, CUR_INC_E = 3, CUR_INC_H = 4 };
-enum { SRC_PULSE = 1, SRC_PLANE = 2 };
+enum { SRC_PULSE = 1, SRC_PLANE = 2, SRC_FOCUSED = 3 };
 
 typedef struct {
     char name[24];
@@ -559,6 +665,7 @@ typedef struct {
     int src_kind;
     pulse_t pulse;
     plane_t plane;
+    focused_t focused;
 } current_t;
 
 #define MAX_CURRENTS 64
@@ -951,7 +1058,14 @@ static void inc_init_storage(or_sim *s, current_t *c, int isH)
 
 static void src_getH(const current_t *c, int i, int j, int k, int time, double out[3])
 {
-    if (c->src_kind == SRC_PULSE)
+    if (c->src_kind == SRC_FOCUSED) {
+        const focused_t *F = &c->focused;
+        double t = (time - 0.5) * F->DT - F->TShift;
+        out[0] = focused_value(F, F->bx, i, j, t, F->zB);
+        out[1] = focused_value(F, F->by, i, j, t, F->zB);
+        out[2] = 0;
+        (void)k;
+    } else if (c->src_kind == SRC_PULSE)
         pulse_getH(&c->pulse, i, j, k, time, out);
     else
         plane_getH(&c->plane, i, j, k, time, out);
@@ -959,7 +1073,14 @@ static void src_getH(const current_t *c, int i, int j, int k, int time, double o
 
 static void src_getE(const current_t *c, int i, int j, int k, int time, double out[3])
 {
-    if (c->src_kind == SRC_PULSE)
+    if (c->src_kind == SRC_FOCUSED) {
+        const focused_t *F = &c->focused;
+        double t = time * F->DT - F->TShift;
+        out[0] = focused_value(F, F->ex, i, j, t, F->zE);
+        out[1] = 0;
+        out[2] = 0;
+        (void)k;
+    } else if (c->src_kind == SRC_PULSE)
         pulse_getE(&c->pulse, i, j, k, time, out);
     else
         plane_getE(&c->plane, i, j, k, time, out);
@@ -1560,7 +1681,7 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
                 else
                     cpml_init_kappa(s, f->d, f->kappa_max);
             } else if (f->kind == MPX_FACTORY_SHORT_PULSE || f->kind == MPX_FACTORY_PLANE_WAVE ||
-                       f->kind == MPX_FACTORY_PLANE_GAUSS) {
+                       f->kind == MPX_FACTORY_PLANE_GAUSS || f->kind == MPX_FACTORY_FOCUSED_PULSE) {
                 /* needCurrent: down face only, src/shortpulseinject.cpp:28-31, src/sources.cpp:8-11 */
                 int dir = DOWN;
                 for (int h = 0; h < 2; ++h) {
@@ -1572,7 +1693,14 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
                     }
                     c->distance = f->d;
                     inc_dims(c, s->dx, s->dy, s->dz);
-                    if (f->kind == MPX_FACTORY_SHORT_PULSE) {
+                    if (f->kind == MPX_FACTORY_FOCUSED_PULSE) {
+                        c->src_kind = SRC_FOCUSED;
+                        if (!focused_setup(&c->focused, f, s->N[0], s->N[1], s->N[2], s->dx, s->dy, s->dz, s->dt)) {
+                            seterr(err, errlen, "out of memory");
+                            or_destroy(s);
+                            return NULL;
+                        }
+                    } else if (f->kind == MPX_FACTORY_SHORT_PULSE) {
                         double prm[8] = {f->length, f->width, f->om0, f->tshift,
                                          f->zshift, f->phase, f->amp, f->eps};
                         c->src_kind = SRC_PULSE;
@@ -1676,6 +1804,9 @@ void or_destroy(or_sim *s)
     for (int q = 0; q < s->ncur; ++q) {
         free(s->cur[q].b.v);
         free(s->cur[q].c.v);
+        free(s->cur[q].focused.ex);
+        free(s->cur[q].focused.bx);
+        free(s->cur[q].focused.by);
     }
     free(s);
 }

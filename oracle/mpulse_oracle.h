@@ -42,6 +42,10 @@ void    or_pulse_efunc(const double params[8], int normalise, double x, double y
 void    or_pulse_bfunc(const double params[8], int normalise, double x, double y, double z,
                        double t, int bx, double out[2]);
 double  or_pulse_amp(const double params[8]);
+/* focused-pulse plane: comp 0 = E_x, 1 = B_x, 2 = B_y at distance z from the focus, Yee
+   offsets (sx, sy); out = (nx+2)(ny+2) interleaved (re, im) */
+void    or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy, double z, int comp,
+                         double sx, double sy, double *out);
 /* CPML coefficient table of one face current (isH, dir) as built by makeCoeff */
 int     or_cpml_coeff(or_sim *s, int isH, int dir, double *b, double *c, int *lo, int *hi);
 

@@ -48,6 +48,8 @@ def load(omp=False):
         "or_pulse_bfunc": (None, [D, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, D]),
         "or_pulse_amp": (C.c_double, [D]),
         "or_cpml_coeff": (C.c_int, [P, C.c_int, C.c_int, D, D, I3, I3]),
+        "or_focused_plane": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                    C.c_double, C.c_double, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -182,3 +184,11 @@ def pulse_amp(params):
     lib = load()
     p = np.ascontiguousarray(params, dtype=np.float64)
     return lib.or_pulse_amp(_dp(p))
+
+
+def focused_plane(fac, nx, ny, dx, dy, z, comp, sx, sy):
+    lib = load()
+    out = np.empty((nx + 2) * (ny + 2) * 2)
+    lib.or_focused_plane(C.byref(fac), nx, ny, dx, dy, z, comp, sx, sy, _dp(out))
+    v = out[0::2] + 1j * out[1::2]
+    return v.reshape(nx + 2, ny + 2)

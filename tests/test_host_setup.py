@@ -60,8 +60,21 @@ def test_setup_errors_follow_the_reference():
         abi.parse_setup("fieldsim { fdtd-plain { } }")
     with pytest.raises(abi.MpxError, match="No field simulation specified"):
         abi.parse_setup("globals { grid-x 8 }")
-    with pytest.raises(abi.MpxError, match="focused_pulse_inject is not provided"):
-        abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { focused_pulse_inject { } } }")
+    with pytest.raises(abi.MpxError, match="gauss_inject is not provided"):
+        abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { gauss_inject { } } }")
+
+
+def test_focused_pulse_keys():
+    """focused_pulse_inject takes the keys of src/focusedpulseinject.cpp:38-56 (oversampling
+    keys accepted) and becomes one focused-pulse factory."""
+    d, _ = abi.parse_setup("globals { } fieldsim { fdtd-plrc-lin { focused_pulse_inject { d 11 "
+                           "length 30 width 20 om0 1 TShift 60 ZShift 4 Phase 0.5 OversampleX 2 "
+                           "OversampleY 2 amp 3 eps 2.25 } } }")
+    assert d.n_factories == 1
+    f = d.factories[0]
+    assert f.kind == abi.FACTORY_FOCUSED_PULSE
+    assert (f.d, f.length, f.width, f.om0, f.tshift, f.zshift, f.phase, f.amp, f.eps) == \
+        (11, 30, 20, 1, 60, 4, 0.5, 3, 2.25)
 
 
 @pytest.mark.gpu

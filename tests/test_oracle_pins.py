@@ -372,3 +372,56 @@ def test_cpml_absorbs():
     b = longer.get("Ex")[2, 2, 11:80]
     refl = np.abs(a - b).max() / np.abs(longer.get("Ex")).max()
     assert refl < 0.06, refl
+
+
+# ---------------------------------------------------------------------------------------
+# focused pulse (the library's own model; the reference's does not compile)
+# ---------------------------------------------------------------------------------------
+
+def test_focused_plane_is_the_periodised_gaussian_at_focus():
+    """At the focus (z = 0) the angular-spectrum sum of A(k) = exp(-k^2 w0^2/4) is, by Poisson
+    summation, the periodised Gaussian exp(-r^2/w0^2) (all components propagating when
+    k w0 >> 1); checked at the E_x Yee offset (i+1/2, j)."""
+    fac = abi.factory_defaults(abi.FACTORY_FOCUSED_PULSE)
+    fac.width, fac.om0, fac.eps = 3.0, 4.0, 1.0
+    nx, ny, h = 48, 40, 0.35
+    U = ol.focused_plane(fac, nx, ny, h, h, 0.0, 0, 0.5, 0.0)
+    x = (np.arange(nx + 2) + 0.5 - 0.5 * (nx + 1)) * h
+    y = (np.arange(ny + 2) - 0.5 * (ny + 1)) * h
+    Lx, Ly = nx * h, ny * h
+    gx = sum(np.exp(-((x - m * Lx) ** 2) / 9.0) for m in (-1, 0, 1))
+    gy = sum(np.exp(-((y - m * Ly) ** 2) / 9.0) for m in (-1, 0, 1))
+    want = np.outer(gx, gy)
+    assert np.abs(U - want).max() < 1e-9
+    # B_y = n E_x k_z/k (1 + kx^2/kz^2): on a broad beam B_y / E_x -> n at the centre
+    Uy = ol.focused_plane(fac, nx, ny, h, h, 0.0, 2, 0.5, 0.0)
+    c = (nx // 2, ny // 2)
+    assert abs(Uy[c] / U[c] - 1.0) < 0.02
+
+
+def focused_column(n, nz, na, kf, d_src=14):
+    d = small_desc((n, n, nz), abi.SOLVER_PLRC_LIN)
+    cp = abi.factory_defaults(abi.FACTORY_CPML)
+    cp.d = 8
+    cp.sigma_max = configs.sigma_max(8)
+    abi.add_factory(d, cp)
+    f = abi.factory_defaults(abi.FACTORY_FOCUSED_PULSE)
+    f.d, f.width, f.om0, f.length, f.tshift, f.amp, f.eps = d_src, 2 / na, 1.0, 12.0, 40.0, 1.0, 1.0
+    f.zshift = (kf - 0.5 * (nz + 1)) * configs.DX
+    abi.add_factory(d, f)
+    return d
+
+
+def test_focused_pulse_injects_one_way_and_focuses():
+    """NA 0.5 in vacuum: the field below the sheets stays under 4 % of the peak, the peak is
+    within 5 % of amp, and the brightest plane lies within 4 cells of the requested focus
+    (Rayleigh range 25 cells; the Yee grid's dispersion shifts the wide-angle waves)."""
+    o = ol.Oracle(focused_column(96, 100, 0.5, 44), omp=True)
+    peak = np.zeros(102)
+    for n in range(650):
+        o.step(1)
+        if n % 3 == 0:
+            peak = np.maximum(peak, np.abs(o.get("Ex")).max(axis=(0, 1)))
+    assert peak[9:12].max() < 0.04
+    assert 0.95 < peak.max() < 1.05
+    assert abs(int(peak.argmax()) - 44) <= 4
