@@ -80,6 +80,50 @@ def cpu_baseline(n=512, n_sample_steps=3):
                       f"{n_sample_steps} steps after 1 untimed step, {dt:.1f} s"}
 
 
+def batch_sweep(steps=500, nruns=64, autotune_steps=3):
+    """Uneven-size parameter sweep: nruns independent cubes (48^3..160^3, one pole + Kerr,
+    10-cell CPML, 500 steps each) stepped together by the batched persistent sweep replayed
+    from captured graphs with autotuned tiles, against the same runs launched one after
+    another on the float64 plain (reference-structured) path, and on the float64 fused and
+    sweep paths.  Times cover the steps only (set-up excluded), wall clock around each set."""
+    import torch
+
+    from mpulse_amd import abi, configs
+    runs = configs.config_sweep(nruns=nruns, steps=steps)
+    descs = [d for d, _ in runs]
+    cells = sum(d.nx * d.ny * d.nz for d in descs)
+    out = {"runs": nruns, "steps": steps, "cells_per_step": cells, "precision": "fp64",
+           "sizes": [d.nx for d in descs]}
+    b = abi.Batch(descs)
+    tune = b.autotune(autotune_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    torch.cuda.empty_cache()
+    out["batched"] = {"s": t_batch, "cell_updates_per_s": cells * steps / t_batch, "tuning": tune}
+    print(f"# batch: {t_batch:.3f} s for {nruns} runs x {steps} steps, tuning {tune}", file=sys.stderr, flush=True)
+    for name, path in (("sequential_plain", abi.PATH_REFERENCE), ("sequential_fused", abi.PATH_FUSED),
+                       ("sequential_sweep", abi.PATH_SWEEP)):
+        total = 0.0
+        for d in descs:
+            d.path = path
+            s = abi.Solver(d)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.sync()
+            total += time.perf_counter() - t0
+            s.close()
+        torch.cuda.empty_cache()
+        out[name] = {"s": total, "cell_updates_per_s": cells * steps / total}
+        out[f"speedup_vs_{name}"] = total / t_batch
+        print(f"# {name}: {total:.3f} s, batched speed-up {total / t_batch:.2f}x", file=sys.stderr, flush=True)
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -98,6 +142,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--forms", default="", help="comma list of fp32_bulk_fused, fp64_fused, fp64_reference")
     ap.add_argument("--copy-probe", action="store_true", help="also time a 4 GiB device copy")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-run batched sweep")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-run batched sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

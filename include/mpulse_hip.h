@@ -195,6 +195,34 @@ int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
    gives the practical copy bandwidth beside the stencil kernels. */
 int  mpx_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
+/* ---- batched independent runs -------------------------------------------------------
+   No counterpart in the reference, which runs one simulation per process.  For uneven-size
+   parameter sweeps (pulse energy, chi3, grid size): every run is an FDTD_PLRC solver on a
+   periodic domain built exactly as mpx_create builds it (sweep path), and all runs advance in
+   lockstep, one persistent launch per step for all of them, replayed from a captured HIP
+   graph.  The runs must share precision, solver (lin / nonlin), number of active poles and
+   whether a CPML is present.  mpx_batch_run hands out a run's handle for reading and writing
+   fields (mpx_get_field, mpx_energy, ...); stepping, timing or destroying such a handle
+   directly is refused. */
+#define MPX_MAX_BATCH 1024
+typedef struct mpx_batch mpx_batch;
+int  mpx_batch_create(const mpx_desc *descs, int nruns, mpx_batch **out);
+int  mpx_batch_destroy(mpx_batch *b);
+int  mpx_batch_size(mpx_batch *b, int *nruns);
+int  mpx_batch_run(mpx_batch *b, int run, mpx_solver **out);
+int  mpx_batch_step(mpx_batch *b, int nsteps);
+int  mpx_batch_sync(mpx_batch *b);
+int  mpx_batch_get_time(mpx_batch *b, int *time);
+/* "variant" (block shape 0..2, as sweep_variant), "seg" (x planes per tile), "blocks_per_cu"
+   (persistent grid = blocks_per_cu x CUs), "graph_steps" (steps per captured graph; 0 = none) */
+int  mpx_batch_set_tuning(mpx_batch *b, const char *key, int value);
+/* Tries every variant x seg {16,32,64,128} x blocks_per_cu {1,2,4} for steps_per_trial steps,
+   keeps the fastest (result = {variant, seg, blocks_per_cu}), then returns every run to its
+   all-zero state at time 0.  Only before the first step. */
+int  mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[3], double *best_ms_per_step);
+/* Advance nsteps and report the mean device time per step (HIP events on the batch stream). */
+int  mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step);
+
 const char *mpx_last_error(void);
 int  mpx_abi_version(void);
 

@@ -140,6 +140,16 @@ def load():
         "mpx_state_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
         "mpx_copy_probe": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "mpx_set_tuning": (C.c_int, [P, C.c_char_p, C.c_int]),
+        "mpx_batch_create": (C.c_int, [C.POINTER(Desc), C.c_int, C.POINTER(P)]),
+        "mpx_batch_destroy": (C.c_int, [P]),
+        "mpx_batch_size": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "mpx_batch_run": (C.c_int, [P, C.c_int, C.POINTER(P)]),
+        "mpx_batch_step": (C.c_int, [P, C.c_int]),
+        "mpx_batch_sync": (C.c_int, [P]),
+        "mpx_batch_get_time": (C.c_int, [P, C.POINTER(C.c_int)]),
+        "mpx_batch_set_tuning": (C.c_int, [P, C.c_char_p, C.c_int]),
+        "mpx_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "mpx_batch_time_steps": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
         "mpx_last_error": (C.c_char_p, []),
         "mpx_abi_version": (C.c_int, []),
         "mpx_desc_defaults": (None, [C.POINTER(Desc)]),
@@ -166,9 +176,18 @@ class Solver:
         self.lib = load()
         self.desc = desc
         self.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
+        self._own = True
         h = C.c_void_p()
         self._check(self.lib.mpx_create(C.byref(desc), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def _view(cls, lib, desc, handle):
+        """A non-owning handle on one run of a Batch (fields, energy; stepping is refused)."""
+        v = cls.__new__(cls)
+        v.lib, v.desc, v.h, v._own = lib, desc, handle, False
+        v.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
+        return v
 
     def _check(self, rc):
         if rc != OK:
@@ -176,7 +195,8 @@ class Solver:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.mpx_destroy(self.h)
+            if getattr(self, "_own", True):
+                self.lib.mpx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -260,6 +280,73 @@ class Solver:
         b = C.c_size_t()
         self._check(self.lib.mpx_state_bytes(self.h, C.byref(b)))
         return b.value
+
+
+class Batch:
+    """Owner of one mpx_batch handle: independent runs stepped together (see mpx_batch_create)."""
+
+    def __init__(self, descs):
+        self.lib = load()
+        self.descs = list(descs)
+        arr = (Desc * len(self.descs))(*self.descs)
+        h = C.c_void_p()
+        self._check(self.lib.mpx_batch_create(arr, len(self.descs), C.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != OK:
+            raise MpxError(rc, self.lib.mpx_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.descs)
+
+    def run(self, i):
+        r = C.c_void_p()
+        self._check(self.lib.mpx_batch_run(self.h, i, C.byref(r)))
+        return Solver._view(self.lib, self.descs[i], r)
+
+    def step(self, n=1):
+        self._check(self.lib.mpx_batch_step(self.h, n))
+
+    def sync(self):
+        self._check(self.lib.mpx_batch_sync(self.h))
+
+    @property
+    def time(self):
+        t = C.c_int()
+        self._check(self.lib.mpx_batch_get_time(self.h, C.byref(t)))
+        return t.value
+
+    def tune(self, key, value):
+        self._check(self.lib.mpx_batch_set_tuning(self.h, key.encode(), int(value)))
+
+    def autotune(self, steps_per_trial=3):
+        res = (C.c_int * 3)()
+        ms = C.c_double()
+        self._check(self.lib.mpx_batch_autotune(self.h, steps_per_trial, res, C.byref(ms)))
+        return {"variant": res[0], "seg": res[1], "blocks_per_cu": res[2], "ms_per_step": ms.value}
+
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        self._check(self.lib.mpx_batch_time_steps(self.h, nsteps, C.byref(ms)))
+        return ms.value
 
 
 def copy_probe(nbytes, reps=20):

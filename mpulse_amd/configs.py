@@ -127,3 +127,83 @@ def desc_summary(d):
     return {"grid": [d.nx, d.ny, d.nz], "dx": d.dx, "dt": d.dt, "solver": d.solver, "precision": d.precision,
             "path": d.path, "chi": d.chi, "poles": [(d.l_eps[q], d.l_om2[q], d.l_delta[q]) for q in range(3)],
             "factories": [f.kind for f in fac]}
+
+
+# fused-silica-like medium of the self-focusing workload: two undamped Lorentz poles
+# (Sellmeier terms at 0.068 um and 9.9 um), Kerr n2 = 2.5e-20 m^2/W
+SILICA_POLES = ((0.69, (0.8 / 0.068) ** 2, 0.0), (0.87, (0.8 / 9.9) ** 2, 0.0))
+EPS0_SI = 8.8541878128e-12
+
+
+def chi3_from_n2(n2, n0):
+    """chi3 [m^2/V^2] of the instantaneous Kerr term chi E^2 E for intensity index n2 [m^2/W]:
+    n2 = 3 chi3 / (4 n0^2 eps0 c)."""
+    return 4.0 * n0 * n0 * EPS0_SI * C0 * n2 / 3.0
+
+
+def field_from_intensity(intensity, n0):
+    """peak field [V/m] of a linearly polarised wave of cycle-averaged intensity [W/m^2]:
+    I = n0 eps0 c E^2 / 2."""
+    return math.sqrt(2.0 * intensity / (n0 * EPS0_SI * C0))
+
+
+def config_focus(precision=abi.PREC_FP32_BULK, path=abi.PATH_FUSED, nx=384, nz=768, steps=2000, d_cpml=16,
+                 d_src=20, na=0.1, fwhm_fs=30.0, intensity=5e16, n2=2.5e-20, poles=SILICA_POLES):
+    """Focused-pulse self-focusing: nx^2 x nz, two Lorentz poles, Kerr from n2, focused pulse of
+    numerical aperture na (waist w0 = lambda / (pi NA) = 2 / NA length units) whose focus lies
+    on the middle z plane, FWHM 30 fs, peak intensity 5e16 W/m^2 at the focus, 16-cell CPML."""
+    eps_l = medium_eps_at_laser(1.0, poles)
+    n0 = math.sqrt(eps_l)
+    d = abi.desc_defaults()
+    d.nx, d.ny, d.nz = nx, nx, nz
+    d.dx = d.dy = d.dz = DX
+    d.dt = DT
+    d.solver = abi.SOLVER_PLRC_NONLIN
+    d.precision = precision
+    d.path = path
+    d.eps = 1.0
+    d.chi = chi3_from_n2(n2, n0) * E0 * E0
+    for q in range(3):
+        d.l_eps[q], d.l_om2[q], d.l_delta[q] = 0.0, 1.0, 0.0
+    for q, (deps, om2, delta) in enumerate(poles):
+        d.l_eps[q], d.l_om2[q], d.l_delta[q] = deps, om2, delta
+    cp = abi.factory_defaults(abi.FACTORY_CPML)
+    cp.d = d_cpml
+    cp.sigma_max = sigma_max(d_cpml)
+    abi.add_factory(d, cp)
+    f = abi.factory_defaults(abi.FACTORY_FOCUSED_PULSE)
+    T = fs_to_t(fwhm_fs) / math.sqrt(2 * math.log(2))
+    f.d = d_src
+    f.length = T
+    f.width = 2.0 / na
+    f.om0 = 1.0
+    f.amp = field_from_intensity(intensity, n0) / E0
+    f.eps = eps_l
+    f.tshift = 2 * T
+    f.zshift = 0.0  # the focus lies on the middle z plane
+    f.phase = 0.0
+    abi.add_factory(d, f)
+    meta = {"steps": steps, "d_cpml": d_cpml, "d_src": d_src, "n0": n0,
+            "probes": probe_cells(min(nx, nz), d_cpml, (d_src - 1, d_src), seed=0)}
+    return d, meta
+
+
+def config_sweep(nruns=64, steps=500, seed=1, precision=abi.PREC_FP64, path=abi.PATH_SWEEP, nmin=48, nmax=160):
+    """Batched parameter sweep: nruns independent cubes with sizes uniform in [nmin, nmax]
+    (SplitMix64, seed), the one-pole Lorentz + Kerr medium, chi3 log-uniform in
+    [1e-23, 1e-21] m^2/V^2, peak field uniform in [2e8, 5e9] V/m, 10-cell CPML, the 10 fs pulse
+    from the plane k = 12 with waist min(15, n/4) cells, `steps` steps each.  Draw order per
+    run: size, chi3, peak field."""
+    rng = splitmix64(seed)
+    out = []
+    for _ in range(nruns):
+        n = nmin + int(next(rng) % (nmax - nmin + 1))
+        u = next(rng) / 2.0 ** 64
+        chi3 = 10.0 ** (-23.0 + 2.0 * u)
+        u = next(rng) / 2.0 ** 64
+        peak = 2e8 + (5e9 - 2e8) * u
+        d, meta = plrc_kerr_pulse(n, 10, 10.0, min(15, n // 4), peak, steps, chi3_si=chi3, d_src=12,
+                                  precision=precision, path=path)
+        meta.update({"n": n, "chi3": chi3, "peak_field": peak})
+        out.append((d, meta))
+    return out

@@ -61,11 +61,9 @@ __device__ inline void plane_getE(const PlaneParams &W, int i, int j, int l, int
     }
 }
 
-__global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables ft)
+// one sheet cell (i, j) at step `time`
+__device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const FaddeevaTables &ft, int i, int j, int time)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
-    if (j >= a.ny2 || i >= a.nx2) return;
     int z = a.sheet.kp + a.off;
     double f[3] = {0, 0, 0};
     const PulseParams &P = a.pulse;
@@ -77,7 +75,7 @@ __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables 
             double posyo = (j - P.centrey) * P.DY;
             double posyh = (j + 0.5 - P.centrey) * P.DY;
             double poszh = (z - 0.5 - P.centrez) * P.DZ - P.ZShift;
-            double posTime = (a.time - 0.5) * P.DT - P.TShift;
+            double posTime = (time - 0.5) * P.DT - P.TShift;
             f[0] = P.amp * pulse_bfunc(P, ft, posxo, posyh, poszh, posTime, true).re;
             f[1] = P.amp * pulse_bfunc(P, ft, posxh, posyo, poszh, posTime, false).re;
         } else {
@@ -85,14 +83,14 @@ __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables 
             double posxh = (i + 0.5 - P.centrex) * P.DX;
             double posyo = (j - P.centrey) * P.DY;
             double poszo = (z - P.centrez) * P.DZ - P.ZShift;
-            double posTime = a.time * P.DT - P.TShift;
+            double posTime = time * P.DT - P.TShift;
             f[0] = P.amp * pulse_efunc(P, ft, posxh, posyo, poszo, posTime).re;
         }
     } else if (a.kind == 3) {
         // amp Re[env(t - z/v) U exp(-i(om0 t - phase))], env(s) = exp(-s^2/T^2)
         const FocusedParams &F = a.focused;
         size_t p = (size_t)i * a.ny2 + j;
-        double t = a.isH ? a.time * F.DT - F.TShift : (a.time - 0.5) * F.DT - F.TShift;
+        double t = a.isH ? time * F.DT - F.TShift : (time - 0.5) * F.DT - F.TShift;
         double z = a.isH ? F.zE : F.zB;
         double s = t - z / F.v;
         double env = F.amp * exp(-(s * s) / (F.T * F.T));
@@ -109,13 +107,58 @@ __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables 
         (void)z;
     } else {
         if (!a.isH)
-            plane_getH(a.plane, i, j, z, a.time, f);
+            plane_getH(a.plane, i, j, z, time, f);
         else
-            plane_getE(a.plane, i, j, z, a.time, f);
+            plane_getE(a.plane, i, j, z, time, f);
     }
     size_t q = (size_t)i * a.ny2 + j;
     a.sheet.J[a.t1][q] = f[a.t2] / a.DX;
     a.sheet.J[a.t2][q] = f[a.t1] / a.DX;
+}
+
+__global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables ft)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= a.ny2 || i >= a.nx2) return;
+    sheet_eval_point(a, ft, i, j, a.time);
+}
+
+// The sheets of many runs in one launch.  Sheet s owns blocks first[s] .. first[s+1]-1, one
+// block per 64 cells of a row; the step is *time + tstep (the counter lives on the device so
+// that a captured graph can be replayed at later steps).
+__global__ __launch_bounds__(64) void k_sheet_eval_batch(const SheetEval *__restrict__ list,
+                                                         const int *__restrict__ first, int n, FaddeevaTables ft,
+                                                         const int *__restrict__ time, int tstep)
+{
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;  // last sheet with first[s] <= b
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const SheetEval &a = list[lo];
+    const int nyb = (a.ny2 + 63) >> 6;
+    const int local = b - first[lo];
+    const int i = local / nyb, j = (local - i * nyb) * 64 + threadIdx.x;
+    if (i >= a.nx2 || j >= a.ny2) return;
+    sheet_eval_point(a, ft, i, j, *time + tstep);
+}
+
+__global__ void k_advance_time(int *time, int k)
+{
+    if (threadIdx.x == 0) *time += k;
+}
+
+void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
+                             const int *time, int tstep, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sheet_eval_batch, dim3(blocks), dim3(64), 0, s, list, first, n, ft, time, tstep);
+}
+
+void launch_advance_time(int *time, int k, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, s, time, k);
 }
 
 void launch_sheet_eval(const SheetEval &a, const FaddeevaTables &ft, hipStream_t s)
@@ -1004,13 +1047,14 @@ __device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py,
 // E_new(p-1) and E_new(p).  Every array the step reads is an "old" buffer and every array it
 // writes a "new" buffer (CPML H auxiliaries excepted: only owners touch them), so halo
 // readers never see another block's results.
+//
+// sweep_tile is one tile's walk; k_sweep maps the blocks of one grid onto tiles, k_sweep_batch
+// walks the tiles of many independent grids from one persistent launch.
 template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
-__global__ __launch_bounds__(64 * BY) void k_sweep(SweepArgs<T> a, int ntz, int nty, int nseg)
+__device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty, int tx,
+                                           T (&lds)[3][3][BY * RPT][64])
 {
     constexpr int TYR = BY * RPT;
-    __shared__ T lds[3][3][TYR][64];
-    int tz, ty, tx;
-    xcd_tile(ntz, nty, nseg, tz, ty, tx);
     const GridDims &g = a.g;
     const int Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
     const int sx = (int)g.sx, pz = g.pz;
@@ -1198,26 +1242,104 @@ __global__ __launch_bounds__(64 * BY) void k_sweep(SweepArgs<T> a, int ntz, int 
     }
 }
 
+template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
+__global__ __launch_bounds__(64 * BY) void k_sweep(SweepArgs<T> a, int ntz, int nty, int nseg)
+{
+    __shared__ T lds[3][3][BY * RPT][64];
+    int tz, ty, tx;
+    xcd_tile(ntz, nty, nseg, tz, ty, tx);
+    sweep_tile<T, NP, NONLIN, CPML, BY, RPT>(a, tz, ty, tx, lds);
+}
+
+// Persistent batched sweep: the tiles of all runs are numbered run by run (z tile fastest,
+// then y, then x segment); the 8 XCDs take contiguous eighths of that list so that tiles
+// sharing halo cells meet in one L2, and each block walks its XCD's eighth with stride
+// gridDim.x / 8 (gridDim.x is a multiple of 8; the hardware deals blocks to XCDs round robin).
+template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
+__global__ __launch_bounds__(64 * BY) void k_sweep_batch(const SweepArgs<T> *__restrict__ args,
+                                                         const BatchRun *__restrict__ runs, int nruns, int total)
+{
+    __shared__ T lds[3][3][BY * RPT][64];
+    const int per_xcd = gridDim.x >> 3;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int chunk = (total + 7) >> 3;
+    const int t0 = xcd * chunk, t1 = min(total, t0 + chunk);
+    int r = 0;
+    for (int t = t0 + slot; t < t1; t += per_xcd) {
+        while (r + 1 < nruns && t >= runs[r + 1].first) ++r;
+        const int ntz = runs[r].ntz, nty = runs[r].nty;
+        int local = t - runs[r].first;
+        const int tz = local % ntz;
+        local /= ntz;
+        const int ty = local % nty, tx = local / nty;
+        sweep_tile<T, NP, NONLIN, CPML, BY, RPT>(args[r], tz, ty, tx, lds);
+        __syncthreads();  // the next tile rewrites the LDS planes
+    }
+}
+
 template <typename T, int NP, bool NONLIN, bool CPML>
 static void launch_sweep_np(const SweepArgs<T> &a, hipStream_t s)
 {
-    const GridDims &g = a.g;
-    int ntz = (g.n[2] + 62) / 63;
-    int nseg = (g.n[0] + a.seg - 1) / a.seg;
-    if (a.variant == 1) {  // 64 x 8 threads, 2 rows each: 16-row tiles
-        int nty = (g.n[1] + 14) / 15;
+    int ntz, nty, nseg;
+    sweep_tiles(a.g, a.variant, a.seg, ntz, nty, nseg);
+    if (a.variant == 1)  // 64 x 8 threads, 2 rows each: 16-row tiles
         hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 8, 2>), dim3(ntz, nty, nseg), dim3(64, 8), 0, s, a, ntz, nty,
                            nseg);
-    } else if (a.variant == 2) {  // 64 x 16 threads, 1 row each
-        int nty = (g.n[1] + 14) / 15;
+    else if (a.variant == 2)  // 64 x 16 threads, 1 row each
         hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 16, 1>), dim3(ntz, nty, nseg), dim3(64, 16), 0, s, a, ntz,
                            nty, nseg);
-    } else {  // 64 x 8 threads, 1 row each: 8-row tiles
-        int nty = (g.n[1] + 6) / 7;
+    else  // 64 x 8 threads, 1 row each: 8-row tiles
         hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 8, 1>), dim3(ntz, nty, nseg), dim3(64, 8), 0, s, a, ntz, nty,
                            nseg);
-    }
 }
+
+// tile grid of one run for the block shape `variant` and segment length `seg`
+void sweep_tiles(const GridDims &g, int variant, int seg, int &ntz, int &nty, int &nseg)
+{
+    ntz = (g.n[2] + 62) / 63;
+    nty = variant == 0 ? (g.n[1] + 6) / 7 : (g.n[1] + 14) / 15;
+    nseg = (g.n[0] + seg - 1) / seg;
+}
+
+template <typename T, int NP, bool NONLIN, bool CPML>
+static void launch_sweep_batch_np(const SweepArgs<T> *args, const BatchRun *runs, int nruns, int total, int variant,
+                                  int blocks, hipStream_t s)
+{
+    if (variant == 1)
+        hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 8, 2>), dim3(blocks), dim3(64, 8), 0, s, args, runs,
+                           nruns, total);
+    else if (variant == 2)
+        hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 16, 1>), dim3(blocks), dim3(64, 16), 0, s, args, runs,
+                           nruns, total);
+    else
+        hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 8, 1>), dim3(blocks), dim3(64, 8), 0, s, args, runs,
+                           nruns, total);
+}
+
+template <typename T>
+void launch_sweep_batch(const SweepArgs<T> *args, const BatchRun *runs, int nruns, int total, int variant,
+                        int blocks, int np, bool nonlin, bool cpml, hipStream_t s)
+{
+#define MPX_SB(NPV, NL, CP) launch_sweep_batch_np<T, NPV, NL, CP>(args, runs, nruns, total, variant, blocks, s)
+#define MPX_SB_NP(NL, CP)             \
+    switch (np) {                     \
+    case 0: MPX_SB(0, NL, CP); break; \
+    case 1: MPX_SB(1, NL, CP); break; \
+    case 2: MPX_SB(2, NL, CP); break; \
+    default: MPX_SB(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cpml) { MPX_SB_NP(true, true) } else { MPX_SB_NP(true, false) }
+    } else {
+        if (cpml) { MPX_SB_NP(false, true) } else { MPX_SB_NP(false, false) }
+    }
+#undef MPX_SB_NP
+#undef MPX_SB
+}
+template void launch_sweep_batch<double>(const SweepArgs<double> *, const BatchRun *, int, int, int, int, int, bool,
+                                         bool, hipStream_t);
+template void launch_sweep_batch<float>(const SweepArgs<float> *, const BatchRun *, int, int, int, int, int, bool,
+                                        bool, hipStream_t);
 
 template <typename T>
 void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s)

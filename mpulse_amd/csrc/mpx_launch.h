@@ -53,6 +53,13 @@ template <typename T>
 void launch_fused_B(const StepArgs<T> &a, hipStream_t s);
 template <typename T>
 void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s);
+void sweep_tiles(const GridDims &g, int variant, int seg, int &ntz, int &nty, int &nseg);
+template <typename T>
+void launch_sweep_batch(const SweepArgs<T> *args, const BatchRun *runs, int nruns, int total, int variant,
+                        int blocks, int np, bool nonlin, bool cpml, hipStream_t s);
+void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
+                             const int *time, int tstep, hipStream_t s);
+void launch_advance_time(int *time, int k, hipStream_t s);
 void launch_simple_stepD(const SimpleArgs &a, hipStream_t s);
 void launch_simple_stepB(const SimpleArgs &a, hipStream_t s);
 template <typename T>

@@ -144,6 +144,7 @@ struct mpx_solver {
     std::vector<int> eface_layer[2];   // per E face: layer holding psi0 / psi1
     int sweep_seg = 32, sweep_variant = 2;
     Timer timer;
+    hipStream_t owner = nullptr;  // stream of the batch that steps this run (nullptr: none)
 
     ~mpx_solver()
     {
@@ -186,6 +187,16 @@ struct mpx_solver {
         timer.ev1.push_back(e);
     }
 };
+
+// a run owned by a batch is stepped only through the batch; state reads wait for its stream
+#define MPX_NOT_IN_BATCH(s)                                                                          \
+    do {                                                                                             \
+        if ((s) && (s)->owner) return fail(MPX_ERR_INVALID, "run belongs to a batch: use mpx_batch_*"); \
+    } while (0)
+#define MPX_OWNER_SYNC(s)                                          \
+    do {                                                           \
+        if ((s) && (s)->owner) MPX_HIP(hipStreamSynchronize((s)->owner)); \
+    } while (0)
 
 // synchronous host-to-device copy ordered after the allocations' clears on the solver stream
 static int upload(mpx_solver *s, void *dst, const void *src, size_t bytes)
@@ -529,6 +540,7 @@ int mpx_create(const mpx_desc *desc, mpx_solver **out)
 
 int mpx_destroy(mpx_solver *s)
 {
+    MPX_NOT_IN_BATCH(s);
     delete s;
     return MPX_OK;
 }
@@ -1269,6 +1281,7 @@ extern "C" {
 int mpx_step(mpx_solver *s, int nsteps)
 {
     if (!s || nsteps < 0) return fail(MPX_ERR_INVALID, "invalid solver or step count");
+    MPX_NOT_IN_BATCH(s);
     for (int n = 0; n < nsteps; ++n) {
         int rc = step_once(s);
         if (rc) return rc;
@@ -1280,6 +1293,7 @@ int mpx_step(mpx_solver *s, int nsteps)
 int mpx_step_init(mpx_solver *s)
 {
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    MPX_NOT_IN_BATCH(s);
     if (s->esz != 8) return fail(MPX_ERR_UNSUPPORTED, "stepSchemeInit is provided in float64 only");
     int rc;
     if (!s->plrc) {
@@ -1336,6 +1350,7 @@ int mpx_get_time(mpx_solver *s, int *t)
 int mpx_set_time(mpx_solver *s, int t)
 {
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    MPX_NOT_IN_BATCH(s);
     s->time = t;
     return MPX_OK;
 }
@@ -1343,6 +1358,7 @@ int mpx_set_time(mpx_solver *s, int t)
 int mpx_sync(mpx_solver *s)
 {
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
+    MPX_OWNER_SYNC(s);
     MPX_HIP(hipStreamSynchronize(s->stream));
     return MPX_OK;
 }
@@ -1407,6 +1423,7 @@ extern "C" {
 int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
 {
     if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_OWNER_SYNC(s);
     const GridDims &g = s->g;
     size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
@@ -1443,6 +1460,7 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
 int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count)
 {
     if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_OWNER_SYNC(s);
     const GridDims &g = s->g;
     size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
@@ -1477,6 +1495,7 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
 int mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_t count, int lo[3], int hi[3])
 {
     if (!s || !name) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_OWNER_SYNC(s);
     for (const Layer &L : s->layers)
         if (L.dir == dir && L.name == name) {
             if (lo && hi)
@@ -1506,6 +1525,7 @@ int mpx_set_probes(mpx_solver *s, int nprobes, const int *ijk, const int *comp, 
 {
     if (!s || nprobes < 0 || capacity < 0 || (nprobes > 0 && (!ijk || !comp)))
         return fail(MPX_ERR_INVALID, "invalid probe arguments");
+    MPX_NOT_IN_BATCH(s);
     std::vector<size_t> idx(nprobes);
     for (int p = 0; p < nprobes; ++p) {
         for (int d = 0; d < 3; ++d)
@@ -1545,6 +1565,7 @@ int mpx_get_probes(mpx_solver *s, double *host, int *nrecorded)
 int mpx_energy(mpx_solver *s, double *energy)
 {
     if (!s || !energy) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_OWNER_SYNC(s);
     if (s->esz == 8) {
         double *e[3] = {(double *)s->E[0], (double *)s->E[1], (double *)s->E[2]};
         double *b[3] = {(double *)s->B[0], (double *)s->B[1], (double *)s->B[2]};
@@ -1565,6 +1586,7 @@ int mpx_energy(mpx_solver *s, double *energy)
 int mpx_time_steps(mpx_solver *s, int nsteps, mpx_timing *out)
 {
     if (!s || !out || nsteps < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
+    MPX_NOT_IN_BATCH(s);
     std::memset(out, 0, sizeof(*out));
     hipEvent_t t0, t1;
     MPX_HIP(hipEventCreate(&t0));
@@ -1655,6 +1677,7 @@ int mpx_copy_probe(size_t bytes, int reps, double *ms_per_copy)
 int mpx_set_tuning(mpx_solver *s, const char *key, int value)
 {
     if (!s || !key) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_NOT_IN_BATCH(s);
     bool march = !std::strcmp(key, "march_e") || !std::strcmp(key, "march_b");
     if (march && value != 1 && value != 2 && value != 4) return fail(MPX_ERR_INVALID, "march must be 1, 2 or 4");
     if (!std::strcmp(key, "march_e"))
@@ -1669,6 +1692,420 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
         s->sweep_variant = value;
     } else
         return fail(MPX_ERR_NAME, std::string("unknown tuning key ") + key);
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// batched independent runs: one persistent sweep launch per step for all runs
+// ---------------------------------------------------------------------------------------
+//
+// The runs are ordinary solvers built by `build` (sweep path), stepped in lockstep from one
+// stream.  Per step two kernels run: the source sheets of every run, then the sweep over the
+// tiles of every run.  The step counter lives on the device, so a captured graph of K steps
+// can be replayed at any later step; the runs' buffer sets alternate every step, and the two
+// argument tables (one per parity) hold both arrangements.
+
+struct mpx_batch {
+    std::vector<mpx_solver *> runs;
+    hipStream_t stream = nullptr;
+    size_t esz = 8;
+    int np = 0;
+    bool nonlin = false, cpml = false;
+    int variant = 2, seg = 32, blocks_per_cu = 2, graph_steps = 50;
+    int cus = 256;
+    int time = 0;
+    int parity = 0;  // steps taken since creation, mod 2
+    void *d_args[2] = {nullptr, nullptr};
+    mpx::BatchRun *d_runs = nullptr;
+    int total_tiles = 0, blocks = 0;
+    mpx::SheetEval *d_sheets = nullptr;
+    int *d_sheet_first = nullptr;
+    int nsheets = 0, sheet_blocks = 0;
+    int *d_time = nullptr;
+    mpx::FaddeevaTables ft{};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::vector<void *> allocs;
+
+    void drop_graphs()
+    {
+        for (int p = 0; p < 2; ++p)
+            if (graph[p]) {
+                (void)hipGraphExecDestroy(graph[p]);
+                graph[p] = nullptr;
+            }
+    }
+    ~mpx_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (mpx_solver *s : runs) {
+            s->owner = nullptr;
+            delete s;
+        }
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int alloc(void **p, size_t bytes)
+    {
+        if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return fail(MPX_ERR_NOMEM, "batch table allocation failed");
+        allocs.push_back(*p);
+        return MPX_OK;
+    }
+};
+
+// argument tables, tile list and sheet list for the current tuning; drops captured graphs
+template <typename T>
+static int batch_prepare_t(mpx_batch *b)
+{
+    const int n = (int)b->runs.size();
+    std::vector<mpx::SweepArgs<T>> a[2];
+    std::vector<mpx::BatchRun> runs(n);
+    int total = 0;
+    for (int r = 0; r < n; ++r) {
+        mpx_solver *s = b->runs[r];
+        s->sweep_variant = b->variant;
+        s->sweep_seg = b->seg < s->g.n[0] ? b->seg : s->g.n[0];
+        a[b->parity].push_back(sweep_args<T>(s));
+        swap_sweep_buffers(s);
+        a[b->parity ^ 1].push_back(sweep_args<T>(s));
+        swap_sweep_buffers(s);
+        mpx::BatchRun &R = runs[r];
+        R.first = total;
+        mpx::sweep_tiles(s->g, b->variant, s->sweep_seg, R.ntz, R.nty, R.nseg);
+        long long t = (long long)R.ntz * R.nty * R.nseg;
+        if (total + t > (1LL << 30)) return fail(MPX_ERR_UNSUPPORTED, "batch has too many tiles");
+        total += (int)t;
+    }
+    b->drop_graphs();
+    MPX_HIP(hipStreamSynchronize(b->stream));
+    size_t bytes = n * sizeof(mpx::SweepArgs<T>);
+    for (int p = 0; p < 2; ++p) {
+        if (!b->d_args[p] && b->alloc(&b->d_args[p], bytes)) return MPX_ERR_NOMEM;
+        MPX_HIP(hipMemcpyAsync(b->d_args[p], a[p].data(), bytes, hipMemcpyHostToDevice, b->stream));
+    }
+    if (!b->d_runs && b->alloc((void **)&b->d_runs, n * sizeof(mpx::BatchRun))) return MPX_ERR_NOMEM;
+    MPX_HIP(hipMemcpyAsync(b->d_runs, runs.data(), n * sizeof(mpx::BatchRun), hipMemcpyHostToDevice, b->stream));
+    MPX_HIP(hipStreamSynchronize(b->stream));
+    b->total_tiles = total;
+    // persistent grid: blocks_per_cu blocks on every CU, a multiple of the 8 XCDs, no more
+    // than the tiles need
+    int want = b->blocks_per_cu * b->cus;
+    int need = (total + 7) / 8 * 8;
+    b->blocks = (want < need ? want : need);
+    b->blocks = (b->blocks + 7) / 8 * 8;
+    return MPX_OK;
+}
+
+static int batch_prepare(mpx_batch *b)
+{
+    return b->esz == 8 ? batch_prepare_t<double>(b) : batch_prepare_t<float>(b);
+}
+
+// enqueue step j of a sequence that starts at the device counter, with argument parity par
+static void batch_enqueue(mpx_batch *b, int j, int par)
+{
+    if (b->nsheets)
+        mpx::launch_sheet_eval_batch(b->d_sheets, b->d_sheet_first, b->nsheets, b->sheet_blocks, b->ft, b->d_time, j,
+                                     b->stream);
+    const int n = (int)b->runs.size();
+    if (b->esz == 8)
+        mpx::launch_sweep_batch<double>((const mpx::SweepArgs<double> *)b->d_args[par], b->d_runs, n,
+                                        b->total_tiles, b->variant, b->blocks, b->np, b->nonlin, b->cpml, b->stream);
+    else
+        mpx::launch_sweep_batch<float>((const mpx::SweepArgs<float> *)b->d_args[par], b->d_runs, n,
+                                       b->total_tiles, b->variant, b->blocks, b->np, b->nonlin, b->cpml, b->stream);
+}
+
+static int batch_capture(mpx_batch *b, int par)
+{
+    hipGraph_t gr = nullptr;
+    MPX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < b->graph_steps; ++j) batch_enqueue(b, j, (par + j) & 1);
+    mpx::launch_advance_time(b->d_time, b->graph_steps, b->stream);
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (e != hipSuccess) return fail(MPX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph[par], gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+        b->graph[par] = nullptr;
+        return fail(MPX_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    return MPX_OK;
+}
+
+// nsteps in lockstep; graph replays for whole groups of graph_steps, direct launches after
+static int batch_step(mpx_batch *b, int nsteps, bool use_graph)
+{
+    int done = 0;
+    const int K = b->graph_steps;
+    if (use_graph && K > 0)
+        while (nsteps - done >= K) {
+            if (!b->graph[b->parity]) {
+                int rc = batch_capture(b, b->parity);
+                if (rc) return rc;
+            }
+            MPX_HIP(hipGraphLaunch(b->graph[b->parity], b->stream));
+            b->parity = (b->parity + K) & 1;
+            done += K;
+        }
+    for (; done < nsteps; ++done) {
+        batch_enqueue(b, 0, b->parity);
+        mpx::launch_advance_time(b->d_time, 1, b->stream);
+        MPX_LAUNCHED("batch step");
+        b->parity ^= 1;
+    }
+    for (mpx_solver *s : b->runs) {
+        s->time += nsteps;
+        if (nsteps & 1) swap_sweep_buffers(s);
+    }
+    b->time += nsteps;
+    return MPX_OK;
+}
+
+// every mutable array of a run back to zero (the state a fresh solver starts from)
+static int reset_state(mpx_solver *s, hipStream_t st)
+{
+    auto z = [&](void *p, size_t bytes) -> int {
+        if (p && hipMemsetAsync(p, 0, bytes, st) != hipSuccess) return fail(MPX_ERR_HIP, "reset failed");
+        return MPX_OK;
+    };
+    int rc = 0;
+    for (int c = 0; c < 3; ++c) {
+        rc |= z(s->E[c], s->cells * s->esz) | z(s->B[c], s->cells * s->esz);
+        rc |= z(s->E2[c], s->cells * s->esz) | z(s->B2[c], s->cells * s->esz);
+        rc |= z(s->Jt[c], s->cells * 8) | z(s->Mt[c], s->cells * 8);
+        for (int n = 0; n < 3; ++n) {
+            rc |= z(s->psi[n][c], s->cells * 16) | z(s->psi2[n][c], s->cells * 16);
+            rc |= z(s->P[n][c], s->cells * 8) | z(s->Pp[n][c], s->cells * 8);
+        }
+    }
+    for (const Layer &L : s->layers) {
+        size_t el = L.name.compare(0, 4, "CPML") == 0 ? s->esz : 8;
+        for (int c = 0; c < 3; ++c) rc |= z(L.comp[c], L.count() * el);
+    }
+    for (int w = 0; w < 2; ++w)
+        for (size_t f = 0; f < s->epsi2[w].size(); ++f) {
+            const CpmlFace &F = s->efaces[f];
+            rc |= z(s->epsi2[w][f], (size_t)(F.hi[0] - F.lo[0] + 1) * F.n1 * F.n2 * s->esz);
+        }
+    s->time = 0;
+    s->probe_rec = 0;
+    return rc ? MPX_ERR_HIP : MPX_OK;
+}
+
+extern "C" {
+
+int mpx_batch_create(const mpx_desc *descs, int nruns, mpx_batch **out)
+{
+    if (!descs || !out || nruns < 1 || nruns > MPX_MAX_BATCH) return fail(MPX_ERR_INVALID, "invalid batch arguments");
+    *out = nullptr;
+    mpx_batch *b = new mpx_batch();
+    auto bail = [&](int rc) {
+        delete b;
+        return rc;
+    };
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return bail(fail(MPX_ERR_HIP, "batch stream or device query failed"));
+    b->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    for (int r = 0; r < nruns; ++r) {
+        mpx_desc d = descs[r];
+        d.path = MPX_PATH_SWEEP;
+        if (d.solver != MPX_SOLVER_PLRC_NONLIN && d.solver != MPX_SOLVER_PLRC_LIN)
+            return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs must use an FDTD_PLRC solver"));
+        mpx_solver *s = nullptr;
+        int rc = mpx_create(&d, &s);
+        if (rc) return bail(rc);
+        if (hipStreamSynchronize(s->stream) != hipSuccess) {
+            delete s;
+            return bail(fail(MPX_ERR_HIP, "run set-up failed"));
+        }
+        s->owner = b->stream;
+        b->runs.push_back(s);
+        bool cp = !s->efaces.empty();
+        if (r == 0) {
+            b->esz = s->esz;
+            b->np = s->np;
+            b->nonlin = s->nonlin;
+            b->cpml = cp;
+            b->ft = s->ft;
+        } else if (s->esz != b->esz || s->np != b->np || s->nonlin != b->nonlin || cp != b->cpml) {
+            return bail(fail(MPX_ERR_UNSUPPORTED,
+                             "batched runs must share precision, solver, active pole count and CPML presence"));
+        }
+        if (s->time != 0) return bail(fail(MPX_ERR_INVALID, "batched runs start at time 0"));
+    }
+    // source sheets of all runs
+    std::vector<mpx::SheetEval> sheets;
+    std::vector<int> first;
+    for (mpx_solver *s : b->runs)
+        for (int side = 0; side < 2; ++side)
+            for (const Current &c : side ? s->hcur : s->ecur)
+                if (c.type == (side ? CUR_INC_H : CUR_INC_E)) {
+                    first.push_back(b->sheet_blocks);
+                    sheets.push_back(c.ev);
+                    b->sheet_blocks += c.ev.nx2 * ((c.ev.ny2 + 63) / 64);
+                }
+    b->nsheets = (int)sheets.size();
+    if (b->alloc((void **)&b->d_time, sizeof(int))) return bail(MPX_ERR_NOMEM);
+    if (b->nsheets) {
+        if (b->alloc((void **)&b->d_sheets, sheets.size() * sizeof(mpx::SheetEval)) ||
+            b->alloc((void **)&b->d_sheet_first, first.size() * sizeof(int)))
+            return bail(MPX_ERR_NOMEM);
+        if (hipMemcpyAsync(b->d_sheets, sheets.data(), sheets.size() * sizeof(mpx::SheetEval), hipMemcpyHostToDevice,
+                           b->stream) != hipSuccess ||
+            hipMemcpyAsync(b->d_sheet_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice,
+                           b->stream) != hipSuccess)
+            return bail(fail(MPX_ERR_HIP, "sheet table upload failed"));
+    }
+    if (hipMemsetAsync(b->d_time, 0, sizeof(int), b->stream) != hipSuccess)
+        return bail(fail(MPX_ERR_HIP, "time counter clear failed"));
+    int rc = batch_prepare(b);
+    if (rc) return bail(rc);
+    *out = b;
+    return MPX_OK;
+}
+
+int mpx_batch_destroy(mpx_batch *b)
+{
+    delete b;
+    return MPX_OK;
+}
+
+int mpx_batch_size(mpx_batch *b, int *nruns)
+{
+    if (!b || !nruns) return fail(MPX_ERR_INVALID, "null argument");
+    *nruns = (int)b->runs.size();
+    return MPX_OK;
+}
+
+int mpx_batch_run(mpx_batch *b, int run, mpx_solver **out)
+{
+    if (!b || !out || run < 0 || run >= (int)b->runs.size()) return fail(MPX_ERR_INVALID, "invalid run index");
+    *out = b->runs[run];
+    return MPX_OK;
+}
+
+int mpx_batch_step(mpx_batch *b, int nsteps)
+{
+    if (!b || nsteps < 0) return fail(MPX_ERR_INVALID, "invalid batch or step count");
+    return batch_step(b, nsteps, true);
+}
+
+int mpx_batch_sync(mpx_batch *b)
+{
+    if (!b) return fail(MPX_ERR_INVALID, "null batch");
+    MPX_HIP(hipStreamSynchronize(b->stream));
+    return MPX_OK;
+}
+
+int mpx_batch_get_time(mpx_batch *b, int *time)
+{
+    if (!b || !time) return fail(MPX_ERR_INVALID, "null argument");
+    *time = b->time;
+    return MPX_OK;
+}
+
+int mpx_batch_set_tuning(mpx_batch *b, const char *key, int value)
+{
+    if (!b || !key) return fail(MPX_ERR_INVALID, "null argument");
+    if (!std::strcmp(key, "variant")) {
+        if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "variant must be 0, 1 or 2");
+        b->variant = value;
+    } else if (!std::strcmp(key, "seg")) {
+        if (value < 1) return fail(MPX_ERR_INVALID, "seg must be >= 1");
+        b->seg = value;
+    } else if (!std::strcmp(key, "blocks_per_cu")) {
+        if (value < 1 || value > 16) return fail(MPX_ERR_INVALID, "blocks_per_cu must be 1..16");
+        b->blocks_per_cu = value;
+    } else if (!std::strcmp(key, "graph_steps")) {
+        if (value < 0 || value > 1000) return fail(MPX_ERR_INVALID, "graph_steps must be 0..1000");
+        b->graph_steps = value;
+        b->drop_graphs();
+        return MPX_OK;
+    } else
+        return fail(MPX_ERR_NAME, std::string("unknown batch tuning key ") + key);
+    return batch_prepare(b);
+}
+
+int mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step)
+{
+    if (!b || !ms_per_step || nsteps < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
+    hipEvent_t e0, e1;
+    MPX_HIP(hipEventCreate(&e0));
+    MPX_HIP(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, b->stream);
+    int rc = batch_step(b, nsteps, true);
+    (void)hipEventRecord(e1, b->stream);
+    if (rc == MPX_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(MPX_ERR_HIP, "event sync failed");
+    float ms = 0;
+    if (rc == MPX_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_step = ms / nsteps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Autotuner: times every (block shape, x segment, blocks per CU) candidate for
+// `steps_per_trial` steps on the batch itself, keeps the fastest, then clears every run back
+// to its initial all-zero state at time 0.  Allowed only before the first step.
+int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[3], double *best_ms)
+{
+    if (!b || steps_per_trial < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
+    if (b->time != 0) return fail(MPX_ERR_INVALID, "autotune runs before the first step only");
+    const int variants[3] = {2, 1, 0}, segs[4] = {16, 32, 64, 128}, bpc[3] = {1, 2, 4};
+    double best = 1e300;
+    int bv = b->variant, bs = b->seg, bb = b->blocks_per_cu;
+    hipEvent_t e0, e1;
+    MPX_HIP(hipEventCreate(&e0));
+    MPX_HIP(hipEventCreate(&e1));
+    int rc = MPX_OK;
+    for (int v : variants)
+        for (int sg : segs)
+            for (int k : bpc) {
+                b->variant = v;
+                b->seg = sg;
+                b->blocks_per_cu = k;
+                if ((rc = batch_prepare(b))) break;
+                if ((rc = batch_step(b, 1, false))) break;  // warm-up
+                (void)hipEventRecord(e0, b->stream);
+                if ((rc = batch_step(b, steps_per_trial, false))) break;
+                (void)hipEventRecord(e1, b->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(MPX_ERR_HIP, "event sync failed");
+                    break;
+                }
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    bv = v;
+                    bs = sg;
+                    bb = k;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    for (mpx_solver *s : b->runs)
+        if ((rc = reset_state(s, b->stream))) return rc;
+    MPX_HIP(hipMemsetAsync(b->d_time, 0, sizeof(int), b->stream));
+    MPX_HIP(hipStreamSynchronize(b->stream));
+    b->time = 0;
+    b->variant = bv;
+    b->seg = bs;
+    b->blocks_per_cu = bb;
+    if ((rc = batch_prepare(b))) return rc;
+    if (result) {
+        result[0] = bv;
+        result[1] = bs;
+        result[2] = bb;
+    }
+    if (best_ms) *best_ms = best / steps_per_trial;
     return MPX_OK;
 }
 

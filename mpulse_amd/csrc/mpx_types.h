@@ -145,4 +145,10 @@ struct SweepArgs {
     int variant;   // thread-block shape
 };
 
+// one run of a batch: its tile grid and the number of its first tile in the batch's list
+struct BatchRun {
+    int first;
+    int ntz, nty, nseg;
+};
+
 }  // namespace mpx

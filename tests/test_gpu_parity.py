@@ -116,7 +116,7 @@ def test_sweep_path_against_oracle():
 
 @pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE, abi.PATH_SWEEP])
 def test_focused_pulse_against_oracle(path):
-    """Focused pulse (NA 0.4) into a Lorentz + Kerr medium behind a CPML, 120 steps."""
+    """Focused pulse (NA 0.4) into the two-pole silica medium + Kerr behind a CPML, 120 steps."""
     d = abi.desc_defaults()
     d.nx, d.ny, d.nz = 40, 36, 48
     d.dx = d.dy = d.dz = configs.DX
@@ -124,14 +124,15 @@ def test_focused_pulse_against_oracle(path):
     d.solver = abi.SOLVER_PLRC_NONLIN
     d.path = path
     d.eps, d.chi = 1.0, 2e-3
-    d.l_eps[0], d.l_om2[0], d.l_delta[0] = 1.1, 4.0, 0.005
+    for q, (deps, om2, delta) in enumerate(configs.SILICA_POLES):
+        d.l_eps[q], d.l_om2[q], d.l_delta[q] = deps, om2, delta
     cp = abi.factory_defaults(abi.FACTORY_CPML)
     cp.d = 6
     cp.sigma_max = configs.sigma_max(6)
     abi.add_factory(d, cp)
     f = abi.factory_defaults(abi.FACTORY_FOCUSED_PULSE)
     f.d, f.width, f.om0, f.length, f.tshift, f.amp = 9, 5.0, 1.0, 4.0, 6.0, 2.0
-    f.eps = configs.medium_eps_at_laser(1.0, ((1.1, 4.0, 0.005),))
+    f.eps = configs.medium_eps_at_laser(1.0, configs.SILICA_POLES)
     f.zshift = -6 * configs.DX
     f.phase = 0.3
     abi.add_factory(d, f)
