@@ -6,9 +6,11 @@ Workload: 512^3 Yee grid, one Lorentz pole (deps 1.1, Om = 2 omega_L) + Kerr (ch
 A step is one full Yee step (E side then B side) of the whole grid; value = cell-updates/s
 = Nx*Ny*Nz*steps / time, with all state resident in HBM.  The float64 fused path and the
 float64 reference-structured path are timed beside it, and the CPU oracle (OpenMP) is timed
-on a bounded sample of the same workload.
+on a bounded sample of the same workload.  The headline form is the faster of the two
+FP32-bulk kernel structures (two fused kernels per step, or the single-pass sweep).  The
+64-run batched parameter sweep is measured beside it (see batch_sweep).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64] [--no-batch]
 """
 import argparse
 import json
@@ -132,6 +134,16 @@ def load_traffic(name):
         return json.load(f)
 
 
+FORMS = {
+    # name: (precision, path, dominant kernel class)
+    "fp32_bulk_fused": (1, 0, "fused_E"),
+    "fp32_bulk_sweep": (1, 2, "sweep"),
+    "fp64_fused": (0, 0, "fused_E"),
+    "fp64_sweep": (0, 2, "sweep"),
+    "fp64_reference": (0, 1, "stepD_ref"),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +152,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--forms", default="", help="comma list of fp32_bulk_fused, fp64_fused, fp64_reference")
+    ap.add_argument("--forms", default="", help="comma list of " + ", ".join(FORMS))
     ap.add_argument("--copy-probe", action="store_true", help="also time a 4 GiB device copy")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-run batched sweep")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-run batched sweep")
@@ -154,17 +166,20 @@ def main():
 
     from mpulse_amd import abi, configs
     torch.cuda.set_device(0)
+    if args.batch_only:
+        print(json.dumps({"batch_sweep": batch_sweep()}))
+        return
     n = args.n
-    results = {}
-    all_forms = [("fp32_bulk_fused", abi.PREC_FP32_BULK, abi.PATH_FUSED), ("fp64_fused", abi.PREC_FP64, abi.PATH_FUSED),
-                 ("fp64_reference", abi.PREC_FP64, abi.PATH_REFERENCE)]
     if args.forms:
-        want = set(args.forms.split(","))
-        forms = [f for f in all_forms if f[0] in want or f[0] == "fp32_bulk_fused"]
+        names = [f for f in args.forms.split(",") if f in FORMS]
+    elif args.no_fp64:
+        names = ["fp32_bulk_fused", "fp32_bulk_sweep"]
     else:
-        forms = all_forms[:1] if args.no_fp64 else all_forms
+        names = list(FORMS)
     cells = n ** 3
-    for name, prec, path in forms:
+    results = {}
+    for name in names:
+        prec, path, _ = FORMS[name]
         d, meta = configs.config_roofline(precision=prec, path=path, n=n)
         wall_ms, prof = run_gpu(d, args.steps, args.warmup)
         results[name] = {"ms_per_step": wall_ms, "cell_updates_per_s": cells / (wall_ms * 1e-3),
@@ -172,14 +187,22 @@ def main():
         print(f"# {name}: {wall_ms:.3f} ms/step, {cells / (wall_ms * 1e-3) / 1e9:.2f} G cell-updates/s, "
               f"kernels {prof['kernels']}", file=sys.stderr, flush=True)
 
-    main_res = results["fp32_bulk_fused"]
-    d, meta = configs.config_roofline(precision=abi.PREC_FP32_BULK, n=n)
+    fp32 = [f for f in ("fp32_bulk_sweep", "fp32_bulk_fused") if f in results]
+    if not fp32:
+        raise SystemExit("the headline needs an fp32_bulk form")
+    head = max(fp32, key=lambda f: results[f]["cell_updates_per_s"])
+    main_res = results[head]
+    prec, path, kclass = FORMS[head]
+    d, meta = configs.config_roofline(precision=prec, path=path, n=n)
     eb, bb = kernel_bytes(d, meta["d_cpml"], 1)
-    km = main_res["kernels_ms"]
-    e_ms = km.get("fused_E")
-    achieved = eb / (e_ms * 1e-3) / 1e9
-    traffic = load_traffic(f"fused_E_fp32_{n}_traffic.json")
-    step_bytes = eb + bb
+    wf = 4 if prec == 1 else 8
+    # the single-pass kernel reads E and B once for both halves of the step
+    step_bytes = eb + bb - 2 * 3 * wf * cells
+    k_bytes = step_bytes if path == 2 else eb
+    k_ms = main_res["kernels_ms"][kclass]
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    traffic = load_traffic(f"{kclass}_fp32_{n}_traffic.json")
+    alg_step = step_bytes if path == 2 else eb + bb
     out = {
         "metric": METRIC,
         "value": main_res["cell_updates_per_s"],
@@ -194,25 +217,26 @@ def main():
         "dtype": "fp32-bulk (f32 E/B/CPML storage, f64 PLRC psi and Kerr solve, f64 arithmetic)",
         "data": "synthetic (zero initial fields, analytic short-pulse source)",
         "config": {"workload": f"{n}^3 PLRC+Kerr, 1 Lorentz pole, 16-cell CPML, 20 fs short pulse, FP32-bulk",
-                   "grid": [n, n, n], "cpml": 16, "poles": 1, "kerr": True, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "fused_E", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "grid": [n, n, n], "cpml": 16, "poles": 1, "kerr": True, "path": head,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kclass, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "algorithmic_bytes_per_launch": eb, "mean_launch_ms": e_ms},
-        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
-                          "achieved_GBs": step_bytes / (main_res["ms_per_step"] * 1e-3) / 1e9,
-                          "frac": step_bytes / (main_res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "algorithmic_bytes_per_launch": k_bytes, "mean_launch_ms": k_ms},
+        "step_roofline": {"algorithmic_bytes_per_step": alg_step,
+                          "achieved_GBs": alg_step / (main_res["ms_per_step"] * 1e-3) / 1e9,
+                          "frac": alg_step / (main_res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "forms": results,
     }
     if args.copy_probe:
         nb = 4 << 30
         ms = abi.copy_probe(nb, 20)
         out["copy_probe"] = {"bytes_moved": 2 * nb, "ms": ms, "GBs": 2 * nb / (ms * 1e-3) / 1e9}
-    if "fp64_fused" in results:
-        out["fp32_over_fp64_fused"] = main_res["cell_updates_per_s"] / results["fp64_fused"]["cell_updates_per_s"]
-    if "fp64_reference" in results:
-        out["fp32_over_fp64_reference_path"] = (main_res["cell_updates_per_s"]
-                                                / results["fp64_reference"]["cell_updates_per_s"])
+    for ref in ("fp64_reference", "fp64_fused", "fp64_sweep"):
+        if ref in results:
+            out[f"fp32_bulk_over_{ref}"] = main_res["cell_updates_per_s"] / results[ref]["cell_updates_per_s"]
+    if not args.no_batch:
+        out["batch_sweep"] = batch_sweep()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -185,8 +185,8 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
 
 /* Launch-shape parameters: "march_e" / "march_b" = x planes walked by one thread in the
    E-side / B-side fused kernel (1, 2, 4); "sweep_seg" = x planes per block of the sweep kernel;
-   "sweep_variant" = its thread-block shape (0: 64x8, 1: 64x8 with two rows per thread,
-   2: 64x16).  No counterpart in the reference: the hook of
+   "sweep_variant" = its thread-block shape (0: 64x8, 1: 64x4, 2: 64x16 threads; one
+   tile row per thread, the last row and lane of a tile being its halo).  No counterpart in the reference: the hook of
    the tile autotuner. */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
 

@@ -297,7 +297,16 @@ template void launch_wrap<float>(const GridDims &, float *const[3], int, hipStre
 // FDTD_PLRCLinCore::plrcStepD    src/fdtd_plrc.cpp:94-261.
 // d* are the B differences in the reference's order, kd* the kappa*delta factors.
 // ------------------------------------------------------------------------------------
-template <int NP, bool NONLIN, typename PIN, typename POUT>
+// quotient x / k, or x * k when k already holds the reciprocal (RECIP)
+template <bool RECIP>
+__device__ __forceinline__ double qt(double x, double k)
+{
+    return RECIP ? x * k : x / k;
+}
+
+// RECIP: kdx, kdy, kdz and denom hold reciprocals (1/(kappa delta), 1/denom) and the divisions
+// become products; otherwise the reference's quotients are formed.
+template <int NP, bool NONLIN, bool RECIP, typename PIN, typename POUT>
 __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
                                                  double dBx_z, double dBz_x, double dBy_x, double dBx_y,
                                                  double kdx, double kdy, double kdz, double jx, double jy,
@@ -325,11 +334,11 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
     if (NONLIN) {
         double E2 = ex * ex + ey * ey + ez * ez;
         double f = numer + chi * E2;
-        double cx = (f * ex + dt * ((dBz_y / kdy - dBy_z / kdz) + jx) + Psix) / denom;
-        double cy = (f * ey + dt * ((dBx_z / kdz - dBz_x / kdx) + jy) + Psiy) / denom;
-        double cz = (f * ez + dt * ((dBy_x / kdx - dBx_y / kdy) + jz) + Psiz) / denom;
+        double cx = qt<RECIP>(f * ex + dt * ((qt<RECIP>(dBz_y, kdy) - qt<RECIP>(dBy_z, kdz)) + jx) + Psix, denom);
+        double cy = qt<RECIP>(f * ey + dt * ((qt<RECIP>(dBx_z, kdz) - qt<RECIP>(dBz_x, kdx)) + jy) + Psiy, denom);
+        double cz = qt<RECIP>(f * ez + dt * ((qt<RECIP>(dBy_x, kdx) - qt<RECIP>(dBx_y, kdy)) + jz) + Psiz, denom);
         double E = sqrt(E2);
-        double A = chi / denom;
+        double A = qt<RECIP>(chi, denom);
         double Cm = sqrt(cx * cx + cy * cy + cz * cz);
         if (Cm > 0) {
             // the reference's stopping rule |E - Eold| <= 1e-9; the cap only guards a cell that
@@ -341,32 +350,39 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
                 double Q = E * E;
                 E = (2 * A * Q * E + Cm) / (3 * A * Q + 1);
             } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
-            ex = E * cx / Cm;
-            ey = E * cy / Cm;
-            ez = E * cz / Cm;
+            if (RECIP) {
+                const double sc = E / Cm;
+                ex = sc * cx;
+                ey = sc * cy;
+                ez = sc * cz;
+            } else {
+                ex = E * cx / Cm;
+                ey = E * cy / Cm;
+                ez = E * cz / Cm;
+            }
         } else {
             ex = 0.0;
             ey = 0.0;
             ez = 0.0;
         }
     } else {
-        double exn = (numer * ex + (dt * ((dBz_y / kdy - dBy_z / kdz) + jx) + Psix)) / denom;
-        double eyn = (numer * ey + (dt * ((dBx_z / kdz - dBz_x / kdx) + jy) + Psiy)) / denom;
-        double ezn = (numer * ez + (dt * ((dBy_x / kdx - dBx_y / kdy) + jz) + Psiz)) / denom;
+        double exn = qt<RECIP>(numer * ex + (dt * ((qt<RECIP>(dBz_y, kdy) - qt<RECIP>(dBy_z, kdz)) + jx) + Psix), denom);
+        double eyn = qt<RECIP>(numer * ey + (dt * ((qt<RECIP>(dBx_z, kdz) - qt<RECIP>(dBz_x, kdx)) + jy) + Psiy), denom);
+        double ezn = qt<RECIP>(numer * ez + (dt * ((qt<RECIP>(dBy_x, kdx) - qt<RECIP>(dBx_y, kdy)) + jz) + Psiz), denom);
         ex = exn;
         ey = eyn;
         ez = ezn;
     }
 }
 
-template <int NP, bool NONLIN>
+template <int NP, bool NONLIN, bool RECIP>
 __device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
                                               double dBx_z, double dBz_x, double dBy_x, double dBx_y,
                                               double kdx, double kdy, double kdz, double jx, double jy, double jz,
                                               double2 *const psi[3][3], size_t c, const PoleConsts &pc,
                                               double dt, double denom, double numer, double chi)
 {
-    plrc_e_update_pp<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kdx, kdy, kdz, jx, jy, jz, psi,
+    plrc_e_update_pp<NP, NONLIN, RECIP>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kdx, kdy, kdz, jx, jy, jz, psi,
                                  psi, c, true, pc, dt, denom, numer, chi);
 }
 
@@ -395,8 +411,8 @@ __global__ __launch_bounds__(256) void k_ref_stepD(StepArgs<double> a)
         jz = a.Jt[2][c];
     }
     double ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
-    plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
-                              a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
+    plrc_e_update<NP, NONLIN, false>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
+                                     a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
     a.E[0][c] = ex;
     a.E[1][c] = ey;
     a.E[2][c] = ez;
@@ -510,11 +526,11 @@ __device__ __forceinline__ void cpml_face(const CpmlFace &f, int coord, int l, d
     double bq = f.b[q], cq = f.c[q];
     T n0, n1;
     if (HSIDE) {
-        n0 = (T)(bq * (double)p0[l] + cq * d1 / f.dxn);
-        n1 = (T)(bq * (double)p1[l] - cq * d0 / f.dxn);
+        n0 = (T)(bq * (double)p0[l] + cq * d1 * f.idxn);
+        n1 = (T)(bq * (double)p1[l] - cq * d0 * f.idxn);
     } else {
-        n0 = (T)(bq * (double)p0[l] - cq * d1 / f.dxn);
-        n1 = (T)(bq * (double)p1[l] + cq * d0 / f.dxn);
+        n0 = (T)(bq * (double)p0[l] - cq * d1 * f.idxn);
+        n1 = (T)(bq * (double)p1[l] + cq * d0 * f.idxn);
     }
     p0[l] = n0;
     p1[l] = n1;
@@ -557,14 +573,15 @@ __device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int
     bz = t / nby;
 }
 
-// The per-thread CPML faces that do not change along x: returns the face index for the
-// high side ([0]) and low side ([1]) of dimension dim at coordinate coord, or -1.
-__device__ __forceinline__ void faces_at(const int fbd[3][2], const CpmlFace *faces, int dim, int coord, int out[2])
+// Whether coordinate coord of dimension dim lies in the layer of the high-side ([0]) and
+// low-side ([1]) CPML face of that dimension.  The face index itself stays uniform across
+// the wave (fbd[dim][s]), so the face descriptors are read with scalar loads.
+__device__ __forceinline__ void in_faces(const int fbd[3][2], const CpmlFace *faces, int dim, int coord, bool out[2])
 {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         int f = fbd[dim][s];
-        out[s] = (f >= 0 && coord >= faces[f].lo[dim] && coord <= faces[f].hi[dim]) ? f : -1;
+        out[s] = f >= 0 && coord >= faces[f].lo[dim] && coord <= faces[f].hi[dim];
     }
 }
 
@@ -588,12 +605,12 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
     const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
     const int cim = (i0 == 1 && a.periodic[0]) ? c + (g.n[0] - 1) * sx : c - sx;
     double bz_im = (double)Bz[cim], by_im = (double)By[cim];
-    int fy[2] = {-1, -1}, fz[2] = {-1, -1};
+    bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
-        faces_at(a.face_by_dim, a.face, 1, j, fy);
-        faces_at(a.face_by_dim, a.face, 2, k, fz);
+        in_faces(a.face_by_dim, a.face, 1, j, iny);
+        in_faces(a.face_by_dim, a.face, 2, k, inz);
     }
-    const double kdy = a.kdE[1][j], kdz = a.kdE[2][k];
+    const double kdy = a.ikdE[1][j], kdz = a.ikdE[2][k];
 #pragma unroll
     for (int u = 0; u < L; ++u, c += sx) {
         const int i = i0 + u;
@@ -614,19 +631,19 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
             // dimension the high face comes first in that list
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fy[s] >= 0) {
-                    const CpmlFace &f = a.face[fy[s]];
+                if (iny[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[1][s]];
                     double v0, v1;
                     cpml_face<T, false>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
                     jz += v0;
                     jx += v1;
                 }
-            int fx[2];
-            faces_at(a.face_by_dim, a.face, 0, i, fx);
+            bool inx[2];
+            in_faces(a.face_by_dim, a.face, 0, i, inx);
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fx[s] >= 0) {
-                    const CpmlFace &f = a.face[fx[s]];
+                if (inx[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[0][s]];
                     double v0, v1;
                     cpml_face<T, false>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
                     jy += v0;
@@ -634,8 +651,8 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
                 }
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fz[s] >= 0) {
-                    const CpmlFace &f = a.face[fz[s]];
+                if (inz[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[2][s]];
                     double v0, v1;
                     cpml_face<T, false>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
                     jx += v0;
@@ -644,8 +661,8 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
         double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
-        plrc_e_update<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], kdy, kdz, jx,
-                                  jy, jz, a.psi, (size_t)c, a.pc, a.dt, a.denom, a.numer, a.chi);
+        plrc_e_update<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy, kdz,
+                                        jx, jy, jz, a.psi, (size_t)c, a.pc, a.dt, a.idenom, a.numer, a.chi);
         a.E[0][c] = (T)ex;
         a.E[1][c] = (T)ey;
         a.E[2][c] = (T)ez;
@@ -672,12 +689,12 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
     const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
     T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
     double ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    int fy[2] = {-1, -1}, fz[2] = {-1, -1};
+    bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
-        faces_at(a.face_by_dim, a.face, 1, j, fy);
-        faces_at(a.face_by_dim, a.face, 2, k, fz);
+        in_faces(a.face_by_dim, a.face, 1, j, iny);
+        in_faces(a.face_by_dim, a.face, 2, k, inz);
     }
-    const double kdy = a.kdH[1][j], kdz = a.kdH[2][k];
+    const double kdy = a.ikdH[1][j], kdz = a.ikdH[2][k];
 #pragma unroll
     for (int u = 0; u < L; ++u, c += sx) {
         const int i = i0 + u;
@@ -695,19 +712,19 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
         if (CPML) {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fy[s] >= 0) {
-                    const CpmlFace &f = a.face[fy[s]];
+                if (iny[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[1][s]];
                     double v0, v1;  // psi0 -> Mz from dEx_y, psi1 -> Mx from dEz_y
                     cpml_face<T, true>(f, j, layer_index(f, i, j, k), dEx_y, dEz_y, v0, v1);
                     mz += v0;
                     mx += v1;
                 }
-            int fx[2];
-            faces_at(a.face_by_dim, a.face, 0, i, fx);
+            bool inx[2];
+            in_faces(a.face_by_dim, a.face, 0, i, inx);
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fx[s] >= 0) {
-                    const CpmlFace &f = a.face[fx[s]];
+                if (inx[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[0][s]];
                     double v0, v1;  // psi0 -> My from dEz_x, psi1 -> Mz from dEy_x
                     cpml_face<T, true>(f, i, layer_index(f, i, j, k), dEz_x, dEy_x, v0, v1);
                     my += v0;
@@ -715,8 +732,8 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
                 }
 #pragma unroll
             for (int s = 0; s < 2; ++s)
-                if (fz[s] >= 0) {
-                    const CpmlFace &f = a.face[fz[s]];
+                if (inz[s]) {
+                    const CpmlFace &f = a.face[a.face_by_dim[2][s]];
                     double v0, v1;  // psi0 -> Mx from dEy_z, psi1 -> My from dEx_z
                     cpml_face<T, true>(f, k, layer_index(f, i, j, k), dEy_z, dEx_z, v0, v1);
                     mx += v0;
@@ -724,10 +741,10 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
                 }
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
-        const double kdx = a.kdH[0][i];
-        Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z / kdz - dEz_y / kdy) + mx));
-        By[c] = (T)((double)By[c] + a.dt * ((dEz_x / kdx - dEx_z / kdz) + my));
-        Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+        const double kdx = a.ikdH[0][i];
+        Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx));
+        By[c] = (T)((double)By[c] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my));
+        Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz));
         ex = ex1;
         ey = ey1;
         ez = ez1;
@@ -1050,6 +1067,22 @@ __device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py,
 //
 // sweep_tile is one tile's walk; k_sweep maps the blocks of one grid onto tiles, k_sweep_batch
 // walks the tiles of many independent grids from one persistent launch.
+// accumulators of the active poles held in registers, indexable like the psi pointer table
+// (psi[n][component][cell]) so that the shared E update reads them in place
+template <int NP>
+struct PsiRegs {
+    double2 v[NP > 0 ? NP : 1][3];
+    struct Comp {
+        double2 x;
+        __device__ double2 operator[](size_t) const { return x; }
+    };
+    struct Pole {
+        const double2 *p;
+        __device__ Comp operator[](int q) const { return Comp{p[q]}; }
+    };
+    __device__ Pole operator[](int n) const { return Pole{v[n]}; }
+};
+
 template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
 __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty, int tx,
                                            T (&lds)[3][3][BY * RPT][64])
@@ -1094,10 +1127,60 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
             bprev[r][2] = Bzo[c];
         }
     }
-    const double kdz_e = a.kdE[2][kk], kdz_h = a.kdH[2][kk];
+    const double kdz_e = a.ikdE[2][kk], kdz_h = a.ikdH[2][kk];  // reciprocals
+    // the grid inputs of a plane (B and its y / z neighbours, E, the pole accumulators) are
+    // loaded one plane ahead, so that their latency overlaps the work and the barrier of the
+    // plane before
+    T nb[RPT][7], ne[RPT][3];
+    PsiRegs<NP> npsi[RPT];
+    auto load_plane = [&](int p) {
+        const int pp = (p == Nx + 1) ? 1 : p;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            if (!eact[r]) continue;
+            const int c = pp * sx + jj[r] * pz + kk + g.z0;
+            nb[r][0] = Bxo[c];
+            nb[r][1] = Byo[c];
+            nb[r][2] = Bzo[c];
+            nb[r][3] = Bzo[c + ojm[r]];
+            nb[r][4] = Bxo[c + ojm[r]];
+            nb[r][5] = Byo[c + okm];
+            nb[r][6] = Bxo[c + okm];
+            ne[r][0] = a.Eo[0][c];
+            ne[r][1] = a.Eo[1][c];
+            ne[r][2] = a.Eo[2][c];
+#pragma unroll
+            for (int n = 0; n < NP; ++n)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = a.psio[n][q][c];
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) nb[r][q] = (T)0;
+        ne[r][0] = ne[r][1] = ne[r][2] = (T)0;
+#pragma unroll
+        for (int n = 0; n < NP; ++n)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = make_double2(0.0, 0.0);
+    }
+    load_plane(i0);
     for (int p = i0; p <= i1 + 1; ++p) {
         const int pp = (p == Nx + 1) ? 1 : p;
         const int buf = p % 3;
+        T cb[RPT][7], ce[RPT][3];
+        PsiRegs<NP> cpsi[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+            for (int q = 0; q < 7; ++q) cb[r][q] = nb[r][q];
+            ce[r][0] = ne[r][0];
+            ce[r][1] = ne[r][1];
+            ce[r][2] = ne[r][2];
+            cpsi[r] = npsi[r];
+        }
+        if (p <= i1) load_plane(p + 1);
         double ecur[RPT][3], bcur[RPT][3];
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
@@ -1106,11 +1189,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
             if (!eact[r]) continue;
             const int j = jj[r];
             const int c = pp * sx + j * pz + kk + g.z0;
-            const double bx = Bxo[c], by = Byo[c], bz = Bzo[c];
-            const double dBz_y = bz - (double)Bzo[c + ojm[r]];
-            const double dBx_y = bx - (double)Bxo[c + ojm[r]];
-            const double dBy_z = by - (double)Byo[c + okm];
-            const double dBx_z = bx - (double)Bxo[c + okm];
+            const double bx = cb[r][0], by = cb[r][1], bz = cb[r][2];
+            const double dBz_y = bz - (double)cb[r][3];
+            const double dBx_y = bx - (double)cb[r][4];
+            const double dBy_z = by - (double)cb[r][5];
+            const double dBx_z = bx - (double)cb[r][6];
             const double dBz_x = bz - bprev[r][2];
             const double dBy_x = by - bprev[r][1];
             bcur[r][0] = bx;
@@ -1136,8 +1219,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
                         const T *o0 = (const T *)a.epsi_old[fi][0], *o1 = (const T *)a.epsi_old[fi][1];
                         double d1 = dim == 0 ? dBz_x : (dim == 1 ? dBx_y : dBy_z);
                         double d0 = dim == 0 ? dBy_x : (dim == 1 ? dBz_y : dBx_z);
-                        T n0 = (T)(f.b[q] * (double)o0[l] - f.c[q] * d1 / f.dxn);
-                        T n1 = (T)(f.b[q] * (double)o1[l] + f.c[q] * d0 / f.dxn);
+                        T n0 = (T)(f.b[q] * (double)o0[l] - f.c[q] * d1 * f.idxn);
+                        T n1 = (T)(f.b[q] * (double)o1[l] + f.c[q] * d0 * f.idxn);
                         if (store) {
                             ((T *)f.psi0)[l] = n0;
                             ((T *)f.psi1)[l] = n1;
@@ -1156,10 +1239,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
                 }
             }
             if (!a.sheets_first) add_sheet_planes(a.sheet, a.nsheets, g.py, pp, j, kk, jx, jy, jz);
-            double ex = (double)a.Eo[0][c], ey = (double)a.Eo[1][c], ez = (double)a.Eo[2][c];
-            plrc_e_update_pp<NP, NONLIN>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][pp],
-                                         a.kdE[1][j], kdz_e, jx, jy, jz, a.psio, a.psin, c, store, a.pc, a.dt,
-                                         a.denom, a.numer, a.chi);
+            double ex = (double)ce[r][0], ey = (double)ce[r][1], ez = (double)ce[r][2];
+            plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y,
+                                               a.ikdE[0][pp], a.ikdE[1][j], kdz_e, jx, jy, jz, cpsi[r], a.psin, c,
+                                               store, a.pc, a.dt, a.idenom, a.numer, a.chi);
             const T tx_ = (T)ex, ty_ = (T)ey, tz_ = (T)ez;
             if (store) {
                 a.En[0][c] = tx_;
@@ -1208,8 +1291,8 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
                             T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
                             double d1 = dim == 0 ? dEz_x : (dim == 1 ? dEx_y : dEy_z);
                             double d0 = dim == 0 ? dEy_x : (dim == 1 ? dEz_y : dEx_z);
-                            T n0 = (T)(f.b[q] * (double)p0[l] + f.c[q] * d1 / f.dxn);
-                            T n1 = (T)(f.b[q] * (double)p1[l] - f.c[q] * d0 / f.dxn);
+                            T n0 = (T)(f.b[q] * (double)p0[l] + f.c[q] * d1 * f.idxn);
+                            T n1 = (T)(f.b[q] * (double)p1[l] - f.c[q] * d0 * f.idxn);
                             p0[l] = n0;
                             p1[l] = n1;
                             if (dim == 0) {
@@ -1226,10 +1309,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
                     }
                 }
                 if (!a.sheets_first) add_sheet_planes(a.hsheet, a.nhsheets, g.py, pb, j, kk, mx, my, mz);
-                const double kdx = a.kdH[0][pb], kdy = a.kdH[1][j];
-                a.Bn[0][c] = (T)(bprev[r][0] + a.dt * ((dEy_z / kdz_h - dEz_y / kdy) + mx));
-                a.Bn[1][c] = (T)(bprev[r][1] + a.dt * ((dEz_x / kdx - dEx_z / kdz_h) + my));
-                a.Bn[2][c] = (T)(bprev[r][2] + a.dt * ((dEx_y / kdy - dEy_x / kdx) + mz));
+                const double kdx = a.ikdH[0][pb], kdy = a.ikdH[1][j];
+                a.Bn[0][c] = (T)(bprev[r][0] + a.dt * ((dEy_z * kdz_h - dEz_y * kdy) + mx));
+                a.Bn[1][c] = (T)(bprev[r][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz_h) + my));
+                a.Bn[2][c] = (T)(bprev[r][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz));
             }
         }
 #pragma unroll
@@ -1282,8 +1365,8 @@ static void launch_sweep_np(const SweepArgs<T> &a, hipStream_t s)
 {
     int ntz, nty, nseg;
     sweep_tiles(a.g, a.variant, a.seg, ntz, nty, nseg);
-    if (a.variant == 1)  // 64 x 8 threads, 2 rows each: 16-row tiles
-        hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 8, 2>), dim3(ntz, nty, nseg), dim3(64, 8), 0, s, a, ntz, nty,
+    if (a.variant == 1)  // 64 x 4 threads: 4-row tiles
+        hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 4, 1>), dim3(ntz, nty, nseg), dim3(64, 4), 0, s, a, ntz, nty,
                            nseg);
     else if (a.variant == 2)  // 64 x 16 threads, 1 row each
         hipLaunchKernelGGL((k_sweep<T, NP, NONLIN, CPML, 16, 1>), dim3(ntz, nty, nseg), dim3(64, 16), 0, s, a, ntz,
@@ -1297,7 +1380,7 @@ static void launch_sweep_np(const SweepArgs<T> &a, hipStream_t s)
 void sweep_tiles(const GridDims &g, int variant, int seg, int &ntz, int &nty, int &nseg)
 {
     ntz = (g.n[2] + 62) / 63;
-    nty = variant == 0 ? (g.n[1] + 6) / 7 : (g.n[1] + 14) / 15;
+    nty = variant == 0 ? (g.n[1] + 6) / 7 : (variant == 1 ? (g.n[1] + 2) / 3 : (g.n[1] + 14) / 15);
     nseg = (g.n[0] + seg - 1) / seg;
 }
 
@@ -1306,7 +1389,7 @@ static void launch_sweep_batch_np(const SweepArgs<T> *args, const BatchRun *runs
                                   int blocks, hipStream_t s)
 {
     if (variant == 1)
-        hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 8, 2>), dim3(blocks), dim3(64, 8), 0, s, args, runs,
+        hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 4, 1>), dim3(blocks), dim3(64, 4), 0, s, args, runs,
                            nruns, total);
     else if (variant == 2)
         hipLaunchKernelGGL((k_sweep_batch<T, NP, NONLIN, CPML, 16, 1>), dim3(blocks), dim3(64, 16), 0, s, args, runs,

@@ -115,6 +115,7 @@ struct mpx_solver {
     int pole_ref[3] = {-1, -1, -1};  // active pole slot -> reference pole index
     std::vector<double> kE[3], kH[3];
     double *d_kdE[3] = {}, *d_kdH[3] = {};
+    double *d_ikdE[3] = {}, *d_ikdH[3] = {};  // reciprocals, for the fused and sweep kernels
 
     std::vector<Layer> layers;
     std::vector<Current> ecur, hcur;
@@ -136,7 +137,7 @@ struct mpx_solver {
 
     double *d_part = nullptr, *d_energy = nullptr;
     int time = 0;
-    int march_e = 1, march_b = 1;  // x planes per thread in the fused kernels (1, 2 or 4)
+    int march_e = 1, march_b = 4;  // x planes per thread in the fused kernels (1, 2 or 4)
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
     double2 *psi2[3][3] = {};
@@ -694,6 +695,7 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     else if (dim == 1) { F.c0 = 2; F.c1 = 0; F.f0 = 2; F.f1 = 0; }
                     else { F.c0 = 0; F.c1 = 1; F.f0 = 0; F.f1 = 1; }
                     F.dxn = s->delta[dim];
+                    F.idxn = 1.0 / F.dxn;
                     size_t cnt = s->layers[l0].count();
                     int lc[3] = {l0, l1, l2};
                     for (int w = 0; w < 2; ++w) {
@@ -849,10 +851,19 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                 kdE[q] = s->kE[d][q] * s->delta[d];
                 kdH[q] = s->kH[d][q] * s->delta[d];
             }
+            std::vector<double> ikdE(kdE.size()), ikdH(kdH.size());
+            for (size_t q = 0; q < kdE.size(); ++q) {
+                ikdE[q] = 1.0 / kdE[q];
+                ikdH[q] = 1.0 / kdH[q];
+            }
             s->d_kdE[d] = s->dalloc<double>(kdE.size(), rc);
             s->d_kdH[d] = s->dalloc<double>(kdH.size(), rc);
+            s->d_ikdE[d] = s->dalloc<double>(kdE.size(), rc);
+            s->d_ikdH[d] = s->dalloc<double>(kdH.size(), rc);
             if (rc) return rc;
-            if (upload(s, s->d_kdE[d], kdE.data(), kdE.size() * 8) || upload(s, s->d_kdH[d], kdH.data(), kdH.size() * 8))
+            if (upload(s, s->d_kdE[d], kdE.data(), kdE.size() * 8) || upload(s, s->d_kdH[d], kdH.data(), kdH.size() * 8) ||
+                upload(s, s->d_ikdE[d], ikdE.data(), kdE.size() * 8) ||
+                upload(s, s->d_ikdH[d], ikdH.data(), kdH.size() * 8))
                 return fail(MPX_ERR_HIP, "kappa upload failed");
         }
 
@@ -935,6 +946,8 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
         a.periodic[d] = s->periodic[d];
         a.kdE[d] = s->d_kdE[d];
         a.kdH[d] = s->d_kdH[d];
+        a.ikdE[d] = s->d_ikdE[d];
+        a.ikdH[d] = s->d_ikdH[d];
         a.E[d] = (T *)s->E[d];
         a.B[d] = (T *)s->B[d];
         a.blo[d] = s->periodic[d] ? 1 : 0;
@@ -945,6 +958,7 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.eps = s->d.eps;
     a.chi = s->d.chi;
     a.denom = s->denom;
+    a.idenom = 1.0 / s->denom;
     a.numer = s->numer;
     a.pc = s->pc;
     const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
@@ -1089,6 +1103,8 @@ static mpx::SweepArgs<T> sweep_args(mpx_solver *s)
         a.Bn[c] = (T *)s->B2[c];
         a.kdE[c] = s->d_kdE[c];
         a.kdH[c] = s->d_kdH[c];
+        a.ikdE[c] = s->d_ikdE[c];
+        a.ikdH[c] = s->d_ikdH[c];
         for (int n = 0; n < 3; ++n) {
             a.psio[n][c] = s->psi[n][c];
             a.psin[n][c] = s->psi2[n][c];
@@ -1097,6 +1113,7 @@ static mpx::SweepArgs<T> sweep_args(mpx_solver *s)
     a.dt = s->d.dt;
     a.chi = s->d.chi;
     a.denom = s->denom;
+    a.idenom = 1.0 / s->denom;
     a.numer = s->numer;
     a.pc = s->pc;
     a.nefaces = (int)s->efaces.size();

@@ -38,6 +38,7 @@ struct CpmlFace {
     const double *b;  // [normal - lo[dim]]
     const double *c;
     double dxn;       // grid spacing of the normal dimension
+    double idxn;      // 1 / dxn
     __host__ __device__ __forceinline__ size_t at(int i, int j, int k) const
     {
         return ((size_t)(i - lo[0]) * (size_t)n1 + (size_t)(j - lo[1])) * (size_t)n2 + (size_t)(k - lo[2]);
@@ -100,9 +101,12 @@ struct StepArgs {
     double2 *psi[3][3];  // [active pole][component]
     const double *kdE[3];  // kappaE * delta per dimension (lines 0..n+1)
     const double *kdH[3];
+    const double *ikdE[3];  // their reciprocals (fused kernels)
+    const double *ikdH[3];
     double dt;
     double eps, chi;
     double denom, numer;  // eps + sumChi0 - sumXi0 (+sigma), eps - sumXi0 (-sigma), sigma = 0
+    double idenom;        // 1 / denom
     PoleConsts pc;
     int nfaces;
     CpmlFace face[kMaxFaces];   // E-side faces for the E kernel, H-side faces for the B kernel
@@ -129,7 +133,9 @@ struct SweepArgs {
     double2 *psin[3][3];
     const double *kdE[3];
     const double *kdH[3];
-    double dt, chi, denom, numer;
+    const double *ikdE[3];
+    const double *ikdH[3];
+    double dt, chi, denom, numer, idenom;
     PoleConsts pc;
     int nefaces, nhfaces;
     CpmlFace eface[kMaxFaces];       // psi0/psi1 = new E auxiliaries
