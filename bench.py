@@ -86,8 +86,8 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     """Uneven-size parameter sweep: nruns independent cubes (48^3..160^3, one pole + Kerr,
     10-cell CPML, 500 steps each) stepped together by the batched persistent sweep replayed
     from captured graphs with autotuned tiles, against the same runs launched one after
-    another on the float64 plain (reference-structured) path, and on the float64 fused and
-    sweep paths.  Times cover the steps only (set-up excluded), wall clock around each set."""
+    another on the float64 plain (reference-structured) path and on the float64 fused path.
+    Times cover the steps only (set-up excluded), wall clock around each set."""
     import torch
 
     from mpulse_amd import abi, configs
@@ -107,8 +107,7 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     torch.cuda.empty_cache()
     out["batched"] = {"s": t_batch, "cell_updates_per_s": cells * steps / t_batch, "tuning": tune}
     print(f"# batch: {t_batch:.3f} s for {nruns} runs x {steps} steps, tuning {tune}", file=sys.stderr, flush=True)
-    for name, path in (("sequential_plain", abi.PATH_REFERENCE), ("sequential_fused", abi.PATH_FUSED),
-                       ("sequential_sweep", abi.PATH_SWEEP)):
+    for name, path in (("sequential_plain", abi.PATH_REFERENCE), ("sequential_fused", abi.PATH_FUSED)):
         total = 0.0
         for d in descs:
             d.path = path

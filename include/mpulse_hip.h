@@ -213,13 +213,17 @@ int  mpx_batch_run(mpx_batch *b, int run, mpx_solver **out);
 int  mpx_batch_step(mpx_batch *b, int nsteps);
 int  mpx_batch_sync(mpx_batch *b);
 int  mpx_batch_get_time(mpx_batch *b, int *time);
-/* "variant" (block shape 0..2, as sweep_variant), "seg" (x planes per tile), "blocks_per_cu"
-   (persistent grid = blocks_per_cu x CUs), "graph_steps" (steps per captured graph; 0 = none) */
+/* "structure" (0: fused E and fused B kernels over all runs, the default; 1: single-pass sweep),
+   "march_b" (fused B: x planes per thread, 1 / 2 / 4), "fused_blocks_per_cu" (fused persistent
+   grid = value x CUs), "variant" (sweep block shape 0..2, as sweep_variant), "seg" (sweep x
+   planes per tile), "blocks_per_cu" (sweep persistent grid), "graph_steps" (steps per captured
+   graph; 0 = none) */
 int  mpx_batch_set_tuning(mpx_batch *b, const char *key, int value);
-/* Tries every variant x seg {16,32,64,128} x blocks_per_cu {1,2,4} for steps_per_trial steps,
-   keeps the fastest (result = {variant, seg, blocks_per_cu}), then returns every run to its
-   all-zero state at time 0.  Only before the first step. */
-int  mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[3], double *best_ms_per_step);
+/* Times the candidate launch shapes of both structures for steps_per_trial steps each, keeps
+   the fastest, then returns every run to its all-zero state at time 0.  result = {structure,
+   then march_b, blocks per CU, 0 (fused) or variant, seg, blocks per CU (sweep)}.  Only
+   before the first step. */
+int  mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double *best_ms_per_step);
 /* Advance nsteps and report the mean device time per step (HIP events on the batch stream). */
 int  mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step);
 

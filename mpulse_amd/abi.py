@@ -338,10 +338,13 @@ class Batch:
         self._check(self.lib.mpx_batch_set_tuning(self.h, key.encode(), int(value)))
 
     def autotune(self, steps_per_trial=3):
-        res = (C.c_int * 3)()
+        res = (C.c_int * 4)()
         ms = C.c_double()
         self._check(self.lib.mpx_batch_autotune(self.h, steps_per_trial, res, C.byref(ms)))
-        return {"variant": res[0], "seg": res[1], "blocks_per_cu": res[2], "ms_per_step": ms.value}
+        if res[0] == 0:
+            return {"structure": "fused", "march_b": res[1], "blocks_per_cu": res[2], "ms_per_step": ms.value}
+        return {"structure": "sweep", "variant": res[1], "seg": res[2], "blocks_per_cu": res[3],
+                "ms_per_step": ms.value}
 
     def time_steps(self, nsteps):
         ms = C.c_double()

@@ -512,6 +512,23 @@ void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s)
 // fused path
 // ------------------------------------------------------------------------------------
 
+// accumulators of the active poles held in registers, indexable like the psi pointer table
+// (psi[n][component][cell]) so that the shared E update reads them in place
+template <int NP>
+struct PsiRegs {
+    double2 v[NP > 0 ? NP : 1][3];
+    struct Comp {
+        double2 x;
+        __device__ double2 operator[](size_t) const { return x; }
+    };
+    struct Pole {
+        const double2 *p;
+        __device__ Comp operator[](int q) const { return Comp{p[q]}; }
+    };
+    __device__ Pole operator[](int n) const { return Pole{v[n]}; }
+};
+
+
 // CPML auxiliaries of one face updated inside the fused kernels.  The face map is
 // x: (psi0 -> Jy from dBz, psi1 -> Jz from dBy), y: (psi0 -> Jz from dBx, psi1 -> Jx from dBz),
 // z: (psi0 -> Jx from dBy, psi1 -> Jy from dBx) (src/cpml_border.cpp:357-386, 456-485).
@@ -587,14 +604,33 @@ __device__ __forceinline__ void in_faces(const int fbd[3][2], const CpmlFace *fa
 
 // Fused E side of one step.  A thread owns one (j, k) column segment of L cells along x and
 // walks it plane by plane, keeping B of the previous plane in registers for the x difference.
-template <typename T, int NP, bool NONLIN, bool CPML, int L>
-__global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby, int nbz)
+// (j, k) of this thread in a tile: 64 z cells x 4 y rows, or (LIN) 256 consecutive cells of
+// the y-z plane taken row after row, which keeps every lane busy on grids whose z extent is
+// not a multiple of 64.  lo = first index of the range in y and z.
+template <bool LIN>
+__device__ __forceinline__ void tile_jk(const GridDims &g, const int lo[3], int tbx, int tby, int &j, int &k)
 {
-    int tbx, tby, tbz;
-    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    if (LIN) {
+        const int nk = g.n[2] - lo[2] + 1;
+        const int lin = tbx * 256 + threadIdx.y * 64 + threadIdx.x;
+        const int q = lin / nk;
+        j = q + lo[1];
+        k = lin - q * nk + lo[2];
+    } else {
+        k = tbx * 64 + threadIdx.x + lo[2];
+        j = tby * 4 + threadIdx.y + lo[1];
+    }
+}
+
+// Fused E side of one step.  A thread owns one (j, k) column segment of L cells along x and
+// walks it plane by plane, keeping B of the previous plane in registers for the x difference.
+template <typename T, int NP, bool NONLIN, bool CPML, int L, bool LIN = false>
+__device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int tby, int tbz)
+{
     const GridDims &g = a.g;
-    const int k = tbx * 64 + threadIdx.x + 1;
-    const int j = tby * 4 + threadIdx.y + 1;
+    const int one[3] = {1, 1, 1};
+    int j, k;
+    tile_jk<LIN>(g, one, tbx, tby, j, k);
     const int i0 = tbz * L + 1;
     if (k > g.n[2] || j > g.n[1] || i0 > g.n[0]) return;
     const int i1 = min(i0 + L - 1, g.n[0]);
@@ -616,6 +652,14 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
         const int i = i0 + u;
         if (i > i1) break;
         const double bx = Bx[c], by = By[c], bz = Bz[c];
+        // E and the pole accumulators are read before any store of the cell (the CPML
+        // auxiliaries), so that all its reads leave together
+        double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
+        PsiRegs<NP> pr;
+#pragma unroll
+        for (int n = 0; n < NP; ++n)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) pr.v[n][q] = a.psi[n][q][c];
         const double dBz_y = bz - (double)Bz[c + ojm];
         const double dBx_y = bx - (double)Bx[c + ojm];
         const double dBy_z = by - (double)By[c + okm];
@@ -660,9 +704,9 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
                 }
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
-        double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
-        plrc_e_update<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy, kdz,
-                                        jx, jy, jz, a.psi, (size_t)c, a.pc, a.dt, a.idenom, a.numer, a.chi);
+        plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy,
+                                           kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, a.idenom, a.numer,
+                                           a.chi);
         a.E[0][c] = (T)ex;
         a.E[1][c] = (T)ey;
         a.E[2][c] = (T)ez;
@@ -671,14 +715,12 @@ __global__ __launch_bounds__(256) void k_fused_E(StepArgs<T> a, int nbx, int nby
 
 // Fused B side of one step: a thread walks L planes along x keeping E of the next plane in
 // registers as the centre of the following iteration.
-template <typename T, bool CPML, int L>
-__global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby, int nbz)
+template <typename T, bool CPML, int L, bool LIN = false>
+__device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int tby, int tbz)
 {
-    int tbx, tby, tbz;
-    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
     const GridDims &g = a.g;
-    const int k = tbx * 64 + threadIdx.x + a.blo[2];
-    const int j = tby * 4 + threadIdx.y + a.blo[1];
+    int j, k;
+    tile_jk<LIN>(g, a.blo, tbx, tby, j, k);
     const int i0 = tbz * L + a.blo[0];
     if (k > g.n[2] || j > g.n[1] || i0 > g.n[0]) return;
     const int i1 = min(i0 + L - 1, g.n[0]);
@@ -695,16 +737,35 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
         in_faces(a.face_by_dim, a.face, 2, k, inz);
     }
     const double kdy = a.ikdH[1][j], kdz = a.ikdH[2][k];
+    // the E and B values of all L planes are read before the first store, so that they leave
+    // together (per plane: E at x+1, E_y and E_x at z+1, E_z and E_x at y+1, B)
+    T pe[L][7], pb[L][3];
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+        const int i = i0 + u;
+        if (i > i1) break;
+        const int cu = c + u * sx;
+        const int cip = (i == g.n[0] && a.periodic[0]) ? cu - (g.n[0] - 1) * sx : cu + sx;
+        pe[u][0] = Ex[cip];
+        pe[u][1] = Ey[cip];
+        pe[u][2] = Ez[cip];
+        pe[u][3] = Ey[cu + okp];
+        pe[u][4] = Ex[cu + okp];
+        pe[u][5] = Ez[cu + ojp];
+        pe[u][6] = Ex[cu + ojp];
+        pb[u][0] = Bx[cu];
+        pb[u][1] = By[cu];
+        pb[u][2] = Bz[cu];
+    }
 #pragma unroll
     for (int u = 0; u < L; ++u, c += sx) {
         const int i = i0 + u;
         if (i > i1) break;
-        const int cip = (i == g.n[0] && a.periodic[0]) ? c - (g.n[0] - 1) * sx : c + sx;
-        const double ex1 = Ex[cip], ey1 = Ey[cip], ez1 = Ez[cip];
-        const double dEy_z = (double)Ey[c + okp] - ey;
-        const double dEx_z = (double)Ex[c + okp] - ex;
-        const double dEz_y = (double)Ez[c + ojp] - ez;
-        const double dEx_y = (double)Ex[c + ojp] - ex;
+        const double ex1 = pe[u][0], ey1 = pe[u][1], ez1 = pe[u][2];
+        const double dEy_z = (double)pe[u][3] - ey;
+        const double dEx_z = (double)pe[u][4] - ex;
+        const double dEz_y = (double)pe[u][5] - ez;
+        const double dEx_y = (double)pe[u][6] - ex;
         const double dEz_x = ez1 - ez;
         const double dEy_x = ey1 - ey;
         double mx = 0.0, my = 0.0, mz = 0.0;
@@ -742,24 +803,156 @@ __global__ __launch_bounds__(256) void k_fused_B(StepArgs<T> a, int nbx, int nby
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
         const double kdx = a.ikdH[0][i];
-        Bx[c] = (T)((double)Bx[c] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx));
-        By[c] = (T)((double)By[c] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my));
-        Bz[c] = (T)((double)Bz[c] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz));
+        Bx[c] = (T)((double)pb[u][0] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx));
+        By[c] = (T)((double)pb[u][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my));
+        Bz[c] = (T)((double)pb[u][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz));
         ex = ex1;
         ey = ey1;
         ez = ez1;
     }
 }
 
+// Walk this block's share of a batch's tile list: the tiles are numbered run by run (z tile
+// fastest, then y, then x), the 8 XCDs take contiguous eighths of the list so that tiles
+// sharing halo cells meet in one L2, and each block strides its XCD's eighth by
+// gridDim.x / 8 (gridDim.x is a multiple of 8; the hardware deals blocks to the XCDs round
+// robin).  f(run, tz, ty, tx) processes one tile.
+template <typename F>
+__device__ __forceinline__ void batch_tiles(const BatchRun *__restrict__ runs, int nruns, int total, F &&f)
+{
+    const int per_xcd = gridDim.x >> 3;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int chunk = (total + 7) >> 3;
+    const int t0 = xcd * chunk, t1 = min(total, t0 + chunk);
+    int r = 0;
+    for (int t = t0 + slot; t < t1; t += per_xcd) {
+        while (r + 1 < nruns && t >= runs[r + 1].first) ++r;
+        const int ntz = runs[r].ntz, nty = runs[r].nty;
+        int local = t - runs[r].first;
+        const int tz = local % ntz;
+        local /= ntz;
+        f(r, tz, local % nty, local / nty);
+    }
+}
+
+// The argument block is read from device memory (the solver keeps a copy there): the face
+// tables are indexed at run time, which a by-value kernel argument would have to copy into
+// per-lane scratch.
+template <typename T, int NP, bool NONLIN, bool CPML, int L>
+__global__ __launch_bounds__(256) void k_fused_E(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
+{
+    int tbx, tby, tbz;
+    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    fused_E_tile<T, NP, NONLIN, CPML, L>(*pa, tbx, tby, tbz);
+}
+
+template <typename T, bool CPML, int L>
+__global__ __launch_bounds__(256) void k_fused_B(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
+{
+    int tbx, tby, tbz;
+    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    fused_B_tile<T, CPML, L>(*pa, tbx, tby, tbz);
+}
+
+// the fused kernels over the tiles of many independent runs, one persistent launch
+template <typename T, int NP, bool NONLIN, bool CPML, int L>
+__global__ __launch_bounds__(256) void k_fused_E_batch(const StepArgs<T> *__restrict__ args,
+                                                       const BatchRun *__restrict__ runs, int nruns, int total)
+{
+    batch_tiles(runs, nruns, total, [&](int r, int tz, int ty, int tx) {
+        fused_E_tile<T, NP, NONLIN, CPML, L, true>(args[r], tz, ty, tx);
+    });
+}
+
+template <typename T, bool CPML, int L>
+__global__ __launch_bounds__(256) void k_fused_B_batch(const StepArgs<T> *__restrict__ args,
+                                                       const BatchRun *__restrict__ runs, int nruns, int total)
+{
+    batch_tiles(runs, nruns, total,
+                [&](int r, int tz, int ty, int tx) { fused_B_tile<T, CPML, L, true>(args[r], tz, ty, tx); });
+}
+
+// tile grid of the batched fused kernels of one run: 256 consecutive y-z cells per tile
+// (tile_jk with LIN), L x planes per thread
+void fused_tiles(const GridDims &g, const int blo[3], bool hside, int L, int &ntz, int &nty, int &nseg)
+{
+    const int one[3] = {1, 1, 1};
+    const int *lo = hside ? blo : one;
+    long long yz = (long long)(g.n[1] - lo[1] + 1) * (g.n[2] - lo[2] + 1);
+    ntz = (int)((yz + 255) / 256);
+    nty = 1;
+    nseg = (g.n[0] - lo[0] + 1 + L - 1) / L;
+}
+
 template <typename T, int L>
-static void launch_fused_E_L(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+static void launch_fused_E_batch_L(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks,
+                                   int np, bool nonlin, bool cpml, hipStream_t s)
+{
+#define MPX_FEB(NPV, NL, CP)                                                                                    \
+    hipLaunchKernelGGL((k_fused_E_batch<T, NPV, NL, CP, L>), dim3(blocks), dim3(64, 4), 0, s, args, runs, nruns, \
+                       total)
+#define MPX_FEB_NP(NL, CP)             \
+    switch (np) {                      \
+    case 0: MPX_FEB(0, NL, CP); break; \
+    case 1: MPX_FEB(1, NL, CP); break; \
+    case 2: MPX_FEB(2, NL, CP); break; \
+    default: MPX_FEB(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cpml) { MPX_FEB_NP(true, true) } else { MPX_FEB_NP(true, false) }
+    } else {
+        if (cpml) { MPX_FEB_NP(false, true) } else { MPX_FEB_NP(false, false) }
+    }
+#undef MPX_FEB_NP
+#undef MPX_FEB
+}
+
+template <typename T>
+void launch_fused_E_batch(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks, int L,
+                          int np, bool nonlin, bool cpml, hipStream_t s)
+{
+    (void)L;  // the batch walks one plane per thread on the E side
+    launch_fused_E_batch_L<T, 1>(args, runs, nruns, total, blocks, np, nonlin, cpml, s);
+}
+
+template <typename T, int L>
+static void launch_fused_B_batch_L(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks,
+                                   bool cpml, hipStream_t s)
+{
+    if (cpml)
+        hipLaunchKernelGGL((k_fused_B_batch<T, true, L>), dim3(blocks), dim3(64, 4), 0, s, args, runs, nruns, total);
+    else
+        hipLaunchKernelGGL((k_fused_B_batch<T, false, L>), dim3(blocks), dim3(64, 4), 0, s, args, runs, nruns, total);
+}
+
+template <typename T>
+void launch_fused_B_batch(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks, int L,
+                          bool cpml, hipStream_t s)
+{
+    switch (L) {
+    case 2: launch_fused_B_batch_L<T, 2>(args, runs, nruns, total, blocks, cpml, s); break;
+    case 4: launch_fused_B_batch_L<T, 4>(args, runs, nruns, total, blocks, cpml, s); break;
+    default: launch_fused_B_batch_L<T, 1>(args, runs, nruns, total, blocks, cpml, s); break;
+    }
+}
+template void launch_fused_E_batch<double>(const StepArgs<double> *, const BatchRun *, int, int, int, int, int, bool,
+                                           bool, hipStream_t);
+template void launch_fused_E_batch<float>(const StepArgs<float> *, const BatchRun *, int, int, int, int, int, bool,
+                                          bool, hipStream_t);
+template void launch_fused_B_batch<double>(const StepArgs<double> *, const BatchRun *, int, int, int, int, bool,
+                                           hipStream_t);
+template void launch_fused_B_batch<float>(const StepArgs<float> *, const BatchRun *, int, int, int, int, bool,
+                                          hipStream_t);
+
+template <typename T, int L>
+static void launch_fused_E_L(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
     const GridDims &g = a.g;
     int nbx = (g.n[2] + 63) / 64, nby = (g.n[1] + 3) / 4, nbz = (g.n[0] + L - 1) / L;
     dim3 blk(64, 4);
     dim3 grd(nbx, nby, nbz);
     bool cp = a.nfaces > 0;
-#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP, L>), grd, blk, 0, s, a, nbx, nby, nbz)
+#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP, L>), grd, blk, 0, s, dev, nbx, nby, nbz)
 #define MPX_FE_NP(NL, CP)            \
     switch (np) {                    \
     case 0: MPX_FE(0, NL, CP); break; \
@@ -777,17 +970,17 @@ static void launch_fused_E_L(const StepArgs<T> &a, int np, bool nonlin, hipStrea
 }
 
 template <typename T>
-void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s)
+void launch_fused_E(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
     switch (a.march) {
-    case 2: launch_fused_E_L<T, 2>(a, np, nonlin, s); break;
-    case 4: launch_fused_E_L<T, 4>(a, np, nonlin, s); break;
-    default: launch_fused_E_L<T, 1>(a, np, nonlin, s); break;
+    case 2: launch_fused_E_L<T, 2>(a, dev, np, nonlin, s); break;
+    case 4: launch_fused_E_L<T, 4>(a, dev, np, nonlin, s); break;
+    default: launch_fused_E_L<T, 1>(a, dev, np, nonlin, s); break;
     }
 }
 
 template <typename T, int L>
-static void launch_fused_B_L(const StepArgs<T> &a, hipStream_t s)
+static void launch_fused_B_L(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s)
 {
     const GridDims &g = a.g;
     int nk = g.n[2] - a.blo[2] + 1, nj = g.n[1] - a.blo[1] + 1, ni = g.n[0] - a.blo[0] + 1;
@@ -795,25 +988,25 @@ static void launch_fused_B_L(const StepArgs<T> &a, hipStream_t s)
     dim3 blk(64, 4);
     dim3 grd(nbx, nby, nbz);
     if (a.nfaces > 0)
-        hipLaunchKernelGGL((k_fused_B<T, true, L>), grd, blk, 0, s, a, nbx, nby, nbz);
+        hipLaunchKernelGGL((k_fused_B<T, true, L>), grd, blk, 0, s, dev, nbx, nby, nbz);
     else
-        hipLaunchKernelGGL((k_fused_B<T, false, L>), grd, blk, 0, s, a, nbx, nby, nbz);
+        hipLaunchKernelGGL((k_fused_B<T, false, L>), grd, blk, 0, s, dev, nbx, nby, nbz);
 }
 
 template <typename T>
-void launch_fused_B(const StepArgs<T> &a, hipStream_t s)
+void launch_fused_B(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s)
 {
     switch (a.march) {
-    case 2: launch_fused_B_L<T, 2>(a, s); break;
-    case 4: launch_fused_B_L<T, 4>(a, s); break;
-    default: launch_fused_B_L<T, 1>(a, s); break;
+    case 2: launch_fused_B_L<T, 2>(a, dev, s); break;
+    case 4: launch_fused_B_L<T, 4>(a, dev, s); break;
+    default: launch_fused_B_L<T, 1>(a, dev, s); break;
     }
 }
 
-template void launch_fused_E<double>(const StepArgs<double> &, int, bool, hipStream_t);
-template void launch_fused_E<float>(const StepArgs<float> &, int, bool, hipStream_t);
-template void launch_fused_B<double>(const StepArgs<double> &, hipStream_t);
-template void launch_fused_B<float>(const StepArgs<float> &, hipStream_t);
+template void launch_fused_E<double>(const StepArgs<double> &, const StepArgs<double> *, int, bool, hipStream_t);
+template void launch_fused_E<float>(const StepArgs<float> &, const StepArgs<float> *, int, bool, hipStream_t);
+template void launch_fused_B<double>(const StepArgs<double> &, const StepArgs<double> *, hipStream_t);
+template void launch_fused_B<float>(const StepArgs<float> &, const StepArgs<float> *, hipStream_t);
 
 // ------------------------------------------------------------------------------------
 // fdtd-plain, fdtd-nl, fdtd-disp (periodic only; no currents)
@@ -1067,22 +1260,6 @@ __device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py,
 //
 // sweep_tile is one tile's walk; k_sweep maps the blocks of one grid onto tiles, k_sweep_batch
 // walks the tiles of many independent grids from one persistent launch.
-// accumulators of the active poles held in registers, indexable like the psi pointer table
-// (psi[n][component][cell]) so that the shared E update reads them in place
-template <int NP>
-struct PsiRegs {
-    double2 v[NP > 0 ? NP : 1][3];
-    struct Comp {
-        double2 x;
-        __device__ double2 operator[](size_t) const { return x; }
-    };
-    struct Pole {
-        const double2 *p;
-        __device__ Comp operator[](int q) const { return Comp{p[q]}; }
-    };
-    __device__ Pole operator[](int n) const { return Pole{v[n]}; }
-};
-
 template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
 __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty, int tx,
                                            T (&lds)[3][3][BY * RPT][64])
@@ -1334,30 +1511,16 @@ __global__ __launch_bounds__(64 * BY) void k_sweep(SweepArgs<T> a, int ntz, int 
     sweep_tile<T, NP, NONLIN, CPML, BY, RPT>(a, tz, ty, tx, lds);
 }
 
-// Persistent batched sweep: the tiles of all runs are numbered run by run (z tile fastest,
-// then y, then x segment); the 8 XCDs take contiguous eighths of that list so that tiles
-// sharing halo cells meet in one L2, and each block walks its XCD's eighth with stride
-// gridDim.x / 8 (gridDim.x is a multiple of 8; the hardware deals blocks to XCDs round robin).
+// Persistent batched sweep over the tiles of all runs (see batch_tiles).
 template <typename T, int NP, bool NONLIN, bool CPML, int BY, int RPT>
 __global__ __launch_bounds__(64 * BY) void k_sweep_batch(const SweepArgs<T> *__restrict__ args,
                                                          const BatchRun *__restrict__ runs, int nruns, int total)
 {
     __shared__ T lds[3][3][BY * RPT][64];
-    const int per_xcd = gridDim.x >> 3;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int chunk = (total + 7) >> 3;
-    const int t0 = xcd * chunk, t1 = min(total, t0 + chunk);
-    int r = 0;
-    for (int t = t0 + slot; t < t1; t += per_xcd) {
-        while (r + 1 < nruns && t >= runs[r + 1].first) ++r;
-        const int ntz = runs[r].ntz, nty = runs[r].nty;
-        int local = t - runs[r].first;
-        const int tz = local % ntz;
-        local /= ntz;
-        const int ty = local % nty, tx = local / nty;
+    batch_tiles(runs, nruns, total, [&](int r, int tz, int ty, int tx) {
         sweep_tile<T, NP, NONLIN, CPML, BY, RPT>(args[r], tz, ty, tx, lds);
         __syncthreads();  // the next tile rewrites the LDS planes
-    }
+    });
 }
 
 template <typename T, int NP, bool NONLIN, bool CPML>

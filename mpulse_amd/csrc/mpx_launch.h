@@ -47,16 +47,24 @@ void launch_wrap(const GridDims &g, T *const f[3], int dim, hipStream_t s);
 void launch_ref_stepD(const StepArgs<double> &a, int np, bool nonlin, hipStream_t s);
 void launch_ref_stepB(const StepArgs<double> &a, hipStream_t s);
 void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s);
+// `a` is the host copy (launch shape), `dev` the same block in device memory
 template <typename T>
-void launch_fused_E(const StepArgs<T> &a, int np, bool nonlin, hipStream_t s);
+void launch_fused_E(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool nonlin, hipStream_t s);
 template <typename T>
-void launch_fused_B(const StepArgs<T> &a, hipStream_t s);
+void launch_fused_B(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s);
 template <typename T>
 void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s);
 void sweep_tiles(const GridDims &g, int variant, int seg, int &ntz, int &nty, int &nseg);
 template <typename T>
 void launch_sweep_batch(const SweepArgs<T> *args, const BatchRun *runs, int nruns, int total, int variant,
                         int blocks, int np, bool nonlin, bool cpml, hipStream_t s);
+void fused_tiles(const GridDims &g, const int blo[3], bool hside, int L, int &ntz, int &nty, int &nseg);
+template <typename T>
+void launch_fused_E_batch(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks, int L,
+                          int np, bool nonlin, bool cpml, hipStream_t s);
+template <typename T>
+void launch_fused_B_batch(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks, int L,
+                          bool cpml, hipStream_t s);
 void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
                              const int *time, int tstep, hipStream_t s);
 void launch_advance_time(int *time, int k, hipStream_t s);

@@ -137,7 +137,7 @@ struct mpx_solver {
 
     double *d_part = nullptr, *d_energy = nullptr;
     int time = 0;
-    int march_e = 1, march_b = 4;  // x planes per thread in the fused kernels (1, 2 or 4)
+    int march_e = 1, march_b = 2;  // x planes per thread in the fused kernels (1, 2 or 4)
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
     double2 *psi2[3][3] = {};
@@ -146,6 +146,10 @@ struct mpx_solver {
     int sweep_seg = 32, sweep_variant = 2;
     Timer timer;
     hipStream_t owner = nullptr;  // stream of the batch that steps this run (nullptr: none)
+    // device copies of the fused kernels' argument blocks (E side, B side), refreshed when the
+    // host-side block changes
+    void *d_fargs[2] = {nullptr, nullptr};
+    std::vector<unsigned char> h_fargs[2];
 
     ~mpx_solver()
     {
@@ -557,6 +561,7 @@ static int build(mpx_solver *s, const mpx_desc *desc)
     s->plrc = (s->solver == MPX_SOLVER_PLRC_NONLIN || s->solver == MPX_SOLVER_PLRC_LIN);
     s->nonlin = (s->solver == MPX_SOLVER_PLRC_NONLIN);
     s->esz = (s->prec == MPX_PREC_FP32_BULK) ? 4 : 8;
+    if (s->esz == 8) s->march_e = 4;  // float64 E side: four x planes per thread measured fastest
     s->delta[0] = desc->dx;
     s->delta[1] = desc->dy;
     s->delta[2] = desc->dz;
@@ -1158,6 +1163,27 @@ static void swap_sweep_buffers(mpx_solver *s)
     }
 }
 
+// device copy of a fused kernel's argument block; uploaded again only when it changed
+template <typename T>
+static const StepArgs<T> *dev_args(mpx_solver *s, const StepArgs<T> &a, int side)
+{
+    const unsigned char *p = (const unsigned char *)&a;
+    std::vector<unsigned char> &h = s->h_fargs[side];
+    if (h.size() == sizeof(a) && std::memcmp(h.data(), p, sizeof(a)) == 0) return (const StepArgs<T> *)s->d_fargs[side];
+    if (!s->d_fargs[side]) {
+        if (hipMalloc(&s->d_fargs[side], sizeof(StepArgs<double>)) != hipSuccess) {
+            s->d_fargs[side] = nullptr;
+            return nullptr;
+        }
+        s->allocs.push_back(s->d_fargs[side]);
+    }
+    h.assign(p, p + sizeof(a));
+    if (hipMemcpyAsync(s->d_fargs[side], h.data(), sizeof(a), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return nullptr;
+    return (const StepArgs<T> *)s->d_fargs[side];
+}
+
 // one full step: FDTD_PLRCSolver::stepScheme (src/fdtd_plrc.t:127-168) or the plain/nl/disp
 // stepScheme (src/fdtd_plain.cpp:31-35)
 static int step_once(mpx_solver *s)
@@ -1251,24 +1277,32 @@ static int step_once(mpx_solver *s)
         if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
         if (s->esz == 8) {
             StepArgs<double> ae = step_args<double>(s, false);
+            const StepArgs<double> *de = dev_args(s, ae, 0);
+            if (!de) return fail(MPX_ERR_HIP, "argument upload failed");
             s->tick(K_FUSED_E);
-            mpx::launch_fused_E<double>(ae, s->np, s->nonlin, s->stream);
+            mpx::launch_fused_E<double>(ae, de, s->np, s->nonlin, s->stream);
             s->tock();
             MPX_LAUNCHED("fused E");
             StepArgs<double> ah = step_args<double>(s, true);
+            const StepArgs<double> *dh = dev_args(s, ah, 1);
+            if (!dh) return fail(MPX_ERR_HIP, "argument upload failed");
             s->tick(K_FUSED_B);
-            mpx::launch_fused_B<double>(ah, s->stream);
+            mpx::launch_fused_B<double>(ah, dh, s->stream);
             s->tock();
             MPX_LAUNCHED("fused B");
         } else {
             StepArgs<float> ae = step_args<float>(s, false);
+            const StepArgs<float> *de = dev_args(s, ae, 0);
+            if (!de) return fail(MPX_ERR_HIP, "argument upload failed");
             s->tick(K_FUSED_E);
-            mpx::launch_fused_E<float>(ae, s->np, s->nonlin, s->stream);
+            mpx::launch_fused_E<float>(ae, de, s->np, s->nonlin, s->stream);
             s->tock();
             MPX_LAUNCHED("fused E");
             StepArgs<float> ah = step_args<float>(s, true);
+            const StepArgs<float> *dh = dev_args(s, ah, 1);
+            if (!dh) return fail(MPX_ERR_HIP, "argument upload failed");
             s->tick(K_FUSED_B);
-            mpx::launch_fused_B<float>(ah, s->stream);
+            mpx::launch_fused_B<float>(ah, dh, s->stream);
             s->tock();
             MPX_LAUNCHED("fused B");
         }
@@ -1718,11 +1752,13 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
 // batched independent runs: one persistent sweep launch per step for all runs
 // ---------------------------------------------------------------------------------------
 //
-// The runs are ordinary solvers built by `build` (sweep path), stepped in lockstep from one
-// stream.  Per step two kernels run: the source sheets of every run, then the sweep over the
-// tiles of every run.  The step counter lives on the device, so a captured graph of K steps
-// can be replayed at any later step; the runs' buffer sets alternate every step, and the two
-// argument tables (one per parity) hold both arrangements.
+// The runs are ordinary solvers built by `build` (sweep path, so that both kernel structures
+// have their buffers), stepped in lockstep from one stream.  Per step the source sheets of
+// every run are evaluated in one launch, then either the fused E and fused B kernels run over
+// the tiles of every run (structure 0) or the single-pass sweep does (structure 1).  The step
+// counter lives on the device, so a captured graph of K steps can be replayed at any later
+// step.  Under the sweep the runs' buffer sets alternate every step, and two argument tables
+// (one per parity) hold both arrangements.
 
 struct mpx_batch {
     std::vector<mpx_solver *> runs;
@@ -1730,13 +1766,18 @@ struct mpx_batch {
     size_t esz = 8;
     int np = 0;
     bool nonlin = false, cpml = false;
-    int variant = 2, seg = 32, blocks_per_cu = 2, graph_steps = 50;
+    int structure = 0;               // 0: fused E + fused B kernels, 1: single-pass sweep
+    int variant = 2, seg = 32, blocks_per_cu = 2, graph_steps = 50;  // sweep tiling
+    int march_b = 4, fused_bpc = 8;  // fused B planes per thread, fused blocks per CU
     int cus = 256;
     int time = 0;
     int parity = 0;  // steps taken since creation, mod 2
     void *d_args[2] = {nullptr, nullptr};
     mpx::BatchRun *d_runs = nullptr;
     int total_tiles = 0, blocks = 0;
+    void *d_fargs[2] = {nullptr, nullptr};           // fused E / B argument tables
+    mpx::BatchRun *d_fruns[2] = {nullptr, nullptr};  // fused E / B tile tables
+    int ftotal[2] = {0, 0}, fblocks[2] = {0, 0};
     mpx::SheetEval *d_sheets = nullptr;
     int *d_sheet_first = nullptr;
     int nsheets = 0, sheet_blocks = 0;
@@ -1808,10 +1849,37 @@ static int batch_prepare_t(mpx_batch *b)
     b->total_tiles = total;
     // persistent grid: blocks_per_cu blocks on every CU, a multiple of the 8 XCDs, no more
     // than the tiles need
-    int want = b->blocks_per_cu * b->cus;
-    int need = (total + 7) / 8 * 8;
-    b->blocks = (want < need ? want : need);
-    b->blocks = (b->blocks + 7) / 8 * 8;
+    auto grid = [&](int bpc, int tiles) {
+        int want = bpc * b->cus, need = (tiles + 7) / 8 * 8;
+        return ((want < need ? want : need) + 7) / 8 * 8;
+    };
+    b->blocks = grid(b->blocks_per_cu, total);
+    // fused structure: E side (one plane per thread) and B side (march_b planes per thread)
+    for (int side = 0; side < 2; ++side) {
+        std::vector<StepArgs<T>> fa;
+        std::vector<mpx::BatchRun> fr(n);
+        int ft = 0;
+        for (int r = 0; r < n; ++r) {
+            mpx_solver *s = b->runs[r];
+            StepArgs<T> sa = step_args<T>(s, side == 1);
+            sa.march = side ? b->march_b : 1;
+            fa.push_back(sa);
+            fr[r].first = ft;
+            mpx::fused_tiles(s->g, sa.blo, side == 1, sa.march, fr[r].ntz, fr[r].nty, fr[r].nseg);
+            long long t = (long long)fr[r].ntz * fr[r].nty * fr[r].nseg;
+            if (ft + t > (1LL << 30)) return fail(MPX_ERR_UNSUPPORTED, "batch has too many tiles");
+            ft += (int)t;
+        }
+        size_t fb = n * sizeof(StepArgs<T>);
+        if (!b->d_fargs[side] && b->alloc(&b->d_fargs[side], fb)) return MPX_ERR_NOMEM;
+        if (!b->d_fruns[side] && b->alloc((void **)&b->d_fruns[side], n * sizeof(mpx::BatchRun))) return MPX_ERR_NOMEM;
+        MPX_HIP(hipMemcpyAsync(b->d_fargs[side], fa.data(), fb, hipMemcpyHostToDevice, b->stream));
+        MPX_HIP(hipMemcpyAsync(b->d_fruns[side], fr.data(), n * sizeof(mpx::BatchRun), hipMemcpyHostToDevice,
+                               b->stream));
+        MPX_HIP(hipStreamSynchronize(b->stream));
+        b->ftotal[side] = ft;
+        b->fblocks[side] = grid(b->fused_bpc, ft);
+    }
     return MPX_OK;
 }
 
@@ -1827,6 +1895,20 @@ static void batch_enqueue(mpx_batch *b, int j, int par)
         mpx::launch_sheet_eval_batch(b->d_sheets, b->d_sheet_first, b->nsheets, b->sheet_blocks, b->ft, b->d_time, j,
                                      b->stream);
     const int n = (int)b->runs.size();
+    if (b->structure == 0) {
+        if (b->esz == 8) {
+            mpx::launch_fused_E_batch<double>((const StepArgs<double> *)b->d_fargs[0], b->d_fruns[0], n, b->ftotal[0],
+                                              b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
+            mpx::launch_fused_B_batch<double>((const StepArgs<double> *)b->d_fargs[1], b->d_fruns[1], n, b->ftotal[1],
+                                              b->fblocks[1], b->march_b, b->cpml, b->stream);
+        } else {
+            mpx::launch_fused_E_batch<float>((const StepArgs<float> *)b->d_fargs[0], b->d_fruns[0], n, b->ftotal[0],
+                                             b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
+            mpx::launch_fused_B_batch<float>((const StepArgs<float> *)b->d_fargs[1], b->d_fruns[1], n, b->ftotal[1],
+                                             b->fblocks[1], b->march_b, b->cpml, b->stream);
+        }
+        return;
+    }
     if (b->esz == 8)
         mpx::launch_sweep_batch<double>((const mpx::SweepArgs<double> *)b->d_args[par], b->d_runs, n,
                                         b->total_tiles, b->variant, b->blocks, b->np, b->nonlin, b->cpml, b->stream);
@@ -1839,7 +1921,7 @@ static int batch_capture(mpx_batch *b, int par)
 {
     hipGraph_t gr = nullptr;
     MPX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < b->graph_steps; ++j) batch_enqueue(b, j, (par + j) & 1);
+    for (int j = 0; j < b->graph_steps; ++j) batch_enqueue(b, j, b->structure == 1 ? (par + j) & 1 : par);
     mpx::launch_advance_time(b->d_time, b->graph_steps, b->stream);
     hipError_t e = hipStreamEndCapture(b->stream, &gr);
     if (e != hipSuccess) return fail(MPX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -1864,18 +1946,18 @@ static int batch_step(mpx_batch *b, int nsteps, bool use_graph)
                 if (rc) return rc;
             }
             MPX_HIP(hipGraphLaunch(b->graph[b->parity], b->stream));
-            b->parity = (b->parity + K) & 1;
+            if (b->structure == 1) b->parity = (b->parity + K) & 1;
             done += K;
         }
     for (; done < nsteps; ++done) {
         batch_enqueue(b, 0, b->parity);
         mpx::launch_advance_time(b->d_time, 1, b->stream);
         MPX_LAUNCHED("batch step");
-        b->parity ^= 1;
+        if (b->structure == 1) b->parity ^= 1;
     }
     for (mpx_solver *s : b->runs) {
         s->time += nsteps;
-        if (nsteps & 1) swap_sweep_buffers(s);
+        if (b->structure == 1 && (nsteps & 1)) swap_sweep_buffers(s);
     }
     b->time += nsteps;
     return MPX_OK;
@@ -2030,7 +2112,16 @@ int mpx_batch_get_time(mpx_batch *b, int *time)
 int mpx_batch_set_tuning(mpx_batch *b, const char *key, int value)
 {
     if (!b || !key) return fail(MPX_ERR_INVALID, "null argument");
-    if (!std::strcmp(key, "variant")) {
+    if (!std::strcmp(key, "structure")) {
+        if (value < 0 || value > 1) return fail(MPX_ERR_INVALID, "structure must be 0 (fused) or 1 (sweep)");
+        b->structure = value;
+    } else if (!std::strcmp(key, "march_b")) {
+        if (value != 1 && value != 2 && value != 4) return fail(MPX_ERR_INVALID, "march_b must be 1, 2 or 4");
+        b->march_b = value;
+    } else if (!std::strcmp(key, "fused_blocks_per_cu")) {
+        if (value < 1 || value > 32) return fail(MPX_ERR_INVALID, "fused_blocks_per_cu must be 1..32");
+        b->fused_bpc = value;
+    } else if (!std::strcmp(key, "variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "variant must be 0, 1 or 2");
         b->variant = value;
     } else if (!std::strcmp(key, "seg")) {
@@ -2067,44 +2158,61 @@ int mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step)
     return rc;
 }
 
-// Autotuner: times every (block shape, x segment, blocks per CU) candidate for
-// `steps_per_trial` steps on the batch itself, keeps the fastest, then clears every run back
-// to its initial all-zero state at time 0.  Allowed only before the first step.
-int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[3], double *best_ms)
+// Autotuner: times every candidate launch shape -- the fused structure with 1 or 4 B planes
+// per thread and 4 or 8 blocks per CU, and the sweep with block shapes 64x16 / 64x8, x
+// segments of 16 / 32 planes and 1 or 2 blocks per CU -- for `steps_per_trial` steps on the
+// batch itself, keeps the fastest, then clears every run back to its initial all-zero state
+// at time 0.  Allowed only before the first step.  result = {structure, a, b, c}: fused
+// {0, march_b, blocks_per_cu, 0}, sweep {1, variant, seg, blocks_per_cu}.
+int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double *best_ms)
 {
     if (!b || steps_per_trial < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
     if (b->time != 0) return fail(MPX_ERR_INVALID, "autotune runs before the first step only");
-    const int variants[3] = {2, 1, 0}, segs[4] = {16, 32, 64, 128}, bpc[3] = {1, 2, 4};
+    struct Cand {
+        int structure, p0, p1, p2;
+    };
+    std::vector<Cand> cands;
+    for (int mb : {1, 4})
+        for (int bpc : {4, 8}) cands.push_back({0, mb, bpc, 0});
+    for (int v : {2, 0})
+        for (int sg : {16, 32})
+            for (int bpc : {1, 2}) cands.push_back({1, v, sg, bpc});
+    auto apply = [&](const Cand &c) {
+        b->structure = c.structure;
+        if (c.structure == 0) {
+            b->march_b = c.p0;
+            b->fused_bpc = c.p1;
+        } else {
+            b->variant = c.p0;
+            b->seg = c.p1;
+            b->blocks_per_cu = c.p2;
+        }
+    };
     double best = 1e300;
-    int bv = b->variant, bs = b->seg, bb = b->blocks_per_cu;
+    Cand bc = {b->structure, b->structure ? b->variant : b->march_b, b->structure ? b->seg : b->fused_bpc,
+               b->structure ? b->blocks_per_cu : 0};
     hipEvent_t e0, e1;
     MPX_HIP(hipEventCreate(&e0));
     MPX_HIP(hipEventCreate(&e1));
     int rc = MPX_OK;
-    for (int v : variants)
-        for (int sg : segs)
-            for (int k : bpc) {
-                b->variant = v;
-                b->seg = sg;
-                b->blocks_per_cu = k;
-                if ((rc = batch_prepare(b))) break;
-                if ((rc = batch_step(b, 1, false))) break;  // warm-up
-                (void)hipEventRecord(e0, b->stream);
-                if ((rc = batch_step(b, steps_per_trial, false))) break;
-                (void)hipEventRecord(e1, b->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) {
-                    rc = fail(MPX_ERR_HIP, "event sync failed");
-                    break;
-                }
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best) {
-                    best = ms;
-                    bv = v;
-                    bs = sg;
-                    bb = k;
-                }
-            }
+    for (const Cand &c : cands) {
+        apply(c);
+        if ((rc = batch_prepare(b))) break;
+        if ((rc = batch_step(b, 1, false))) break;  // warm-up
+        (void)hipEventRecord(e0, b->stream);
+        if ((rc = batch_step(b, steps_per_trial, false))) break;
+        (void)hipEventRecord(e1, b->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(MPX_ERR_HIP, "event sync failed");
+            break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+            best = ms;
+            bc = c;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
@@ -2113,14 +2221,13 @@ int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[3], double 
     MPX_HIP(hipMemsetAsync(b->d_time, 0, sizeof(int), b->stream));
     MPX_HIP(hipStreamSynchronize(b->stream));
     b->time = 0;
-    b->variant = bv;
-    b->seg = bs;
-    b->blocks_per_cu = bb;
+    apply(bc);
     if ((rc = batch_prepare(b))) return rc;
     if (result) {
-        result[0] = bv;
-        result[1] = bs;
-        result[2] = bb;
+        result[0] = bc.structure;
+        result[1] = bc.p0;
+        result[2] = bc.p1;
+        result[3] = bc.p2;
     }
     if (best_ms) *best_ms = best / steps_per_trial;
     return MPX_OK;

@@ -25,25 +25,34 @@ def small_runs(precision):
     return out
 
 
-def singles(descs, steps):
+def singles(descs, steps, path=abi.PATH_SWEEP):
     res = []
     for d in descs:
+        d.path = path
         with abi.Solver(d) as s:
             s.step(steps)
             res.append({n: s.get(n) for n in FIELDS})
+        d.path = abi.PATH_SWEEP
     return res
 
 
+# (structure, tuning, graph steps): the fused structure is compared with the fused path run
+# on its own, the sweep structure with the sweep path
+CASES = [(0, {"march_b": 4, "fused_blocks_per_cu": 8}, 7), (0, {"march_b": 1, "fused_blocks_per_cu": 1}, 0),
+         (1, {"variant": 2, "seg": 32}, 7), (1, {"variant": 0, "seg": 5}, 4), (1, {"variant": 1, "seg": 16}, 0)]
+
+
 @pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK])
-@pytest.mark.parametrize("variant,seg,graph", [(2, 32, 7), (0, 5, 4), (1, 16, 0)])
-def test_batch_matches_single_runs(prec, variant, seg, graph):
+@pytest.mark.parametrize("structure,tuning,graph", CASES)
+def test_batch_matches_single_runs(prec, structure, tuning, graph):
     descs = small_runs(prec)
     steps = 45
-    print(f"case: batch prec {prec} variant {variant} seg {seg} graph {graph}", flush=True)
-    want = singles(descs, steps)
+    print(f"case: batch prec {prec} structure {structure} {tuning} graph {graph}", flush=True)
+    want = singles(descs, steps, abi.PATH_FUSED if structure == 0 else abi.PATH_SWEEP)
     with abi.Batch(descs) as b:
-        b.tune("variant", variant)
-        b.tune("seg", seg)
+        b.tune("structure", structure)
+        for k, v in tuning.items():
+            b.tune(k, v)
         b.tune("graph_steps", graph)
         b.step(20)
         b.step(steps - 20)
@@ -59,26 +68,28 @@ def test_batch_matches_single_runs(prec, variant, seg, graph):
 def test_batch_run_against_oracle():
     descs = small_runs(abi.PREC_FP64)
     steps = 40
-    with abi.Batch(descs) as b:
-        b.tune("graph_steps", 10)
-        b.step(steps)
-        got = {n: b.run(2).get(n) for n in FIELDS[:6]}
-    o = Oracle(descs[2])
-    o.step(steps)
-    for n in FIELDS[:6]:
-        r = o.get(n)
-        err = np.linalg.norm(got[n] - r) / np.linalg.norm(r)
-        print(n, err)
-        assert err < 1e-12, (n, err)
+    for structure in (0, 1):
+        with abi.Batch(descs) as b:
+            b.tune("structure", structure)
+            b.tune("graph_steps", 10)
+            b.step(steps)
+            got = {n: b.run(2).get(n) for n in FIELDS[:6]}
+        o = Oracle(descs[2])
+        o.step(steps)
+        for n in FIELDS[:6]:
+            r = o.get(n)
+            err = np.linalg.norm(got[n] - r) / np.linalg.norm(r)
+            print(structure, n, err)
+            assert err < 1e-12, (structure, n, err)
 
 
 def test_batch_autotune_restores_initial_state():
     descs = small_runs(abi.PREC_FP64)
-    want = singles(descs, 30)
     with abi.Batch(descs) as b:
         best = b.autotune(steps_per_trial=2)
         print("autotune", best)
-        assert best["variant"] in (0, 1, 2) and best["seg"] in (16, 32, 64, 128)
+        assert best["structure"] in ("fused", "sweep")
+        want = singles(descs, 30, abi.PATH_FUSED if best["structure"] == "fused" else abi.PATH_SWEEP)
         assert b.time == 0
         assert not np.any(b.run(0).get("Ex"))
         b.step(30)
