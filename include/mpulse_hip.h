@@ -159,7 +159,9 @@ int  mpx_get_time(mpx_solver *s, int *time);
 int  mpx_set_time(mpx_solver *s, int time);
 
 /* Full-storage grids by reference name: "Ex".."Bz", "PsiR1x".."PsiI3z" (PLRC),
-   "P1x".."P3zp" (fdtd-disp), "JxTotal".."MzTotal" (reference path only).
+   "P1x".."P3zp" (fdtd-disp), "JxTotal".."MzTotal" (reference path only), and "Eps": a
+   per-cell permittivity of the PLRC solvers (no reference counterpart; reads give the uniform
+   eps until one is set; setting it needs the fused or reference-structured path).
    Buffer: (Nx+2)*(Ny+2)*(Nz+2) float64, z fastest. */
 int  mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count);
 int  mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count);
@@ -226,6 +228,23 @@ int  mpx_batch_set_tuning(mpx_batch *b, const char *key, int value);
 int  mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double *best_ms_per_step);
 /* Advance nsteps and report the mean device time per step (HIP events on the batch stream). */
 int  mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step);
+
+/* ---- gradient of a field objective by the discrete adjoint ---------------------------
+   No counterpart in the reference.  Objective: J = weight * sum over the nsteps steps of
+   sum_{i,j} |E(i, j, plane_k)|^2 after each step.  The solver is stepped nsteps from its
+   current state; the adjoint of every step is then run backwards through the same discrete
+   update (fused float64 kernels), recomputing the forward E of each segment of
+   checkpoint_every steps from a stored checkpoint.  Gradients: dJ/dEps(x) for every interior
+   cell (full-storage layout, zero in the ghosts) and dJ/dchi.  The solver ends at the final
+   forward state.  Scope: FDTD_PLRC solvers without active Lorentz poles, float64, fused
+   path, periodic boundary (CPML and current sheets allowed).  With dJ_deps and dJ_dchi both
+   NULL only the forward objective is computed. */
+typedef struct mpx_objective {
+    int    plane_k;   /* z index of the observation plane, 1..nz */
+    double weight;
+} mpx_objective;
+int  mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int checkpoint_every, double *J,
+                      double *dJ_deps, double *dJ_dchi);
 
 const char *mpx_last_error(void);
 int  mpx_abi_version(void);

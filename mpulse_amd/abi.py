@@ -102,6 +102,10 @@ def add_factory(desc, fac):
     return desc
 
 
+class Objective(C.Structure):
+    _fields_ = [("plane_k", C.c_int), ("weight", C.c_double)]
+
+
 class MpxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"mpx error {code}: {msg}")
@@ -150,6 +154,8 @@ def load():
         "mpx_batch_set_tuning": (C.c_int, [P, C.c_char_p, C.c_int]),
         "mpx_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "mpx_batch_time_steps": (C.c_int, [P, C.c_int, C.POINTER(C.c_double)]),
+        "mpx_adjoint_grad": (C.c_int, [P, C.POINTER(Objective), C.c_int, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mpx_last_error": (C.c_char_p, []),
         "mpx_abi_version": (C.c_int, []),
         "mpx_desc_defaults": (None, [C.POINTER(Desc)]),
@@ -280,6 +286,22 @@ class Solver:
         b = C.c_size_t()
         self._check(self.lib.mpx_state_bytes(self.h, C.byref(b)))
         return b.value
+
+    def adjoint_grad(self, plane_k, weight, nsteps, checkpoint_every=50, grad=True):
+        """Step nsteps forward, accumulating J = weight * sum_steps sum_ij |E(i, j, plane_k)|^2,
+        and (grad) return (J, dJ/dEps full-storage array, dJ/dchi) from the discrete adjoint;
+        without grad return (J, None, None)."""
+        obj = Objective(int(plane_k), float(weight))
+        J = C.c_double()
+        if not grad:
+            self._check(self.lib.mpx_adjoint_grad(self.h, C.byref(obj), nsteps, checkpoint_every, C.byref(J),
+                                                  None, None))
+            return J.value, None, None
+        deps = np.empty(self.shape, dtype=np.float64)
+        dchi = C.c_double()
+        self._check(self.lib.mpx_adjoint_grad(self.h, C.byref(obj), nsteps, checkpoint_every, C.byref(J),
+                                              _dptr(deps), C.byref(dchi)))
+        return J.value, deps, dchi.value
 
 
 class Batch:

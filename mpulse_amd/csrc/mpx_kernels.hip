@@ -375,6 +375,21 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
     }
 }
 
+// Denominator and numerator of the E update at cell c: the uniform ones, or those of a
+// per-cell permittivity formed with the pole sums in the order the uniform ones are formed.
+template <bool NONLIN, typename A>
+__device__ __forceinline__ void cell_eps(const A &a, size_t c, double &denom, double &numer)
+{
+    if (a.epsf) {
+        const double e = a.epsf[c];
+        denom = NONLIN ? e + a.sumChi0 - a.sumXi0 : e + (a.sumChi0 - a.sumXi0);
+        numer = e - a.sumXi0;
+    } else {
+        denom = a.denom;
+        numer = a.numer;
+    }
+}
+
 template <int NP, bool NONLIN, bool RECIP>
 __device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
                                               double dBx_z, double dBz_x, double dBy_x, double dBx_y,
@@ -411,8 +426,10 @@ __global__ __launch_bounds__(256) void k_ref_stepD(StepArgs<double> a)
         jz = a.Jt[2][c];
     }
     double ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
+    double denom, numer;
+    cell_eps<NONLIN>(a, c, denom, numer);
     plrc_e_update<NP, NONLIN, false>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
-                                     a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, a.denom, a.numer, a.chi);
+                                     a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, denom, numer, a.chi);
     a.E[0][c] = ex;
     a.E[1][c] = ey;
     a.E[2][c] = ez;
@@ -704,8 +721,14 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
                 }
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+        double idenom = a.idenom, numer = a.numer;
+        if (a.epsf) {
+            double denom;
+            cell_eps<NONLIN>(a, (size_t)c, denom, numer);
+            idenom = 1.0 / denom;
+        }
         plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy,
-                                           kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, a.idenom, a.numer,
+                                           kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom, numer,
                                            a.chi);
         a.E[0][c] = (T)ex;
         a.E[1][c] = (T)ey;

@@ -68,6 +68,9 @@ void launch_fused_B_batch(const StepArgs<T> *args, const BatchRun *runs, int nru
 void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
                              const int *time, int tstep, hipStream_t s);
 void launch_advance_time(int *time, int k, hipStream_t s);
+void launch_adjoint_step(const AdjArgs &a, hipStream_t s);
+void launch_plane_objective(const GridDims &g, const double *const E[3], int kobj, double w, double *acc,
+                            hipStream_t s);
 void launch_simple_stepD(const SimpleArgs &a, hipStream_t s);
 void launch_simple_stepB(const SimpleArgs &a, hipStream_t s);
 template <typename T>

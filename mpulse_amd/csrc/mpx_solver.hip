@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <array>
 #include <vector>
 
 #include "../../include/mpulse_hip.h"
@@ -116,6 +118,7 @@ struct mpx_solver {
     std::vector<double> kE[3], kH[3];
     double *d_kdE[3] = {}, *d_kdH[3] = {};
     double *d_ikdE[3] = {}, *d_ikdH[3] = {};  // reciprocals, for the fused and sweep kernels
+    double *d_eps = nullptr;                    // per-cell permittivity, once set (full storage)
 
     std::vector<Layer> layers;
     std::vector<Current> ecur, hcur;
@@ -965,6 +968,9 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.denom = s->denom;
     a.idenom = 1.0 / s->denom;
     a.numer = s->numer;
+    a.epsf = s->d_eps;
+    a.sumChi0 = s->sumChi0;
+    a.sumXi0 = s->sumXi0;
     a.pc = s->pc;
     const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
     a.nfaces = (int)faces.size();
@@ -1466,6 +1472,11 @@ static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, 
         *elem = 8;
         return MPX_OK;  // always zero: no OptField on this path
     }
+    if (!std::strcmp(name, "Eps") && s->plrc) {
+        *elem = 8;
+        *ptr = s->d_eps;  // nullptr: the uniform eps of the descriptor
+        return MPX_OK;
+    }
     return MPX_ERR_NAME;
 }
 
@@ -1485,7 +1496,10 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
     int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
     if (rc) return fail(rc, std::string("unknown grid ") + name);
     if (!ptr) {
-        std::memset(host, 0, count * sizeof(double));
+        if (!std::strcmp(name, "Eps"))
+            for (size_t q = 0; q < count; ++q) host[q] = s->d.eps;
+        else
+            std::memset(host, 0, count * sizeof(double));
         return MPX_OK;
     }
     if ((rc = materialise_ghosts(s))) return rc;
@@ -1515,6 +1529,14 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
     const GridDims &g = s->g;
     size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
+    if (!std::strcmp(name, "Eps") && s->plrc && !s->d_eps) {
+        // a per-cell permittivity: the fused and reference-structured kernels read it
+        if (s->path == MPX_PATH_SWEEP) return fail(MPX_ERR_UNSUPPORTED, "a per-cell Eps needs the fused or reference path");
+        if (s->owner) return fail(MPX_ERR_INVALID, "run belongs to a batch: use mpx_batch_*");
+        int rc = 0;
+        s->d_eps = s->dalloc<double>(s->cells, rc);
+        if (rc) return rc;
+    }
     void *ptr;
     size_t elem;
     bool isPsi;
@@ -2230,6 +2252,211 @@ int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double 
         result[3] = bc.p2;
     }
     if (best_ms) *best_ms = best / steps_per_trial;
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// gradient of a plane objective by the discrete adjoint (kernels in mpx_adjoint.hip)
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+size_t face_count(const CpmlFace &F) { return (size_t)(F.hi[0] - F.lo[0] + 1) * F.n1 * F.n2; }
+
+// device buffers owned by one adjoint call
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs()
+    {
+        for (void *q : p) (void)hipFree(q);
+    }
+    double *get(size_t count, hipStream_t st)
+    {
+        void *q = nullptr;
+        if (hipMalloc(&q, (count ? count : 1) * sizeof(double)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        if (hipMemsetAsync(q, 0, (count ? count : 1) * sizeof(double), st) != hipSuccess) return nullptr;
+        return (double *)q;
+    }
+};
+
+// the forward state of the fused float64 path without poles: E, B and the CPML auxiliaries
+struct StateRef {
+    std::vector<std::pair<void *, size_t>> arr;  // (device pointer, doubles)
+};
+
+StateRef state_of(mpx_solver *s)
+{
+    StateRef r;
+    for (int c = 0; c < 3; ++c) {
+        r.arr.push_back({s->E[c], s->cells});
+        r.arr.push_back({s->B[c], s->cells});
+    }
+    for (const CpmlFace &F : s->efaces) {
+        r.arr.push_back({F.psi0, face_count(F)});
+        r.arr.push_back({F.psi1, face_count(F)});
+    }
+    for (const CpmlFace &F : s->hfaces) {
+        r.arr.push_back({F.psi0, face_count(F)});
+        r.arr.push_back({F.psi1, face_count(F)});
+    }
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K, double *J, double *dJ_deps,
+                     double *dJ_dchi)
+{
+    if (!s || !obj || !J || nsteps < 1 || K < 1) return fail(MPX_ERR_INVALID, "invalid adjoint arguments");
+    MPX_NOT_IN_BATCH(s);
+    if (!s->plrc || s->np != 0 || s->esz != 8 || s->path != MPX_PATH_FUSED)
+        return fail(MPX_ERR_UNSUPPORTED,
+                    "the adjoint covers the float64 fused FDTD_PLRC update without active Lorentz poles");
+    if (!s->periodic[0] || !s->periodic[1] || !s->periodic[2])
+        return fail(MPX_ERR_UNSUPPORTED, "the adjoint needs the periodic boundary");
+    if (obj->plane_k < 1 || obj->plane_k > s->g.n[2]) return fail(MPX_ERR_INVALID, "objective plane out of range");
+    const bool grad = dJ_deps || dJ_dchi;
+    hipStream_t st = s->stream;
+    DevBufs mem;
+    double *dJ = mem.get(1, st);
+    if (!dJ) return fail(MPX_ERR_NOMEM, "adjoint allocation failed");
+    const int t0 = s->time;
+    const int nseg = (nsteps + K - 1) / K;
+    StateRef cur = state_of(s);
+    auto snapshot = [&](std::vector<double *> &dst) -> int {
+        if (dst.empty())
+            for (auto &a : cur.arr) {
+                double *q = mem.get(a.second, st);
+                if (!q) return fail(MPX_ERR_NOMEM, "checkpoint allocation failed");
+                dst.push_back(q);
+            }
+        for (size_t q = 0; q < cur.arr.size(); ++q)
+            MPX_HIP(hipMemcpyAsync(dst[q], cur.arr[q].first, cur.arr[q].second * 8, hipMemcpyDeviceToDevice, st));
+        return MPX_OK;
+    };
+    auto restore = [&](const std::vector<double *> &src) -> int {
+        for (size_t q = 0; q < cur.arr.size(); ++q)
+            MPX_HIP(hipMemcpyAsync(cur.arr[q].first, src[q], cur.arr[q].second * 8, hipMemcpyDeviceToDevice, st));
+        return MPX_OK;
+    };
+    std::vector<std::vector<double *>> ckpt(grad ? nseg : 0);
+    int rc;
+    // forward pass: objective after every step, a checkpoint at the start of every segment
+    for (int n = 0; n < nsteps; ++n) {
+        if (grad && n % K == 0 && (rc = snapshot(ckpt[n / K]))) return rc;
+        if ((rc = step_once(s))) return rc;
+        double *E[3] = {(double *)s->E[0], (double *)s->E[1], (double *)s->E[2]};
+        mpx::launch_plane_objective(s->g, E, obj->plane_k, obj->weight, dJ, st);
+        MPX_LAUNCHED("plane objective");
+    }
+    MPX_HIP(hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, st));
+    MPX_HIP(hipStreamSynchronize(st));
+    if (!grad) return MPX_OK;
+    std::vector<double *> fin;
+    if ((rc = snapshot(fin))) return rc;
+
+    // adjoint state, zero after the last step
+    mpx::AdjArgs a{};
+    a.g = s->g;
+    a.dt = s->d.dt;
+    a.chi = s->d.chi;
+    a.eps = s->d.eps;
+    a.epsf = s->d_eps;
+    for (int d = 0; d < 3; ++d) {
+        a.ikdE[d] = s->d_ikdE[d];
+        a.ikdH[d] = s->d_ikdH[d];
+        a.lE[d] = mem.get(s->cells, st);
+        a.lB[d] = mem.get(s->cells, st);
+    }
+    for (int q = 0; q < 6; ++q) {
+        a.dEb[q] = mem.get(s->cells, st);
+        a.dBb[q] = mem.get(s->cells, st);
+    }
+    a.epsb = mem.get(s->cells, st);
+    a.chib = mem.get(s->cells, st);
+    a.kobj = obj->plane_k;
+    a.wobj = obj->weight;
+    a.nefaces = (int)s->efaces.size();
+    a.nhfaces = (int)s->hfaces.size();
+    for (int d = 0; d < 3; ++d)
+        for (int q = 0; q < 2; ++q) a.efbd[d][q] = a.hfbd[d][q] = -1;
+    for (size_t f = 0; f < s->efaces.size(); ++f) {
+        a.eface[f] = s->efaces[f];
+        for (int w = 0; w < 2; ++w) a.lpsiE[f][w] = mem.get(face_count(s->efaces[f]), st);
+    }
+    for (size_t f = 0; f < s->hfaces.size(); ++f) {
+        a.hface[f] = s->hfaces[f];
+        for (int w = 0; w < 2; ++w) a.lpsiH[f][w] = mem.get(face_count(s->hfaces[f]), st);
+    }
+    for (const Current &c : s->ecur)
+        if (c.face >= 0) a.efbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    for (const Current &c : s->hcur)
+        if (c.face >= 0) a.hfbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    // forward E of one segment: (K + 1) planes of E
+    std::vector<std::array<double *, 3>> hist(K + 1);
+    for (auto &h : hist)
+        for (int c = 0; c < 3; ++c) h[c] = mem.get(s->cells, st);
+    for (int d = 0; d < 3; ++d)
+        if (!a.lE[d] || !a.lB[d]) return fail(MPX_ERR_NOMEM, "adjoint allocation failed");
+    for (int q = 0; q < 6; ++q)
+        if (!a.dEb[q] || !a.dBb[q]) return fail(MPX_ERR_NOMEM, "adjoint allocation failed");
+    if (!a.epsb || !a.chib) return fail(MPX_ERR_NOMEM, "adjoint allocation failed");
+    for (auto &h : hist)
+        for (int c = 0; c < 3; ++c)
+            if (!h[c]) return fail(MPX_ERR_NOMEM, "adjoint history allocation failed");
+
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int len = std::min(K, nsteps - seg * K);
+        if ((rc = restore(ckpt[seg]))) return rc;
+        s->time = t0 + seg * K;
+        auto keepE = [&](int m) -> int {
+            for (int c = 0; c < 3; ++c)
+                MPX_HIP(hipMemcpyAsync(hist[m][c], s->E[c], s->cells * 8, hipMemcpyDeviceToDevice, st));
+            return MPX_OK;
+        };
+        if ((rc = keepE(0))) return rc;
+        for (int m = 1; m <= len; ++m) {
+            if ((rc = step_once(s))) return rc;
+            if ((rc = keepE(m))) return rc;
+        }
+        for (int m = len - 1; m >= 0; --m) {
+            for (int c = 0; c < 3; ++c) {
+                a.E0[c] = hist[m][c];
+                a.E1[c] = hist[m + 1][c];
+            }
+            mpx::launch_adjoint_step(a, st);
+            MPX_LAUNCHED("adjoint step");
+        }
+    }
+    if ((rc = restore(fin))) return rc;
+    s->time = t0 + nsteps;
+    // gradients to the host: dJ/dEps in full-storage layout, dJ/dchi summed in a fixed order
+    const GridDims &g = s->g;
+    const size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+    std::vector<double> hb(s->cells);
+    if (dJ_deps) {
+        MPX_HIP(hipMemcpyAsync(hb.data(), a.epsb, s->cells * 8, hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < nx2; ++i)
+            for (size_t j = 0; j < ny2; ++j)
+                for (size_t k = 0; k < nz2; ++k)
+                    dJ_deps[(i * ny2 + j) * nz2 + k] = hb[g.at((int)i, (int)j, (int)k)];
+    }
+    if (dJ_dchi) {
+        MPX_HIP(hipMemcpyAsync(hb.data(), a.chib, s->cells * 8, hipMemcpyDeviceToHost, st));
+        MPX_HIP(hipStreamSynchronize(st));
+        double sum = 0.0;
+        for (int i = 1; i <= g.n[0]; ++i)
+            for (int j = 1; j <= g.n[1]; ++j)
+                for (int k = 1; k <= g.n[2]; ++k) sum += hb[g.at(i, j, k)];
+        *dJ_dchi = sum;
+    }
+    MPX_HIP(hipStreamSynchronize(st));
     return MPX_OK;
 }
 

@@ -107,6 +107,8 @@ struct StepArgs {
     double eps, chi;
     double denom, numer;  // eps + sumChi0 - sumXi0 (+sigma), eps - sumXi0 (-sigma), sigma = 0
     double idenom;        // 1 / denom
+    const double *epsf;   // per-cell permittivity (full storage) or nullptr for the uniform eps
+    double sumChi0, sumXi0;
     PoleConsts pc;
     int nfaces;
     CpmlFace face[kMaxFaces];   // E-side faces for the E kernel, H-side faces for the B kernel
@@ -149,6 +151,29 @@ struct SweepArgs {
     int sheets_first;
     int seg;       // x planes per block
     int variant;   // thread-block shape
+};
+
+// Adjoint of one step of the fused float64 PLRC update without Lorentz poles.  The adjoints
+// of the six curl differences of each kernel are kept per cell in the off-diagonal slots
+// [dim][comp] (slot dim*2 + (comp > dim ? comp-1 : comp)): the difference along dim of field
+// component comp.
+struct AdjArgs {
+    GridDims g;
+    double dt, chi, eps;      // eps: uniform permittivity when epsf is null
+    const double *epsf;
+    const double *ikdE[3], *ikdH[3];
+    double *lE[3], *lB[3];    // adjoints of E and B
+    double *dEb[6];           // adjoints of the E differences of the B update
+    double *dBb[6];           // adjoints of the B differences of the E update
+    double *epsb, *chib;      // per-cell gradient accumulators
+    const double *E0[3], *E1[3];  // forward E before and after the step
+    int kobj;                 // objective plane (z index)
+    double wobj;
+    int nefaces, nhfaces;
+    CpmlFace eface[kMaxFaces];
+    CpmlFace hface[kMaxFaces];
+    int efbd[3][2], hfbd[3][2];
+    double *lpsiE[kMaxFaces][2], *lpsiH[kMaxFaces][2];  // adjoints of the CPML auxiliaries
 };
 
 // one run of a batch: its tile grid and the number of its first tile in the batch's list

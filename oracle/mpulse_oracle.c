@@ -680,6 +680,7 @@ struct or_sim {
     int solver;
     /* grids */
     grid_t E[3], B[3], Sigma;
+    grid_t Eps; /* per-cell permittivity (allocated when set; not a reference grid) */
     grid_t PsiR[3][3], PsiI[3][3]; /* [pole][component] */
     grid_t P[3][3], Pp[3][3];      /* fdtd-disp */
     grid_t J[3], M[3];
@@ -1290,7 +1291,9 @@ static inline void plrc_step_d_cell(or_sim *s, int nonlin, double dt, int i, int
     }
     grid_t *Bx = &s->B[0], *By = &s->B[1], *Bz = &s->B[2];
     double sigma = 0.5 * AT(&s->Sigma, i, j, k) * dt;
-    double eps = s->d.eps;
+    /* a per-cell permittivity replaces the uniform eps where it has been set (the reference
+       has no such grid: with Eps = eps everywhere the update is the reference's) */
+    double eps = s->Eps.v ? AT(&s->Eps, i, j, k) : s->d.eps;
     if (nonlin) {
         double chi = s->d.chi;
         double denom = eps + s->sumChi0 - s->sumXi0 + sigma;
@@ -1800,6 +1803,7 @@ void or_destroy(or_sim *s)
         }
     }
     free(s->Sigma.v);
+    free(s->Eps.v);
     for (int q = 0; q < s->nlayers; ++q) free(s->layers[q].g.v);
     for (int q = 0; q < s->ncur; ++q) {
         free(s->cur[q].b.v);
@@ -1878,6 +1882,13 @@ static grid_t *find_grid(or_sim *s, const char *name)
             return g->v ? g : NULL;
         }
     if (strcmp(name, "Sigma") == 0) return s->Sigma.v ? &s->Sigma : NULL;
+    if (strcmp(name, "Eps") == 0) {
+        if (!s->Eps.v) {
+            if (!grid_alloc(&s->Eps, s->low, s->high)) return NULL;
+            grid_fill(&s->Eps, s->d.eps);
+        }
+        return &s->Eps;
+    }
     return NULL;
 }
 

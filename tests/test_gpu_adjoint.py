@@ -1,0 +1,174 @@
+"""Per-cell permittivity and the discrete adjoint of the fused float64 update (GPU).
+
+The forward update with a per-cell Eps grid is compared with the oracle (1e-12) and, for
+Eps equal to the uniform eps everywhere, with the run without the grid (bit for bit).  The
+adjoint gradients dJ/dEps and dJ/dchi of J = w sum_steps sum_ij |E(i, j, k_obj)|^2 are
+compared with central finite differences of J (h = 1e-4) at seeded cells, to 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from mpulse_amd import abi, configs
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def smooth_eps(shape, seed, lo=1.0, hi=2.25, sigma=2.0):
+    """Gaussian-filtered noise mapped affinely to [lo, hi] (periodic filter by FFT)."""
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) for n in shape]
+    kk = sum(np.meshgrid(*[q ** 2 for q in k], indexing="ij"))
+    f = np.real(np.fft.ifftn(np.fft.fftn(f) * np.exp(-2 * np.pi ** 2 * sigma ** 2 * kk)))
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+def desc_kerr(n=(20, 20, 36), d_cpml=4, d_src=7, chi=0.05, poles=(), path=abi.PATH_FUSED):
+    nx, ny, nz = n
+    d, _ = configs.plrc_kerr_pulse(nx, d_cpml, 4.0, 4, 1.5e9, 0, d_src=d_src, nz=nz, path=path, poles=poles)
+    d.ny = ny
+    d.chi = chi
+    d.factories[1].tshift = 6.0
+    d.factories[1].eps = 1.5
+    return d
+
+
+def with_eps(desc, eps_grid):
+    s = abi.Solver(desc)
+    full = np.full(s.shape, desc.eps)
+    full[1:-1, 1:-1, 1:-1] = eps_grid
+    s.set("Eps", full)
+    return s, full
+
+
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+def test_eps_field_forward_against_oracle(path):
+    d = desc_kerr(poles=((1.1, 4.0, 0.005),), path=path)
+    eps = smooth_eps((d.nx, d.ny, d.nz), seed=3)
+    s, full = with_eps(d, eps)
+    s.step(50)
+    g = {f: s.get(f) for f in FIELDS}
+    s.close()
+    o = Oracle(d)
+    o.set("Eps", full)
+    o.step(50)
+    for f in FIELDS:
+        r = o.get(f)
+        err = np.linalg.norm(g[f] - r) / np.linalg.norm(r)
+        print(path, f, err)
+        assert err < 1e-12, (f, err)
+
+
+def test_uniform_eps_field_reduces_to_the_scalar():
+    d = desc_kerr(poles=((1.1, 4.0, 0.005),))
+    with abi.Solver(d) as a:
+        a.step(40)
+        want = {f: a.get(f) for f in FIELDS}
+    s, _ = with_eps(d, np.full((d.nx, d.ny, d.nz), d.eps))
+    s.step(40)
+    for f in FIELDS:
+        assert np.array_equal(s.get(f), want[f]), f
+    s.close()
+
+
+NSTEPS, KOBJ, W = 70, 28, 1.0
+
+
+def objective(d, eps_full):
+    with abi.Solver(d) as s:
+        s.set("Eps", eps_full)
+        J, _, _ = s.adjoint_grad(KOBJ, W, NSTEPS, grad=False)
+    return J
+
+
+def test_adjoint_gradient_against_finite_differences():
+    d = desc_kerr()
+    eps = smooth_eps((d.nx, d.ny, d.nz), seed=2)
+    s, full = with_eps(d, eps)
+    J, geps, gchi = s.adjoint_grad(KOBJ, W, NSTEPS, checkpoint_every=9)
+    s.close()
+    print("J", J, "dJ/dchi", gchi, "max |dJ/deps|", np.abs(geps).max())
+    assert J > 0
+    assert np.all(geps[0] == 0) and np.all(geps[:, :, 0] == 0)
+    # seeded cells in the box the pulse crosses between the source and the objective plane
+    rng = configs.splitmix64(2)
+    cells = []
+    while len(cells) < 8:
+        c = (6 + next(rng) % 9, 6 + next(rng) % 9, 9 + next(rng) % 19)
+        if c not in cells:
+            cells.append(c)
+    h = 1e-4
+    gmax = np.abs(geps).max()
+    for c in cells:
+        fp, fm = full.copy(), full.copy()
+        fp[c] += h
+        fm[c] -= h
+        fd = (objective(d, fp) - objective(d, fm)) / (2 * h)
+        rel = abs(geps[c] - fd) / max(abs(fd), 1e-3 * gmax)
+        print(c, geps[c], fd, rel)
+        assert rel <= 1e-4, (c, geps[c], fd)
+    dp, dm = desc_kerr(), desc_kerr()
+    hc = 1e-4 * d.chi
+    dp.chi, dm.chi = d.chi + hc, d.chi - hc
+    fd = (objective(dp, full) - objective(dm, full)) / (2 * hc)
+    print("chi", gchi, fd)
+    assert abs(gchi - fd) <= 1e-4 * abs(fd)
+
+
+def test_adjoint_is_independent_of_the_segment_length_and_keeps_the_final_state():
+    d = desc_kerr()
+    eps = smooth_eps((d.nx, d.ny, d.nz), seed=5)
+    res = []
+    for K in (NSTEPS, 9, 1):
+        s, full = with_eps(d, eps)
+        J, geps, gchi = s.adjoint_grad(KOBJ, W, NSTEPS, checkpoint_every=K)
+        ex = s.get("Ex")
+        t = s.time
+        s.close()
+        res.append((J, geps, gchi, ex, t))
+    s, full = with_eps(d, eps)
+    s.step(NSTEPS)
+    ex = s.get("Ex")
+    s.close()
+    for J, geps, gchi, ex_k, t in res:
+        assert t == NSTEPS
+        assert np.array_equal(ex_k, ex)
+        assert J == res[0][0] and gchi == res[0][2]
+        assert np.array_equal(geps, res[0][1])
+
+
+def test_autograd_wrapper():
+    """A thin torch.autograd.Function over the C entry: forward J(eps), backward dJ/deps."""
+    torch = pytest.importorskip("torch")
+    d = desc_kerr()
+
+    class PlaneObjective(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, eps_t):
+            s = abi.Solver(d)
+            s.set("Eps", eps_t.detach().cpu().numpy())
+            J, g, _ = s.adjoint_grad(KOBJ, W, NSTEPS, checkpoint_every=10)
+            s.close()
+            ctx.save_for_backward(torch.from_numpy(g))
+            return eps_t.new_tensor(J)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (g,) = ctx.saved_tensors
+            return gJ * g
+
+    eps = smooth_eps((d.nx, d.ny, d.nz), seed=2)
+    full = np.full((d.nx + 2, d.ny + 2, d.nz + 2), d.eps)
+    full[1:-1, 1:-1, 1:-1] = eps
+    e = torch.tensor(full, dtype=torch.float64, requires_grad=True)
+    J = PlaneObjective.apply(e)
+    J.backward()
+    s, _ = with_eps(d, eps)
+    J2, g2, _ = s.adjoint_grad(KOBJ, W, NSTEPS, checkpoint_every=10)
+    s.close()
+    assert float(J) == J2
+    assert np.array_equal(e.grad.numpy(), g2)
