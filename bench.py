@@ -125,6 +125,61 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     return out
 
 
+def adjoint_gradient(fd=True):
+    """dJ/dEps and dJ/dchi of the plane objective on the 128^2 x 256 float64 workload
+    (per-cell permittivity, Kerr, 10-cell CPML, 2000 steps, checkpoints every 50 steps), timed
+    next to the forward run alone, and checked against central finite differences (h = 1e-4)
+    at 16 seeded cells and for chi."""
+    import numpy as np
+    import torch
+
+    from mpulse_amd import abi, configs
+    d, full, meta = configs.config_adjoint()
+    k, w, n, K = meta["k_obj"], meta["weight"], meta["steps"], meta["checkpoint_every"]
+
+    def run(desc, eps_full, grad):
+        s = abi.Solver(desc)
+        s.set("Eps", eps_full)
+        s.sync()
+        t0 = time.perf_counter()
+        res = s.adjoint_grad(k, w, n, checkpoint_every=K, grad=grad)
+        s.sync()
+        dt_ = time.perf_counter() - t0
+        s.close()
+        torch.cuda.empty_cache()
+        return res, dt_
+
+    (J0, _, _), t_fwd = run(d, full, False)
+    (J, g, gchi), t_adj = run(d, full, True)
+    cells = d.nx * d.ny * d.nz
+    out = {"grid": [d.nx, d.ny, d.nz], "steps": n, "checkpoint_every": K, "J": J,
+           "forward_s": t_fwd, "forward_and_adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+           "forward_cell_updates_per_s": cells * n / t_fwd, "dJ_dchi": gchi}
+    print(f"# adjoint: J {J:.6e}, forward {t_fwd:.2f} s, forward+adjoint {t_adj:.2f} s", file=sys.stderr,
+          flush=True)
+    if fd:
+        h = meta["fd_h"]
+        errs = []
+        gmax = float(np.abs(g).max())
+        for c in meta["fd_cells"]:
+            fp, fm = full.copy(), full.copy()
+            fp[c] += h
+            fm[c] -= h
+            fd_ = (run(d, fp, False)[0][0] - run(d, fm, False)[0][0]) / (2 * h)
+            errs.append({"cell": list(c), "adjoint": float(g[c]), "fd": fd_,
+                         "rel": abs(g[c] - fd_) / max(abs(fd_), 1e-300)})
+        dp, _, _ = configs.config_adjoint()
+        dm, _, _ = configs.config_adjoint()
+        hc = 1e-4 * d.chi
+        dp.chi, dm.chi = d.chi + hc, d.chi - hc
+        fdc = (run(dp, full, False)[0][0] - run(dm, full, False)[0][0]) / (2 * hc)
+        out["fd_check"] = {"h": h, "cells": errs, "max_rel_eps": max(e["rel"] for e in errs),
+                           "max_abs_grad": gmax, "chi_fd": fdc, "chi_rel": abs(gchi - fdc) / abs(fdc)}
+        print(f"# adjoint vs finite differences: max rel (eps) {out['fd_check']['max_rel_eps']:.2e}, "
+              f"chi rel {out['fd_check']['chi_rel']:.2e}", file=sys.stderr, flush=True)
+    return out
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -155,6 +210,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="also time a 4 GiB device copy")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-run batched sweep")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-run batched sweep")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient workload")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint-gradient workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -167,6 +224,9 @@ def main():
     torch.cuda.set_device(0)
     if args.batch_only:
         print(json.dumps({"batch_sweep": batch_sweep()}))
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint_gradient": adjoint_gradient()}))
         return
     n = args.n
     if args.forms:
@@ -236,6 +296,8 @@ def main():
             out[f"fp32_bulk_over_{ref}"] = main_res["cell_updates_per_s"] / results[ref]["cell_updates_per_s"]
     if not args.no_batch:
         out["batch_sweep"] = batch_sweep()
+    if not args.no_adjoint:
+        out["adjoint_gradient"] = adjoint_gradient()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -207,3 +207,56 @@ def config_sweep(nruns=64, steps=500, seed=1, precision=abi.PREC_FP64, path=abi.
         meta.update({"n": n, "chi3": chi3, "peak_field": peak})
         out.append((d, meta))
     return out
+
+
+def splitmix64_array(seed, n):
+    """The first n outputs of SplitMix64(seed) as a numpy uint64 array (the generator's state
+    advances by a constant, so output i depends on seed + (i + 1) * gamma only)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def smooth_permittivity(shape, seed=2, sigma=6.0, lo=1.0, hi=2.25):
+    """N(0,1) noise (Box-Muller on SplitMix64(seed) uniforms, cell order x, y, z with z fastest)
+    filtered by a periodic Gaussian of sigma cells (in Fourier space) and mapped affinely onto
+    [lo, hi]."""
+    n = int(np.prod(shape))
+    u = (splitmix64_array(seed, 2 * n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[0::2], u[1::2]
+    noise = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+    f = noise.reshape(shape)
+    k2 = sum(np.meshgrid(*[np.fft.fftfreq(m) ** 2 for m in shape], indexing="ij"))
+    f = np.real(np.fft.ifftn(np.fft.fftn(f) * np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * k2)))
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+def config_adjoint(nx=128, nz=256, steps=2000, d_cpml=10, d_src=16, k_obj=232, chi3_si=1e-22,
+                   checkpoint_every=50, fd_cells=16):
+    """Adjoint gradient: nx^2 x nz float64, no Lorentz pole, per-cell permittivity from
+    smooth_permittivity (seed 2, sigma 6 cells, [1, 2.25]), Kerr chi3 = 1e-22 m^2/V^2, 10-cell
+    CPML, the 10 fs / 15-cell-waist pulse at 1e9 V/m injected upward from the plane k = 16,
+    objective J = dx dy dt sum_steps sum_ij |E(i, j, 232)|^2, checkpoints every 50 steps.
+    Returns (desc, full-storage Eps, meta) with meta holding the objective, the
+    finite-difference cells (SplitMix64 seed 2, inside the pulse's path between the source and
+    the objective plane) and the step h = 1e-4."""
+    d, _ = plrc_kerr_pulse(nx, d_cpml, 10.0, 15, 1e9, steps, chi3_si=chi3_si, d_src=d_src, nz=nz,
+                           precision=abi.PREC_FP64, path=abi.PATH_FUSED, poles=())
+    eps = smooth_permittivity((nx, nx, nz))
+    full = np.full((nx + 2, nx + 2, nz + 2), 1.0)
+    full[1:-1, 1:-1, 1:-1] = eps
+    d.factories[1].eps = float(eps[:, :, d_src - 1].mean())
+    rng = splitmix64(2)
+    cells = []
+    lo, hi = nx // 2 - 12, nx // 2 + 12
+    while len(cells) < fd_cells:
+        c = (lo + int(next(rng) % (hi - lo + 1)), lo + int(next(rng) % (hi - lo + 1)),
+             d_src + 8 + int(next(rng) % (k_obj - d_src - 16)))
+        if c not in cells:
+            cells.append(c)
+    meta = {"steps": steps, "k_obj": k_obj, "weight": d.dx * d.dy * d.dt, "checkpoint_every": checkpoint_every,
+            "fd_cells": cells, "fd_h": 1e-4}
+    return d, full, meta
