@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MPX_ABI_VERSION 1
+#define MPX_ABI_VERSION 2
 #define MPX_MAX_FACTORIES 8
 #define MPX_MAX_POLES 3
 
@@ -75,6 +75,7 @@ extern "C" {
 #define MPX_FACTORY_SHORT_PULSE   3  /* "short_pulse_inject"   src/shortpulseinject.cpp      */
 #define MPX_FACTORY_PLANE_WAVE    4  /* "wave_inject"          src/sources.cpp:8-173         */
 #define MPX_FACTORY_PLANE_GAUSS   5  /* "plane_gauss_inject"   src/sources.cpp:179-331       */
+#define MPX_FACTORY_PLASMA_CURRENT 7 /* "plasma_current"     src/plasmacurrent.cpp:6-74   */
 #define MPX_FACTORY_FOCUSED_PULSE 6  /* "focused_pulse_inject" keys of src/focusedpulseinject.cpp:38-56;
                                         the field model is this library's own (the reference's does
                                         not compile): x-polarised Gaussian pupil of waist "width",
@@ -105,6 +106,10 @@ typedef struct mpx_factory {
     double ramp;          /* "ramp" (0.5)          */
     double gwidth;        /* plane_gauss "width"  (10) */
     double goffset;       /* plane_gauss "offset" (40) */
+    /* plasma_current keys, src/plasmacurrent.cpp:6-15 */
+    double em;            /* "em"     (1)   e/m of the carriers */
+    double gamma;         /* "gamma"  (0.1) friction           */
+    double rho_bg;        /* "rho_bg" (0)   background density */
 } mpx_factory;
 
 typedef struct mpx_desc {
@@ -129,6 +134,13 @@ typedef struct mpx_desc {
     double d_om2[MPX_MAX_POLES];    /* "L1Om2"..  */
     int    n_factories;   /* current factories in setup-file order */
     mpx_factory factories[MPX_MAX_FACTORIES];
+    /* "plasma" block of the PLRC solvers (PlasmaDensity, src/plasmadensity.cpp:62-73):
+       multiphoton + avalanche ionisation feeding the density and the resistivity Sigma */
+    int    plasma;        /* 1 when the block is present */
+    int    plasma_k;      /* "K"    (5) multiphoton order */
+    double plasma_nu;     /* "nu"   (1) avalanche rate    */
+    double plasma_mpa;    /* "mpa"  (1) multiphoton rate  */
+    double plasma_wion;   /* "Wion" (1) ionisation energy */
 } mpx_desc;
 
 typedef struct mpx_solver mpx_solver;
@@ -188,7 +200,8 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
 /* Launch-shape parameters: "march_e" / "march_b" = x planes walked by one thread in the
    E-side / B-side fused kernel (1, 2, 4); "sweep_seg" = x planes per block of the sweep kernel;
    "sweep_variant" = its thread-block shape (0: 64x8, 1: 64x4, 2: 64x16 threads; one
-   tile row per thread, the last row and lane of a tile being its halo).  No counterpart in the reference: the hook of
+   tile row per thread, the last row and lane of a tile being its halo).  "tile_order" = block order
+   of the fused kernels (0: z tiles fastest, 1: x chunks fastest).  No counterpart in the reference: the hook of
    the tile autotuner. */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
 

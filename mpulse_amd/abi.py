@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpulse_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_FACTORIES = 8
 
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
@@ -22,6 +22,7 @@ PREC_FP64, PREC_FP32_BULK = 0, 1
 PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
 FACTORY_FOCUSED_PULSE = 6
+FACTORY_PLASMA_CURRENT = 7
 
 # Direction enum of the reference (src/mpulse.h:28)
 NORTH, SOUTH, WEST, EAST, UP, DOWN = 0, 1, 2, 3, 4, 5
@@ -39,6 +40,7 @@ class Factory(C.Structure):
         ("amp", C.c_double), ("eps", C.c_double),
         ("k", C.c_double * 3), ("h", C.c_double * 3), ("h_bg", C.c_double * 3), ("e_bg", C.c_double * 3),
         ("ramp", C.c_double), ("gwidth", C.c_double), ("goffset", C.c_double),
+        ("em", C.c_double), ("gamma", C.c_double), ("rho_bg", C.c_double),
     ]
 
 
@@ -53,6 +55,8 @@ class Desc(C.Structure):
         ("d_eps2", C.c_double * 3), ("d_om2", C.c_double * 3),
         ("n_factories", C.c_int),
         ("factories", Factory * MAX_FACTORIES),
+        ("plasma", C.c_int), ("plasma_k", C.c_int),
+        ("plasma_nu", C.c_double), ("plasma_mpa", C.c_double), ("plasma_wion", C.c_double),
     ]
 
 
@@ -78,6 +82,8 @@ def desc_defaults():
     d.eps, d.chi = 1.0, 0.1
     for n in range(3):
         d.l_om2[n] = 1.0
+    d.plasma_k = 5  # src/plasmadensity.cpp:66-69
+    d.plasma_nu = d.plasma_mpa = d.plasma_wion = 1.0
     return d
 
 
@@ -91,6 +97,7 @@ def factory_defaults(kind):
         f.length, f.width, f.om0, f.amp, f.eps = 1.0, 1.0, 2 * np.pi, 1.0, 1.0
         f.k[2] = 1.0
         f.ramp, f.gwidth, f.goffset = 0.5, 10.0, 40.0
+    f.em, f.gamma, f.rho_bg = 1.0, 0.1, 0.0  # src/plasmacurrent.cpp:10-12
     return f
 
 

@@ -237,7 +237,8 @@ __global__ __launch_bounds__(256) void k_ref_sum(GridDims g, SumList L, double *
     for (int v = 0; v < L.n; ++v) {
         const SumView &w = L.v[v];
         if (i < w.lo[0] || i > w.hi[0] || j < w.lo[1] || j > w.hi[1] || k < w.lo[2] || k > w.hi[2]) continue;
-        size_t l = ((size_t)(i - w.lo[0]) * w.n1 + (size_t)(j - w.lo[1])) * w.n2 + (size_t)(k - w.lo[2]);
+        size_t l = w.padded ? g.at(i, j, k)
+                            : ((size_t)(i - w.lo[0]) * w.n1 + (size_t)(j - w.lo[1])) * w.n2 + (size_t)(k - w.lo[2]);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             if (w.comp[c]) t[c] += w.comp[c][l];
@@ -377,17 +378,38 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
 
 // Denominator and numerator of the E update at cell c: the uniform ones, or those of a
 // per-cell permittivity formed with the pole sums in the order the uniform ones are formed.
+// sigma = Sigma dt / 2 of the plasma resistivity (src/fdtd_plrc.cpp:356-359).
 template <bool NONLIN, typename A>
-__device__ __forceinline__ void cell_eps(const A &a, size_t c, double &denom, double &numer)
+__device__ __forceinline__ void cell_eps(const A &a, size_t c, double sigma, double &denom, double &numer)
 {
-    if (a.epsf) {
-        const double e = a.epsf[c];
-        denom = NONLIN ? e + a.sumChi0 - a.sumXi0 : e + (a.sumChi0 - a.sumXi0);
-        numer = e - a.sumXi0;
+    if (a.epsf || a.plasma) {
+        const double e = a.epsf ? a.epsf[c] : a.eps;
+        denom = NONLIN ? e + a.sumChi0 - a.sumXi0 + sigma : e + (a.sumChi0 - a.sumXi0) + sigma;
+        numer = e - a.sumXi0 - sigma;
     } else {
         denom = a.denom;
         numer = a.numer;
     }
+}
+
+// PlasmaDensity::stepScheme at one cell (src/plasmadensity.cpp:40-53): rho <- (rho (1 + n I)
+// + m I^K) / (1 - n I) with I = |E|^2; returns I^(K-1), which Sigma receives times w.
+__device__ __forceinline__ double plasma_density_cell(double ex, double ey, double ez, double &rho, double n,
+                                                      double m, int Km)
+{
+    const double I = ex * ex + ey * ey + ez * ez;
+    double IKm = 1.0;
+    for (int q = 0; q < Km; ++q) IKm *= I;  // ipow, src/util.h:67-73
+    const double IK = I * IKm;
+    rho = (rho * (1 + n * I) + m * IK) / (1 - n * I);
+    return IKm;
+}
+
+// PlasmaCurrent::stepScheme at one cell (src/plasmacurrent.cpp:63-71)
+__device__ __forceinline__ double plasma_current_comp(double j, double e, double rho_t, double gdtn, double gdtd,
+                                                      double emdt)
+{
+    return (j * gdtn - emdt * e * rho_t) / gdtd;
 }
 
 template <int NP, bool NONLIN, bool RECIP>
@@ -427,7 +449,7 @@ __global__ __launch_bounds__(256) void k_ref_stepD(StepArgs<double> a)
     }
     double ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
     double denom, numer;
-    cell_eps<NONLIN>(a, c, denom, numer);
+    cell_eps<NONLIN>(a, c, a.plasma ? 0.5 * a.sigma[c] * a.dt : 0.0, denom, numer);
     plrc_e_update<NP, NONLIN, false>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
                                      a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, denom, numer, a.chi);
     a.E[0][c] = ex;
@@ -481,6 +503,48 @@ void launch_ref_stepD(const StepArgs<double> &a, int np, bool nonlin, hipStream_
         }
     }
 #undef MPX_REF_D
+}
+
+// PlasmaDensity over the whole storage box 0..N+1 (Sigma was cleared: it becomes w I^(K-1))
+__global__ __launch_bounds__(256) void k_ref_plasma_density(StepArgs<double> a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    int i = blockIdx.z;
+    const GridDims &g = a.g;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    size_t c = g.at(i, j, k);
+    double r = a.rho[c];
+    const double ikm = plasma_density_cell(a.E[0][c], a.E[1][c], a.E[2][c], r, a.pn, a.pm, a.pk - 1);
+    a.rho[c] = r;
+    a.sigma[c] = 0.0 + a.pw * ikm;
+}
+
+// PlasmaCurrent over low..high-1 = 0..N of the storage box
+__global__ __launch_bounds__(256) void k_ref_plasma_current(StepArgs<double> a)
+{
+    int k = blockIdx.x * 64 + threadIdx.x;
+    int j = blockIdx.y * 4 + threadIdx.y;
+    int i = blockIdx.z;
+    const GridDims &g = a.g;
+    if (k > g.n[2] || j > g.n[1]) return;
+    size_t c = g.at(i, j, k);
+    const double rt = a.rho[c] + a.rho_bg;
+    for (int q = 0; q < 3; ++q) a.jp[q][c] = plasma_current_comp(a.jp[q][c], a.E[q][c], rt, a.gdtn, a.gdtd, a.emdt);
+}
+
+void launch_ref_plasma_density(const StepArgs<double> &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    hipLaunchKernelGGL(k_ref_plasma_density, dim3((g.n[2] + 2 + 63) / 64, (g.n[1] + 2 + 3) / 4, g.n[0] + 2),
+                       dim3(64, 4), 0, s, a);
+}
+
+void launch_ref_plasma_current(const StepArgs<double> &a, hipStream_t s)
+{
+    const GridDims &g = a.g;
+    hipLaunchKernelGGL(k_ref_plasma_current, dim3((g.n[2] + 1 + 63) / 64, (g.n[1] + 1 + 3) / 4, g.n[0] + 1),
+                       dim3(64, 4), 0, s, a);
 }
 
 void launch_ref_stepB(const StepArgs<double> &a, hipStream_t s)
@@ -685,46 +749,82 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
         const double dBy_x = by - by_im;
         bz_im = bz;
         by_im = by;
-        double jx = 0.0, jy = 0.0, jz = 0.0;
-        if (a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
-        if (CPML) {
-            // current totals in the reference's list order N,S (y), E,W (x), U,D (z); within a
-            // dimension the high face comes first in that list
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-                if (iny[s]) {
-                    const CpmlFace &f = a.face[a.face_by_dim[1][s]];
-                    double v0, v1;
-                    cpml_face<T, false>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
-                    jz += v0;
-                    jx += v1;
+        // the plasma OptField and the plasma current run before the E update, on E of this step
+        double sg = 0.0, jpv[3] = {0.0, 0.0, 0.0};
+        if (a.plasma || a.pcur) {
+            double r = a.rho[c];
+            if (a.plasma) {
+                sg = a.pw * plasma_density_cell(ex, ey, ez, r, a.pn, a.pm, a.pk - 1);
+                a.rho[c] = r;
+                a.sigma[c] = sg;
+            }
+            if (a.pcur) {
+                const double rt = r + a.rho_bg, ev[3] = {ex, ey, ez};
+                for (int q = 0; q < 3; ++q) {
+                    jpv[q] = plasma_current_comp(a.jp[q][c], ev[q], rt, a.gdtn, a.gdtd, a.emdt);
+                    a.jp[q][c] = jpv[q];
                 }
-            bool inx[2];
-            in_faces(a.face_by_dim, a.face, 0, i, inx);
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-                if (inx[s]) {
-                    const CpmlFace &f = a.face[a.face_by_dim[0][s]];
-                    double v0, v1;
-                    cpml_face<T, false>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
-                    jy += v0;
-                    jz += v1;
-                }
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-                if (inz[s]) {
-                    const CpmlFace &f = a.face[a.face_by_dim[2][s]];
-                    double v0, v1;
-                    cpml_face<T, false>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
-                    jx += v0;
-                    jy += v1;
-                }
+            }
         }
-        if (!a.sheets_first) add_sheets(a, i, j, k, jx, jy, jz);
+        // current totals in list order: the CPML and sheet groups, the plasma current at its place
+        double jx = 0.0, jy = 0.0, jz = 0.0;
+        auto add_cpml = [&]() {
+            if (CPML) {
+                // current totals in the reference's list order N,S (y), E,W (x), U,D (z); within a
+                // dimension the high face comes first in that list
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (iny[s]) {
+                        const CpmlFace &f = a.face[a.face_by_dim[1][s]];
+                        double v0, v1;
+                        cpml_face<T, false>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
+                        jz += v0;
+                        jx += v1;
+                    }
+                bool inx[2];
+                in_faces(a.face_by_dim, a.face, 0, i, inx);
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (inx[s]) {
+                        const CpmlFace &f = a.face[a.face_by_dim[0][s]];
+                        double v0, v1;
+                        cpml_face<T, false>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
+                        jy += v0;
+                        jz += v1;
+                    }
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (inz[s]) {
+                        const CpmlFace &f = a.face[a.face_by_dim[2][s]];
+                        double v0, v1;
+                        cpml_face<T, false>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
+                        jx += v0;
+                        jy += v1;
+                    }
+            }
+        };
+        auto add_plasma = [&](int pos) {
+            if (a.pcur && a.pcur_pos == pos) {
+                jx += jpv[0];
+                jy += jpv[1];
+                jz += jpv[2];
+            }
+        };
+        add_plasma(0);
+        if (a.sheets_first)
+            add_sheets(a, i, j, k, jx, jy, jz);
+        else
+            add_cpml();
+        add_plasma(1);
+        if (a.sheets_first)
+            add_cpml();
+        else
+            add_sheets(a, i, j, k, jx, jy, jz);
+        add_plasma(2);
         double idenom = a.idenom, numer = a.numer;
-        if (a.epsf) {
+        if (a.epsf || a.plasma) {
             double denom;
-            cell_eps<NONLIN>(a, (size_t)c, denom, numer);
+            cell_eps<NONLIN>(a, (size_t)c, 0.5 * sg * a.dt, denom, numer);
             idenom = 1.0 / denom;
         }
         plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy,
@@ -865,7 +965,10 @@ template <typename T, int NP, bool NONLIN, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_E(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    if (pa->order)
+        xcd_tile(nbz, nbx, nby, tbz, tbx, tby);
+    else
+        xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
     fused_E_tile<T, NP, NONLIN, CPML, L>(*pa, tbx, tby, tbz);
 }
 
@@ -873,7 +976,10 @@ template <typename T, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_B(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    if (pa->order)
+        xcd_tile(nbz, nbx, nby, tbz, tbx, tby);
+    else
+        xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
     fused_B_tile<T, CPML, L>(*pa, tbx, tby, tbz);
 }
 

@@ -12,6 +12,7 @@ constexpr int kMaxSumViews = kMaxFaces + kMaxSheets;
 struct SumView {
     int lo[3], hi[3];
     int n1, n2;
+    int padded;  // comp[] are full-storage grids (GridDims layout) rather than compact boxes
     const double *comp[3];
 };
 struct SumList {
@@ -46,6 +47,8 @@ template <typename T>
 void launch_wrap(const GridDims &g, T *const f[3], int dim, hipStream_t s);
 void launch_ref_stepD(const StepArgs<double> &a, int np, bool nonlin, hipStream_t s);
 void launch_ref_stepB(const StepArgs<double> &a, hipStream_t s);
+void launch_ref_plasma_density(const StepArgs<double> &a, hipStream_t s);
+void launch_ref_plasma_current(const StepArgs<double> &a, hipStream_t s);
 void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s);
 // `a` is the host copy (launch shape), `dev` the same block in device memory
 template <typename T>

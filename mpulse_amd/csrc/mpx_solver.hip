@@ -49,7 +49,7 @@ const int kFaceOrder[6] = {NORTH, SOUTH, EAST, WEST, UP, DOWN};  // src/cpml_bor
 int dim_of(int dir) { return (dir == EAST || dir == WEST) ? 0 : ((dir == NORTH || dir == SOUTH) ? 1 : 2); }
 bool is_high(int dir) { return dir == EAST || dir == NORTH || dir == UP; }
 
-enum CurType { CUR_CPML_E = 1, CUR_CPML_H, CUR_INC_E, CUR_INC_H };
+enum CurType { CUR_CPML_E = 1, CUR_CPML_H, CUR_INC_E, CUR_INC_H, CUR_PLASMA };
 
 struct Layer {
     std::string name;
@@ -119,6 +119,11 @@ struct mpx_solver {
     double *d_kdE[3] = {}, *d_kdH[3] = {};
     double *d_ikdE[3] = {}, *d_ikdH[3] = {};  // reciprocals, for the fused and sweep kernels
     double *d_eps = nullptr;                    // per-cell permittivity, once set (full storage)
+    // plasma: PlasmaDensity OptField and PlasmaCurrent (float64 grids over the storage box)
+    bool plasma = false, pcur = false;
+    int pcur_pos = 0;
+    double pcur_em = 0, pcur_gamma = 0, pcur_rhobg = 0;
+    double *d_rho = nullptr, *d_jp[3] = {}, *d_sigma = nullptr;
 
     std::vector<Layer> layers;
     std::vector<Current> ecur, hcur;
@@ -141,6 +146,7 @@ struct mpx_solver {
     double *d_part = nullptr, *d_energy = nullptr;
     int time = 0;
     int march_e = 1, march_b = 2;  // x planes per thread in the fused kernels (1, 2 or 4)
+    int tile_order = 0;            // fused kernels: block order, 0 = z tiles fastest, 1 = x chunks fastest
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
     double2 *psi2[3][3] = {};
@@ -497,6 +503,9 @@ void mpx_factory_defaults(mpx_factory *f, int kind)
         f->goffset = 40;
         if (kind == MPX_FACTORY_PLANE_WAVE || kind == MPX_FACTORY_PLANE_GAUSS) f->amp = 0;
     }
+    f->em = 1.0;  // src/plasmacurrent.cpp:10-12
+    f->gamma = 0.1;
+    f->rho_bg = 0.0;
 }
 
 void mpx_desc_defaults(mpx_desc *d)
@@ -513,6 +522,8 @@ void mpx_desc_defaults(mpx_desc *d)
     d->eps = 1.0;  // src/fdtd_plrc.t:417
     d->chi = 0.1;  // src/fdtd_plrc.cpp:463
     for (int n = 0; n < 3; ++n) d->l_om2[n] = 1.0;  // src/fdtd_plrc.t:427-429
+    d->plasma_k = 5;  // src/plasmadensity.cpp:66-69
+    d->plasma_nu = d->plasma_mpa = d->plasma_wion = 1.0;
 }
 
 static int build(mpx_solver *s, const mpx_desc *desc);
@@ -529,6 +540,8 @@ int mpx_create(const mpx_desc *desc, mpx_solver **out)
         return fail(MPX_ERR_INVALID, "unknown solver");
     if (desc->boundary != MPX_BOUNDARY_PERIODIC && desc->boundary != MPX_BOUNDARY_XY_PERIODIC)
         return fail(MPX_ERR_INVALID, "unknown boundary");
+    if (desc->plasma && desc->solver != MPX_SOLVER_PLRC_NONLIN && desc->solver != MPX_SOLVER_PLRC_LIN)
+        return fail(MPX_ERR_UNSUPPORTED, "the plasma block belongs to the FDTD_PLRC solvers");
     if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK)
         return fail(MPX_ERR_INVALID, "unknown precision");
     if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE && desc->path != MPX_PATH_SWEEP)
@@ -670,6 +683,24 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     c.factory = fi;
                     (h ? s->hcur : s->ecur).push_back(c);
                 }
+            } else if (f.kind == MPX_FACTORY_PLASMA_CURRENT) {
+                // PlasmaCurrentFactory::initCurrents, src/plasmacurrent.cpp:18-21
+                if (s->pcur) return fail(MPX_ERR_UNSUPPORTED, "at most one plasma_current is supported");
+                s->pcur = true;
+                s->pcur_em = f.em;
+                s->pcur_gamma = f.gamma;
+                s->pcur_rhobg = f.rho_bg;
+                bool seen_cpml = false, seen_sheet = false;
+                for (const Current &c : s->ecur) {
+                    seen_cpml |= c.type == CUR_CPML_E;
+                    seen_sheet |= c.type == CUR_INC_E;
+                }
+                s->pcur_pos = (int)seen_cpml + (int)seen_sheet;
+                Current c;
+                c.type = CUR_PLASMA;
+                c.dir = -1;
+                c.factory = fi;
+                s->ecur.push_back(c);
             } else {
                 return fail(MPX_ERR_UNSUPPORTED, "unsupported current factory kind " + std::to_string(f.kind));
             }
@@ -679,6 +710,10 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         auto init_side = [&](std::vector<Current> &list, bool isH) -> int {
             std::vector<Current> kept;
             for (Current c : list) {
+                if (c.type == CUR_PLASMA) {  // a grid current over the storage box, no layer
+                    kept.push_back(c);
+                    continue;
+                }
                 const mpx_factory &f = desc->factories[c.factory];
                 if (c.type == CUR_CPML_E || c.type == CUR_CPML_H) {
                     int distance = isH ? (is_high(c.dir) ? 1 : 0) : 1;
@@ -852,6 +887,26 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         s->haveJ = !s->ecur.empty();
         s->haveM = !s->hcur.empty();
 
+        // the plasma OptField (src/fdtd_plrc.t:431-433, PlasmaDensity) and the grids it shares
+        // with the plasma current ("PlasmaDensity", "PlasmaJx".., "Sigma")
+        s->plasma = desc->plasma != 0;
+        if ((s->plasma || s->pcur) && s->path == MPX_PATH_SWEEP)
+            return fail(MPX_ERR_UNSUPPORTED, "plasma needs the fused or reference-structured path");
+        if (s->plasma && desc->plasma_k < 1) return fail(MPX_ERR_INVALID, "plasma K must be >= 1");
+        if (s->plasma || s->pcur) {
+            s->d_rho = s->dalloc<double>(s->cells, rc);
+            if (rc) return rc;
+        }
+        if (s->pcur)
+            for (int c = 0; c < 3; ++c) {
+                s->d_jp[c] = s->dalloc<double>(s->cells, rc);
+                if (rc) return rc;
+            }
+        if (s->plasma) {
+            s->d_sigma = s->dalloc<double>(s->cells, rc);
+            if (rc) return rc;
+        }
+
         // kappa*delta lines, as plrcStepD/B form them per cell (src/fdtd_plrc.cpp:313-315, 435-437)
         for (int d = 0; d < 3; ++d) {
             std::vector<double> kdE(g.n[d] + 2), kdH(g.n[d] + 2);
@@ -972,6 +1027,24 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.sumChi0 = s->sumChi0;
     a.sumXi0 = s->sumXi0;
     a.pc = s->pc;
+    if (!hside) {
+        // src/plasmadensity.cpp:30-33, src/plasmacurrent.cpp:57-59
+        const double dt = s->d.dt;
+        a.plasma = s->plasma ? 1 : 0;
+        a.pcur = s->pcur ? 1 : 0;
+        a.pcur_pos = s->pcur_pos;
+        a.pk = s->d.plasma_k;
+        a.pn = 0.5 * dt * s->d.plasma_nu;
+        a.pm = dt * s->d.plasma_mpa;
+        a.pw = s->d.plasma_mpa * s->d.plasma_wion;
+        a.gdtn = 1 - 0.5 * s->pcur_gamma * dt;
+        a.gdtd = 1 + 0.5 * s->pcur_gamma * dt;
+        a.emdt = s->pcur_em * dt;
+        a.rho_bg = s->pcur_rhobg;
+        a.rho = s->d_rho;
+        a.sigma = s->d_sigma;
+        for (int c = 0; c < 3; ++c) a.jp[c] = s->d_jp[c];
+    }
     const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
     a.nfaces = (int)faces.size();
     for (int d = 0; d < 3; ++d) a.face_by_dim[d][0] = a.face_by_dim[d][1] = -1;
@@ -988,6 +1061,7 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.sheets_first = s->sheets_first;
     a.haveJ = 0;
     a.march = hside ? s->march_b : s->march_e;
+    a.order = s->tile_order;
     return a;
 }
 
@@ -998,6 +1072,16 @@ static mpx::SumList sum_list(mpx_solver *s, bool hside)
     const std::vector<CpmlFace> &faces = hside ? s->hfaces : s->efaces;
     for (const Current &c : list) {
         mpx::SumView &v = L.v[L.n++];
+        if (c.type == CUR_PLASMA) {  // the plasma current's grid spans the storage box
+            for (int d = 0; d < 3; ++d) {
+                v.lo[d] = 0;
+                v.hi[d] = s->g.n[d] + 1;
+                v.comp[d] = s->d_jp[d];
+            }
+            v.n1 = v.n2 = 0;
+            v.padded = 1;
+            continue;
+        }
         const Layer &Ly = s->layers[c.layer];
         for (int d = 0; d < 3; ++d) {
             v.lo[d] = Ly.lo[d];
@@ -1209,6 +1293,11 @@ static int step_once(mpx_solver *s)
         if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
     } else if (s->path == MPX_PATH_REFERENCE) {
         StepArgs<double> ae = step_args<double>(s, false);
+        // Sigma := 0 and the E-side OptFields, src/fdtd_plrc.t:129-135
+        if (s->plasma) {
+            mpx::launch_ref_plasma_density(ae, s->stream);
+            MPX_LAUNCHED("plasma density");
+        }
         // E currents in list order
         for (Current &c : s->ecur) {
             if (c.type == CUR_CPML_E) {
@@ -1216,6 +1305,9 @@ static int step_once(mpx_solver *s)
                 mpx::launch_ref_cpml(ae, c.face, false, s->stream);
                 s->tock();
                 MPX_LAUNCHED("cpml E");
+            } else if (c.type == CUR_PLASMA) {
+                mpx::launch_ref_plasma_current(ae, s->stream);
+                MPX_LAUNCHED("plasma current");
             } else {
                 c.ev.time = s->time;
                 s->tick(K_SHEET);
@@ -1470,14 +1562,63 @@ static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, 
         }
     if (!std::strcmp(name, "Sigma") && s->plrc) {
         *elem = 8;
-        return MPX_OK;  // always zero: no OptField on this path
+        *ptr = s->d_sigma;  // stored with the plasma OptField; otherwise identically zero
+        return MPX_OK;
     }
+    if (!std::strcmp(name, "PlasmaDensity") && s->d_rho) {
+        *elem = 8;
+        *ptr = s->d_rho;
+        return MPX_OK;
+    }
+    static const char *pj[3] = {"PlasmaJx", "PlasmaJy", "PlasmaJz"};
+    for (int q = 0; q < 3; ++q)
+        if (!std::strcmp(name, pj[q]) && s->d_jp[q]) {
+            *elem = 8;
+            *ptr = s->d_jp[q];
+            return MPX_OK;
+        }
     if (!std::strcmp(name, "Eps") && s->plrc) {
         *elem = 8;
         *ptr = s->d_eps;  // nullptr: the uniform eps of the descriptor
         return MPX_OK;
     }
     return MPX_ERR_NAME;
+}
+
+extern "C" {
+
+}  // extern "C"
+
+// Ghost values of the plasma grids on the fused path, which updates interior cells only.
+// The reference updates the density and Sigma over the whole storage box from ghost E values
+// that are periodic images, so their ghosts are the images too; its plasma current loop stops
+// at N (src/plasmacurrent.cpp:61-63), so the high ghosts stay zero.  Along a non-periodic z
+// the ghost E is zero: the density and current stay zero there and Sigma is w 0^(K-1).
+static void plasma_ghosts(const mpx_solver *s, double *h, int kind /* 0 density, 1 Sigma, 2 current */)
+{
+    const int n[3] = {s->g.n[0], s->g.n[1], s->g.n[2]};
+    const size_t e[3] = {(size_t)n[0] + 2, (size_t)n[1] + 2, (size_t)n[2] + 2};
+    auto at = [&](size_t i, size_t j, size_t k) -> double & { return h[(i * e[1] + j) * e[2] + k]; };
+    const double zero_sigma = s->d.plasma_k == 1 ? s->d.plasma_mpa * s->d.plasma_wion : 0.0;
+    for (int d = 0; d < 3; ++d) {
+        for (size_t a = 0; a < e[(d + 1) % 3]; ++a)
+            for (size_t b = 0; b < e[(d + 2) % 3]; ++b) {
+                size_t idx[3];
+                auto cell = [&](size_t v) -> double & {
+                    idx[d] = v;
+                    idx[(d + 1) % 3] = a;
+                    idx[(d + 2) % 3] = b;
+                    return at(idx[0], idx[1], idx[2]);
+                };
+                if (s->periodic[d]) {
+                    cell(0) = cell((size_t)n[d]);
+                    cell((size_t)n[d] + 1) = kind == 2 ? 0.0 : cell(1);
+                } else {
+                    cell(0) = kind == 1 ? zero_sigma : 0.0;
+                    cell((size_t)n[d] + 1) = kind == 1 ? zero_sigma : 0.0;
+                }
+            }
+    }
 }
 
 extern "C" {
@@ -1519,6 +1660,14 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
                 else
                     host[dst] = ((const float *)buf.data())[src];
             }
+    if (s->path != MPX_PATH_REFERENCE) {
+        if (ptr == s->d_rho)
+            plasma_ghosts(s, host, 0);
+        else if (ptr == s->d_sigma)
+            plasma_ghosts(s, host, 1);
+        else if (ptr == s->d_jp[0] || ptr == s->d_jp[1] || ptr == s->d_jp[2])
+            plasma_ghosts(s, host, 2);
+    }
     return MPX_OK;
 }
 
@@ -1757,7 +1906,10 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
         s->march_e = value;
     else if (!std::strcmp(key, "march_b"))
         s->march_b = value;
-    else if (!std::strcmp(key, "sweep_seg")) {
+    else if (!std::strcmp(key, "tile_order")) {
+        if (value != 0 && value != 1) return fail(MPX_ERR_INVALID, "tile_order must be 0 or 1");
+        s->tile_order = value;
+    } else if (!std::strcmp(key, "sweep_seg")) {
         if (value < 1) return fail(MPX_ERR_INVALID, "sweep_seg must be >= 1");
         s->sweep_seg = value < s->g.n[0] ? value : s->g.n[0];
     } else if (!std::strcmp(key, "sweep_variant")) {
@@ -2038,6 +2190,10 @@ int mpx_batch_create(const mpx_desc *descs, int nruns, mpx_batch **out)
         d.path = MPX_PATH_SWEEP;
         if (d.solver != MPX_SOLVER_PLRC_NONLIN && d.solver != MPX_SOLVER_PLRC_LIN)
             return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs must use an FDTD_PLRC solver"));
+        if (d.plasma) return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take the plasma block"));
+        for (int q = 0; q < d.n_factories; ++q)
+            if (d.factories[q].kind == MPX_FACTORY_PLASMA_CURRENT)
+                return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take a plasma current"));
         mpx_solver *s = nullptr;
         int rc = mpx_create(&d, &s);
         if (rc) return bail(rc);
@@ -2314,9 +2470,9 @@ int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K,
 {
     if (!s || !obj || !J || nsteps < 1 || K < 1) return fail(MPX_ERR_INVALID, "invalid adjoint arguments");
     MPX_NOT_IN_BATCH(s);
-    if (!s->plrc || s->np != 0 || s->esz != 8 || s->path != MPX_PATH_FUSED)
-        return fail(MPX_ERR_UNSUPPORTED,
-                    "the adjoint covers the float64 fused FDTD_PLRC update without active Lorentz poles");
+    if (!s->plrc || s->np != 0 || s->esz != 8 || s->path != MPX_PATH_FUSED || s->plasma || s->pcur)
+        return fail(MPX_ERR_UNSUPPORTED, "the adjoint covers the float64 fused FDTD_PLRC update without active "
+                                         "Lorentz poles or plasma");
     if (!s->periodic[0] || !s->periodic[1] || !s->periodic[2])
         return fail(MPX_ERR_UNSUPPORTED, "the adjoint needs the periodic boundary");
     if (obj->plane_k < 1 || obj->plane_k > s->g.n[2]) return fail(MPX_ERR_INVALID, "objective plane out of range");

@@ -109,6 +109,12 @@ struct StepArgs {
     double idenom;        // 1 / denom
     const double *epsf;   // per-cell permittivity (full storage) or nullptr for the uniform eps
     double sumChi0, sumXi0;
+    // plasma: the PlasmaDensity OptField (plasma) and one PlasmaCurrent (pcur), float64 state
+    // over the storage box; pcur_pos = number of current groups (CPML, sheets) listed before it
+    int plasma, pcur, pcur_pos, pk;
+    double pn, pm, pw;              // dt nu / 2, dt mpa, mpa Wion
+    double gdtn, gdtd, emdt, rho_bg;
+    double *rho, *jp[3], *sigma;
     PoleConsts pc;
     int nfaces;
     CpmlFace face[kMaxFaces];   // E-side faces for the E kernel, H-side faces for the B kernel
@@ -121,6 +127,7 @@ struct StepArgs {
     int haveJ;
     int blo[3];                 // first updated B index per dimension (fused path)
     int march;                  // fused path: x planes walked by one thread
+    int order;                  // fused path: 0 = z tiles fastest in the block order, 1 = x chunks fastest
 };
 
 // Single-pass sweep path: "o" = buffers of step n (read), "n" = buffers of step n+1 (written).

@@ -283,14 +283,47 @@ struct Unsupported : CurrentFactory {
   protected:
     void MakeParamMap(ParamMap &) override {}
 };
-static const char kPlasma[] = "plasma";
-static const char kPlasmaCurrent[] = "plasma_current";
 static const char kSideInject[] = "side_inject";
 static const char kGaussInject[] = "gauss_inject";
 
 static void push_factory(void *owner, Rebuildable *r)
 {
     static_cast<FDTD_PLRCSolverHIP *>(owner)->currentFactories.push_back(static_cast<CurrentFactory *>(r));
+}
+
+static void set_plasma(void *owner, Rebuildable *r)
+{
+    FDTD_PLRCSolverHIP *s = static_cast<FDTD_PLRCSolverHIP *>(owner);
+    if (s->plasma) {
+        delete r;
+        throw SetupError("one plasma block per solver is provided by the GPU solver");
+    }
+    s->plasma = static_cast<PlasmaDensityBlock *>(r);
+}
+
+void PlasmaCurrentFactory::MakeParamMap(ParamMap &pm)
+{
+    pm["em"] = pdouble(&em, 1.0);
+    pm["gamma"] = pdouble(&gamma, 0.1);
+    pm["rho_bg"] = pdouble(&rho_bg, 0.0);
+}
+
+mpx_factory PlasmaCurrentFactory::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_PLASMA_CURRENT);
+    f.em = em;
+    f.gamma = gamma;
+    f.rho_bg = rho_bg;
+    return f;
+}
+
+void PlasmaDensityBlock::MakeParamMap(ParamMap &pm)
+{
+    pm["nu"] = pdouble(&nu, 1.0);
+    pm["mpa"] = pdouble(&mpa, 1.0);
+    pm["Wion"] = pdouble(&Wion, 1.0);
+    pm["K"] = pint(&K, 5);
 }
 
 void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
@@ -311,8 +344,8 @@ void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
     pm["short_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, ShortPulseInject>(this, push_factory);
     pm["wave_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlaneWaveSource>(this, push_factory);
     pm["plane_gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlaneGaussSource>(this, push_factory);
-    pm["plasma"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kPlasma>>(this, push_factory);
-    pm["plasma_current"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kPlasmaCurrent>>(this, push_factory);
+    pm["plasma"] = pobject<FDTD_PLRCSolverHIP, Rebuildable, PlasmaDensityBlock>(this, set_plasma);
+    pm["plasma_current"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlasmaCurrentFactory>(this, push_factory);
     pm["side_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kSideInject>>(this, push_factory);
     pm["gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kGaussInject>>(this, push_factory);
     pm["focused_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, FocusedPulseInject>(this, push_factory);
@@ -321,6 +354,7 @@ void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
 FDTD_PLRCSolverHIP::~FDTD_PLRCSolverHIP()
 {
     for (CurrentFactory *f : currentFactories) delete f;
+    delete plasma;
 }
 
 void FDTD_PLRCSolverHIP::describe(mpx_desc &d) const
@@ -337,6 +371,13 @@ void FDTD_PLRCSolverHIP::describe(mpx_desc &d) const
     for (CurrentFactory *f : currentFactories) {
         if (d.n_factories >= MPX_MAX_FACTORIES) throw SetupError("too many current factories");
         d.factories[d.n_factories++] = f->describe();
+    }
+    d.plasma = plasma ? 1 : 0;
+    if (plasma) {
+        d.plasma_k = plasma->K;
+        d.plasma_nu = plasma->nu;
+        d.plasma_mpa = plasma->mpa;
+        d.plasma_wion = plasma->Wion;
     }
 }
 

@@ -169,6 +169,25 @@ class FocusedPulseInject : public IncidentSource {
     void MakeParamMap(ParamMap &pm) override;
 };
 
+class PlasmaCurrentFactory : public CurrentFactory {  // src/plasmacurrent.cpp:6-21
+  public:
+    double em = 1, gamma = 0.1, rho_bg = 0;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
+// The "plasma" OptField of the PLRC solvers (PlasmaDensity, src/plasmadensity.cpp:62-73)
+class PlasmaDensityBlock : public Rebuildable {
+  public:
+    double nu = 1, mpa = 1, Wion = 1;
+    int K = 5;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
 class PlaneWaveSource : public IncidentSource {  // src/sources.cpp:58-82
   public:
     double kx = 0, ky = 0, kz = 1, Hx = 0, Hy = 0, Hz = 0, Hx_bg = 0, Hy_bg = 0, Hz_bg = 0;
@@ -226,6 +245,7 @@ class FDTD_PLRCSolverHIP : public FieldSolver {
     double eps = 1, chi = 0.1;
     double LEps[3] = {0, 0, 0}, LDelta[3] = {0, 0, 0}, LOm2[3] = {1, 1, 1};
     std::list<CurrentFactory *> currentFactories;
+    PlasmaDensityBlock *plasma = nullptr;
     ~FDTD_PLRCSolverHIP() override;
     void describe(mpx_desc &d) const override;
 

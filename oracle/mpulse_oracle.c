@@ -635,7 +635,7 @@ void or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy
 /* Currents (CPML borders and incident-source sheets)                                   */
 /* ------------------------------------------------------------------------------------ */
 
-enum { CUR_CPML_E = 1, CUR_CPML_H = 2, CUR_INC_E = 3, CUR_INC_H = 4 };
+enum { CUR_CPML_E = 1, CUR_CPML_H = 2, CUR_INC_E = 3, CUR_INC_H = 4, CUR_PLASMA = 5 };
 enum { SRC_PULSE = 1, SRC_PLANE = 2, SRC_FOCUSED = 3 };
 
 typedef struct {
@@ -666,6 +666,8 @@ typedef struct {
     pulse_t pulse;
     plane_t plane;
     focused_t focused;
+    /* plasma current */
+    double em, gamma, rho_bg;
 } current_t;
 
 #define MAX_CURRENTS 64
@@ -681,6 +683,7 @@ struct or_sim {
     /* grids */
     grid_t E[3], B[3], Sigma;
     grid_t Eps; /* per-cell permittivity (allocated when set; not a reference grid) */
+    grid_t Rho, Jp[3]; /* "PlasmaDensity", "PlasmaJx".."PlasmaJz" */
     grid_t PsiR[3][3], PsiI[3][3]; /* [pole][component] */
     grid_t P[3][3], Pp[3][3];      /* fdtd-disp */
     grid_t J[3], M[3];
@@ -1125,6 +1128,46 @@ static void inc_step_H(or_sim *s, current_t *c)
             }
 }
 
+/* PlasmaDensity::stepScheme, src/plasmadensity.cpp:18-59: over the whole storage box,
+   rho <- (rho (1 + n I) + m I^K) / (1 - n I), Sigma += w I^(K-1), I = |E|^2, with
+   n = dt nu / 2, m = dt mpa, w = mpa Wion (the absorption warning is not reproduced) */
+static void plasma_density_step(or_sim *s, double dt)
+{
+    const double n = 0.5 * dt * s->d.plasma_nu, m = dt * s->d.plasma_mpa, w = s->d.plasma_mpa * s->d.plasma_wion;
+    const int Km = s->d.plasma_k - 1;
+#pragma omp parallel for schedule(static)
+    for (int i = s->low[0]; i <= s->high[0]; ++i)
+        for (int j = s->low[1]; j <= s->high[1]; ++j)
+            for (int k = s->low[2]; k <= s->high[2]; ++k) {
+                double ex = AT(&s->E[0], i, j, k), ey = AT(&s->E[1], i, j, k), ez = AT(&s->E[2], i, j, k);
+                double I = ex * ex + ey * ey + ez * ez;
+                double IKm = 1.0;
+                for (int q = 0; q < Km; ++q) IKm *= I; /* ipow, src/util.h:67-73 */
+                double IK = I * IKm;
+                double *rho = &AT(&s->Rho, i, j, k);
+                *rho = (*rho * (1 + n * I) + m * IK) / (1 - n * I);
+                AT(&s->Sigma, i, j, k) += w * IKm;
+            }
+}
+
+/* PlasmaCurrent::stepScheme, src/plasmacurrent.cpp:41-74: over low..high-1 of the box,
+   J <- (J (1 - gamma dt / 2) - em dt E (rho + rho_bg)) / (1 + gamma dt / 2) */
+static void plasma_current_step(or_sim *s, current_t *c)
+{
+    const double dt = s->dt;
+    const double gdtn = 1 - 0.5 * c->gamma * dt, gdtd = 1 + 0.5 * c->gamma * dt, emdt = c->em * dt;
+#pragma omp parallel for schedule(static)
+    for (int i = s->low[0]; i < s->high[0]; ++i)
+        for (int j = s->low[1]; j < s->high[1]; ++j)
+            for (int k = s->low[2]; k < s->high[2]; ++k) {
+                double rho = AT(&s->Rho, i, j, k) + c->rho_bg;
+                for (int q = 0; q < 3; ++q) {
+                    double *jq = &AT(&s->Jp[q], i, j, k);
+                    *jq = (*jq * gdtn - emdt * AT(&s->E[q], i, j, k) * rho) / gdtd;
+                }
+            }
+}
+
 static void current_step(or_sim *s, current_t *c)
 {
     switch (c->type) {
@@ -1132,6 +1175,7 @@ static void current_step(or_sim *s, current_t *c)
     case CUR_CPML_H: cpml_step_H(s, c); break;
     case CUR_INC_E: inc_step_E(s, c); break;
     case CUR_INC_H: inc_step_H(s, c); break;
+    case CUR_PLASMA: plasma_current_step(s, c); break;
     }
 }
 
@@ -1434,6 +1478,7 @@ static void plrc_init_accumulator(or_sim *s, double dt)
 static void plrc_step_scheme(or_sim *s, double dt)
 {
     grid_fill(&s->Sigma, 0);
+    if (s->d.plasma) plasma_density_step(s, dt); /* the E-side OptField */
     for (int q = 0; q < s->nE; ++q) current_step(s, &s->cur[s->elist[q]]);
     plrc_stepD(s, dt);
     for (int q = 0; q < s->nH; ++q) current_step(s, &s->cur[s->hlist[q]]);
@@ -1445,9 +1490,11 @@ static void plrc_step_scheme_init(or_sim *s, double dt)
 {
     plrc_init_accumulator(s, dt);
     plrc_stepB(s, 0.5 * dt);
+    /* PlasmaDensity::stepSchemeInit = stepScheme(dt / 2), src/plasmadensity.h:35 */
+    if (s->d.plasma) plasma_density_step(s, 0.5 * dt);
     for (int q = 0; q < s->nE; ++q) {
         current_t *c = &s->cur[s->elist[q]];
-        if (c->type == CUR_INC_E) current_step(s, c); /* CPML E: empty stepSchemeInit */
+        if (c->type == CUR_INC_E) current_step(s, c); /* CPML E, plasma: empty stepSchemeInit */
     }
     for (int q = 0; q < s->nH; ++q) current_step(s, &s->cur[s->hlist[q]]);
 }
@@ -1633,6 +1680,11 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
         free(s);
         return NULL;
     }
+    if (!isPLRC && desc->plasma) {
+        seterr(err, errlen, "the plasma block belongs to the FDTD_PLRC solvers");
+        free(s);
+        return NULL;
+    }
     if (isPLRC && (desc->l_om2[0] == 0 || desc->l_om2[1] == 0 || desc->l_om2[2] == 0)) {
         /* src/fdtd_plrc.t:20-24 */
         seterr(err, errlen, "FDTD_PLRCSolver: Omega should not be zero!");
@@ -1719,11 +1771,40 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
                     else
                         s->elist[s->nE++] = (int)(c - s->cur);
                 }
+            } else if (f->kind == MPX_FACTORY_PLASMA_CURRENT) {
+                /* PlasmaCurrentFactory::initCurrents, src/plasmacurrent.cpp:18-21 */
+                current_t *c = new_current(s, CUR_PLASMA, -1);
+                if (!c) {
+                    seterr(err, errlen, "too many currents");
+                    or_destroy(s);
+                    return NULL;
+                }
+                c->em = f->em;
+                c->gamma = f->gamma;
+                c->rho_bg = f->rho_bg;
+                c->valid = 1;
+                for (int q = 0; q < 3; ++q) {
+                    if (!s->Jp[q].v && !grid_alloc(&s->Jp[q], s->low, s->high)) {
+                        seterr(err, errlen, "out of memory");
+                        or_destroy(s);
+                        return NULL;
+                    }
+                    c->J[q] = &s->Jp[q];
+                }
+                s->elist[s->nE++] = (int)(c - s->cur);
             } else {
                 seterr(err, errlen, "unsupported current factory");
                 or_destroy(s);
                 return NULL;
             }
+        }
+        /* the plasma density grid, shared by the OptField and the plasma current */
+        int needRho = desc->plasma;
+        for (int q = 0; q < s->nE; ++q) needRho |= s->cur[s->elist[q]].type == CUR_PLASMA;
+        if (needRho && !grid_alloc(&s->Rho, s->low, s->high)) {
+            seterr(err, errlen, "out of memory");
+            or_destroy(s);
+            return NULL;
         }
         /* coreInitStorage, src/fdtd_plrc.cpp:13-86 */
         if (!grid_alloc(&s->Sigma, s->low, s->high)) {
@@ -1744,7 +1825,7 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
             current_t *c = &s->cur[s->elist[q]];
             if (c->type == CUR_CPML_E)
                 cpml_init_storage(s, c, 0);
-            else
+            else if (c->type == CUR_INC_E)
                 inc_init_storage(s, c, 0);
             if (c->valid) s->elist[ne++] = s->elist[q];
         }
@@ -1804,6 +1885,8 @@ void or_destroy(or_sim *s)
     }
     free(s->Sigma.v);
     free(s->Eps.v);
+    free(s->Rho.v);
+    for (int q = 0; q < 3; ++q) free(s->Jp[q].v);
     for (int q = 0; q < s->nlayers; ++q) free(s->layers[q].g.v);
     for (int q = 0; q < s->ncur; ++q) {
         free(s->cur[q].b.v);
@@ -1882,6 +1965,10 @@ static grid_t *find_grid(or_sim *s, const char *name)
             return g->v ? g : NULL;
         }
     if (strcmp(name, "Sigma") == 0) return s->Sigma.v ? &s->Sigma : NULL;
+    if (strcmp(name, "PlasmaDensity") == 0) return s->Rho.v ? &s->Rho : NULL;
+    static const char *pj[3] = {"PlasmaJx", "PlasmaJy", "PlasmaJz"};
+    for (int q = 0; q < 3; ++q)
+        if (strcmp(name, pj[q]) == 0) return s->Jp[q].v ? &s->Jp[q] : NULL;
     if (strcmp(name, "Eps") == 0) {
         if (!s->Eps.v) {
             if (!grid_alloc(&s->Eps, s->low, s->high)) return NULL;

@@ -60,3 +60,29 @@ def test_fp32_bulk_small():
 @pytest.mark.skipif(os.environ.get("MPX_SKIP_LARGE") == "1", reason="large case disabled by MPX_SKIP_LARGE=1")
 def test_fp32_bulk_512_1000_steps():
     check(compare(512, 1000))
+
+
+def test_fp32_bulk_focused_two_poles():
+    """The two-pole (silica-like) medium with a focused pulse: FP32-bulk against float64."""
+    out = {}
+    num = {"E": 0.0, "B": 0.0}
+    den = {"E": 0.0, "B": 0.0}
+    runs = []
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_BULK):
+        d, _ = configs.config_focus(precision=prec, nx=64, nz=96, steps=300, d_cpml=8, d_src=12, na=0.3)
+        d.factories[1].tshift = 20.0
+        d.factories[1].length = 8.0
+        runs.append(d)
+    with abi.Solver(runs[0]) as a, abi.Solver(runs[1]) as b:
+        a.step(300)
+        b.step(300)
+        for f in FIELDS:
+            x, y = b.get(f), a.get(f)
+            out[f] = rel_l2(x, y)
+            num[f[0]] += float(np.sum((x - y) ** 2))
+            den[f[0]] += float(np.sum(y ** 2))
+    for g in ("E", "B"):
+        out[g] = (num[g] / den[g]) ** 0.5
+    print({k: f"{v:.2e}" for k, v in out.items()}, flush=True)
+    check(out)
+

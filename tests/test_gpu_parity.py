@@ -142,3 +142,27 @@ def test_focused_pulse_against_oracle(path):
         print(n, err)
         assert err < 1e-12, (n, err)
     assert np.abs(r["Ex"]).max() > 0.5
+
+
+PLASMA_GRIDS = ("PlasmaDensity", "PlasmaJx", "PlasmaJy", "PlasmaJz", "Sigma")
+
+
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+def test_plasma_against_oracle(path):
+    """Pulse into a Kerr medium with multiphoton ionisation, resistivity and a plasma current
+    (listed between the CPML and the source), 90 steps; the full storage of every field."""
+    d, _ = configs.plrc_kerr_pulse(30, 6, 6.0, 6, 4e9, 0, d_src=9, path=path, poles=((1.1, 4.0, 0.005),))
+    d.factories[1].tshift = 12.0
+    f = abi.factory_defaults(abi.FACTORY_PLASMA_CURRENT)
+    f.em, f.gamma, f.rho_bg = 0.8, 0.2, 0.0
+    src = abi.Factory.from_buffer_copy(d.factories[1])
+    d.factories[1] = f
+    d.factories[2] = src
+    d.n_factories = 3
+    d.plasma, d.plasma_k, d.plasma_nu, d.plasma_mpa, d.plasma_wion = 1, 3, 0.05, 0.003, 0.5
+    g, r, _, _, _ = run_pair(d, 90, FIELDS + PLASMA_GRIDS)
+    assert np.abs(r["PlasmaDensity"]).max() > 1e-4
+    for n in g:
+        err = rel_l2(g[n], r[n])
+        print(n, err)
+        assert err < 1e-12, (n, err)

@@ -425,3 +425,78 @@ def test_focused_pulse_injects_one_way_and_focuses():
     assert peak[9:12].max() < 0.04
     assert 0.95 < peak.max() < 1.05
     assert abs(int(peak.argmax()) - 44) <= 4
+
+
+# ---------------------------------------------------------------------------------------
+# plasma: PlasmaDensity (src/plasmadensity.cpp:18-59) and PlasmaCurrent (src/plasmacurrent.cpp)
+# ---------------------------------------------------------------------------------------
+
+def uniform_box(dt, plasma=None, pcur=None, n=4):
+    """A periodic box with a uniform E_x and no magnetic field: the curl vanishes, so each
+    cell follows the 0-D equations of the plasma terms."""
+    d = abi.desc_defaults()
+    d.nx = d.ny = d.nz = n
+    d.dx = d.dy = d.dz = 0.5
+    d.dt = dt
+    d.solver = abi.SOLVER_PLRC_LIN
+    d.eps, d.chi = 1.0, 0.0
+    if plasma:
+        d.plasma = 1
+        d.plasma_k, d.plasma_nu, d.plasma_mpa, d.plasma_wion = plasma
+    if pcur:
+        f = abi.factory_defaults(abi.FACTORY_PLASMA_CURRENT)
+        f.em, f.gamma, f.rho_bg = pcur
+        abi.add_factory(d, f)
+    return d
+
+
+def run_uniform(d, T, E0):
+    o = ol.Oracle(d)
+    o.set("Ex", np.full((d.nx + 2, d.ny + 2, d.nz + 2), E0))
+    o.step(int(round(T / d.dt)))
+    return o
+
+
+def test_plasma_density_is_crank_nicolson_of_the_rate_equation():
+    """d rho/dt = nu I rho + mpa I^K at constant I (Wion = 0 keeps E constant): the density
+    converges to (mpa I^(K-1) / nu)(exp(nu I t) - 1) with second order in dt."""
+    E0, T, K, nu, mpa = 0.8, 4.0, 3, 0.5, 0.3
+    I = E0 * E0
+    exact = mpa * I ** (K - 1) / nu * (np.exp(nu * I * T) - 1)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        o = run_uniform(uniform_box(dt, plasma=(K, nu, mpa, 0.0)), T, E0)
+        assert o.get("Ex")[2, 2, 2] == E0
+        errs.append(abs(o.get("PlasmaDensity")[2, 2, 2] - exact))
+    assert errs[-1] < 1e-5
+    assert 3.9 < errs[0] / errs[1] < 4.1 and 3.9 < errs[1] / errs[2] < 4.1
+
+
+def test_plasma_resistivity_damps_the_field():
+    """K = 1 gives the constant resistivity Sigma = mpa Wion, so eps dE/dt = -Sigma E:
+    E -> E0 exp(-Sigma t / eps), second order in dt (the Sigma dt / 2 of both sides)."""
+    E0, T, mpa, wion = 0.8, 4.0, 0.3, 0.7
+    exact = E0 * np.exp(-mpa * wion * T)
+    errs = []
+    for dt in (0.1, 0.05, 0.025):
+        o = run_uniform(uniform_box(dt, plasma=(1, 0.0, mpa, wion)), T, E0)
+        errs.append(abs(o.get("Ex")[2, 2, 2] - exact))
+        assert np.allclose(o.get("Sigma")[1:-1, 1:-1, 1:-1], mpa * wion)
+    assert errs[-1] < 1e-5
+    assert 3.9 < errs[0] / errs[1] < 4.1 and 3.9 < errs[1] / errs[2] < 4.1
+
+
+def test_plasma_current_gives_the_damped_plasma_oscillation():
+    """With a background density and no ionisation, dJ/dt = -gamma J - em rho_bg E and
+    eps dE/dt = J: E(t) = E0 exp(-gamma t/2)(cos W t + gamma/(2W) sin W t), W^2 = em rho_bg -
+    gamma^2/4.  The current is a step ahead of the field in the update, so the error is of
+    first order in dt."""
+    E0, T, em, g, rb = 0.8, 4.0, 1.3, 0.4, 2.0
+    W = np.sqrt(em * rb - g * g / 4)
+    exact = E0 * np.exp(-g * T / 2) * (np.cos(W * T) + g / (2 * W) * np.sin(W * T))
+    errs = []
+    for dt in (0.02, 0.01, 0.005):
+        o = run_uniform(uniform_box(dt, pcur=(em, g, rb)), T, E0)
+        errs.append(abs(o.get("Ex")[2, 2, 2] - exact))
+    assert errs[-1] < 3e-4
+    assert 1.9 < errs[0] / errs[1] < 2.1 and 1.9 < errs[1] / errs[2] < 2.1
