@@ -201,7 +201,9 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
    E-side / B-side fused kernel (1, 2, 4); "sweep_seg" = x planes per block of the sweep kernel;
    "sweep_variant" = its thread-block shape (0: 64x8, 1: 64x4, 2: 64x16 threads; one
    tile row per thread, the last row and lane of a tile being its halo).  "tile_order" = block order
-   of the fused kernels (0: z tiles fastest, 1: x chunks fastest).  No counterpart in the reference: the hook of
+   of the fused kernels (0: z tiles fastest, 1: x chunks fastest, 2, the default: bands of
+   "tile_band" tile rows walked x plane by x plane; "tile_band" 0, the default, chooses the
+   height from the grid).  No counterpart in the reference: the hook of
    the tile autotuner. */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
 

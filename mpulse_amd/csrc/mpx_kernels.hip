@@ -298,6 +298,77 @@ template void launch_wrap<float>(const GridDims &, float *const[3], int, hipStre
 // FDTD_PLRCLinCore::plrcStepD    src/fdtd_plrc.cpp:94-261.
 // d* are the B differences in the reference's order, kd* the kappa*delta factors.
 // ------------------------------------------------------------------------------------
+// Every grid, table and layer pointer of the solver points into hipMalloc'd memory.  Kernels
+// that read their argument block from device memory see those pointers as generic ones, and
+// generic (flat) accesses also count against the scalar-load counter, so each wait for an
+// argument load would wait for the outstanding grid loads too.  gld / gst access through the
+// global address space instead.
+#define MPX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    return *(const MPX_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+    *(MPX_GLOBAL T *)p = v;
+}
+
+// Words a launch touches exactly once (the pole accumulators, a cell's own E in the E kernel
+// and B in the B kernel) move with nontemporal loads and stores: they then do not evict from L2
+// the neighbour planes that other threads of the launch re-read.
+typedef double double2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 load_once(const double2 *p)
+{
+    const double2v v = __builtin_nontemporal_load((const MPX_GLOBAL double2v *)p);
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void store_once(double2 *p, double2 v)
+{
+    double2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (MPX_GLOBAL double2v *)p);
+}
+template <typename T>
+__device__ __forceinline__ T load_once(const T *p)
+{
+    return __builtin_nontemporal_load((const MPX_GLOBAL T *)p);
+}
+template <typename T>
+__device__ __forceinline__ void store_once(T *p, T v)
+{
+    __builtin_nontemporal_store(v, (MPX_GLOBAL T *)p);
+}
+
+// The value a neighbouring lane of the wavefront holds, by DPP: lane_prev gives lane l the
+// value of lane l - 1 (wave_shr:1), lane_next that of lane l + 1 (wave_shl:1).  Lanes of a
+// fused-kernel wave hold consecutive z cells of one row, so the z neighbour of a cell is in
+// the next lane except at the ends of the wave and of the row, where it is loaded.
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // quotient x / k, or x * k when k already holds the reciprocal (RECIP)
 template <bool RECIP>
 __device__ __forceinline__ double qt(double x, double k)
@@ -327,9 +398,9 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
         Psiy += pyr;
         Psiz += pzr;
         if (store) {
-            psi[n][0][c] = make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr));
-            psi[n][1][c] = make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr));
-            psi[n][2][c] = make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr));
+            store_once(&psi[n][0][c], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
+            store_once(&psi[n][1][c], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
+            store_once(&psi[n][2][c], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
         }
     }
     if (NONLIN) {
@@ -383,7 +454,7 @@ template <bool NONLIN, typename A>
 __device__ __forceinline__ void cell_eps(const A &a, size_t c, double sigma, double &denom, double &numer)
 {
     if (a.epsf || a.plasma) {
-        const double e = a.epsf ? a.epsf[c] : a.eps;
+        const double e = a.epsf ? gld(a.epsf + c) : a.eps;
         denom = NONLIN ? e + a.sumChi0 - a.sumXi0 + sigma : e + (a.sumChi0 - a.sumXi0) + sigma;
         numer = e - a.sumXi0 - sigma;
     } else {
@@ -621,17 +692,20 @@ __device__ __forceinline__ void cpml_face(const CpmlFace &f, int coord, int l, d
 {
     int q = coord - f.lo[f.dim];
     T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
-    double bq = f.b[q], cq = f.c[q];
+    double bq = gld(f.b + q), cq = gld(f.c + q);
     T n0, n1;
+    // ordinary loads: a face row covers only part of a cache line (z faces: 16 cells), and the
+    // rest of the line is read by the neighbouring row
+    const double o0 = (double)gld(p0 + l), o1 = (double)gld(p1 + l);
     if (HSIDE) {
-        n0 = (T)(bq * (double)p0[l] + cq * d1 * f.idxn);
-        n1 = (T)(bq * (double)p1[l] - cq * d0 * f.idxn);
+        n0 = (T)(bq * o0 + cq * d1 * f.idxn);
+        n1 = (T)(bq * o1 - cq * d0 * f.idxn);
     } else {
-        n0 = (T)(bq * (double)p0[l] - cq * d1 * f.idxn);
-        n1 = (T)(bq * (double)p1[l] + cq * d0 * f.idxn);
+        n0 = (T)(bq * o0 - cq * d1 * f.idxn);
+        n1 = (T)(bq * o1 + cq * d0 * f.idxn);
     }
-    p0[l] = n0;
-    p1[l] = n1;
+    gst(p0 + l, n0);
+    gst(p1 + l, n1);
     v0 = (double)n0;
     v1 = (double)n1;
 }
@@ -650,9 +724,9 @@ __device__ __forceinline__ void add_sheets(const StepArgs<T> &a, int i, int j, i
     for (int s = 0; s < a.nsheets; ++s)
         if (k == a.sheet[s].kp) {
             int q = i * a.g.py + j;
-            x += a.sheet[s].J[0][q];
-            y += a.sheet[s].J[1][q];
-            z += a.sheet[s].J[2][q];
+            x += gld(a.sheet[s].J[0] + q);
+            y += gld(a.sheet[s].J[1] + q);
+            z += gld(a.sheet[s].J[2] + q);
         }
 }
 
@@ -669,6 +743,38 @@ __device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int
     t /= nbx;
     by = t % nby;
     bz = t / nby;
+}
+
+// Banded order: the tile rows are cut into bands of R rows; inside a band the z tiles run
+// fastest, then the rows of the band, then the x chunks, so the x-neighbour plane of a tile
+// was read a band-plane (nbx * R tiles) earlier and is still in the XCD's L2.  Each XCD
+// takes a contiguous share of the order as in xcd_tile.
+__device__ __forceinline__ void band_tile(int nbx, int nby, int nbz, int R, int &bx, int &by, int &bz)
+{
+    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    unsigned total = (unsigned)nbx * nby * nbz;
+    unsigned per = total / 8u;
+    unsigned xcd = lin & 7u, idx = lin >> 3;
+    unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
+    const unsigned full = (unsigned)nbx * R * nbz;  // tiles of a full band; only the last is shorter
+    const unsigned b = t / full;
+    unsigned rem = t - b * full;
+    const unsigned rows = min((unsigned)R, (unsigned)nby - b * R);
+    bx = rem % nbx;
+    rem /= nbx;
+    by = b * R + rem % rows;
+    bz = rem / rows;
+}
+
+__device__ __forceinline__ void fused_tile_of(int order, int band, int nbx, int nby, int nbz, int &bx, int &by,
+                                              int &bz)
+{
+    if (order == 2)
+        band_tile(nbx, nby, nbz, band, bx, by, bz);
+    else if (order == 1)
+        xcd_tile(nbz, nbx, nby, bz, bx, by);
+    else
+        xcd_tile(nbx, nby, nbz, bx, by, bz);
 }
 
 // Whether coordinate coord of dimension dim lies in the layer of the high-side ([0]) and
@@ -721,30 +827,42 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
     int c = i0 * sx + j * pz + k + g.z0;
     const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
     const int cim = (i0 == 1 && a.periodic[0]) ? c + (g.n[0] - 1) * sx : c - sx;
-    double bz_im = (double)Bz[cim], by_im = (double)By[cim];
+    double bz_im = (double)gld(Bz + cim), by_im = (double)gld(By + cim);
     bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
         in_faces(a.face_by_dim, a.face, 1, j, iny);
         in_faces(a.face_by_dim, a.face, 2, k, inz);
     }
-    const double kdy = a.ikdE[1][j], kdz = a.ikdE[2][k];
+    const double kdy = gld(a.ikdE[1] + j), kdz = gld(a.ikdE[2] + k);
 #pragma unroll
     for (int u = 0; u < L; ++u, c += sx) {
         const int i = i0 + u;
         if (i > i1) break;
-        const double bx = Bx[c], by = By[c], bz = Bz[c];
+        // z neighbours: the first lane of a wave and the first cell of a row load them, every
+        // other lane takes them from the previous lane (the loads are issued first, so that one
+        // wait covers them and the cell's own loads)
+        const bool zload = threadIdx.x == 0 || k == 1;
+        T bym = (T)0, bxm = (T)0;
+        if (zload) {
+            bym = gld(By + c + okm);
+            bxm = gld(Bx + c + okm);
+        }
+        const T bxt = gld(Bx + c), byt = gld(By + c);
+        const double bx = bxt, by = byt, bz = gld(Bz + c);
         // E and the pole accumulators are read before any store of the cell (the CPML
         // auxiliaries), so that all its reads leave together
-        double ex = (double)a.E[0][c], ey = (double)a.E[1][c], ez = (double)a.E[2][c];
+        double ex = (double)load_once(&a.E[0][c]), ey = (double)load_once(&a.E[1][c]);
+        double ez = (double)load_once(&a.E[2][c]);
         PsiRegs<NP> pr;
 #pragma unroll
         for (int n = 0; n < NP; ++n)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) pr.v[n][q] = a.psi[n][q][c];
-        const double dBz_y = bz - (double)Bz[c + ojm];
-        const double dBx_y = bx - (double)Bx[c + ojm];
-        const double dBy_z = by - (double)By[c + okm];
-        const double dBx_z = bx - (double)Bx[c + okm];
+            for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][c]);
+        const double dBz_y = bz - (double)gld(Bz + c + ojm);
+        const double dBx_y = bx - (double)gld(Bx + c + ojm);
+        const T bys = lane_prev(byt), bxs = lane_prev(bxt);
+        const double dBy_z = by - (double)(zload ? bym : bys);
+        const double dBx_z = bx - (double)(zload ? bxm : bxs);
         const double dBz_x = bz - bz_im;
         const double dBy_x = by - by_im;
         bz_im = bz;
@@ -752,17 +870,17 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
         // the plasma OptField and the plasma current run before the E update, on E of this step
         double sg = 0.0, jpv[3] = {0.0, 0.0, 0.0};
         if (a.plasma || a.pcur) {
-            double r = a.rho[c];
+            double r = gld(a.rho + c);
             if (a.plasma) {
                 sg = a.pw * plasma_density_cell(ex, ey, ez, r, a.pn, a.pm, a.pk - 1);
-                a.rho[c] = r;
-                a.sigma[c] = sg;
+                gst(a.rho + c, r);
+                gst(a.sigma + c, sg);
             }
             if (a.pcur) {
                 const double rt = r + a.rho_bg, ev[3] = {ex, ey, ez};
                 for (int q = 0; q < 3; ++q) {
-                    jpv[q] = plasma_current_comp(a.jp[q][c], ev[q], rt, a.gdtn, a.gdtd, a.emdt);
-                    a.jp[q][c] = jpv[q];
+                    jpv[q] = plasma_current_comp(gld(a.jp[q] + c), ev[q], rt, a.gdtn, a.gdtd, a.emdt);
+                    gst(a.jp[q] + c, jpv[q]);
                 }
             }
         }
@@ -827,12 +945,12 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
             cell_eps<NONLIN>(a, (size_t)c, 0.5 * sg * a.dt, denom, numer);
             idenom = 1.0 / denom;
         }
-        plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.ikdE[0][i], kdy,
+        plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i), kdy,
                                            kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom, numer,
                                            a.chi);
-        a.E[0][c] = (T)ex;
-        a.E[1][c] = (T)ey;
-        a.E[2][c] = (T)ez;
+        store_once(&a.E[0][c], (T)ex);
+        store_once(&a.E[1][c], (T)ey);
+        store_once(&a.E[2][c], (T)ez);
     }
 }
 
@@ -853,13 +971,26 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
     int c = i0 * sx + j * pz + k + g.z0;
     const T *Ex = a.E[0], *Ey = a.E[1], *Ez = a.E[2];
     T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
-    double ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    // z neighbours: the last lane of a wave and the last cell of a row load them, every other
+    // lane takes them from the next lane
+    const bool zload = threadIdx.x == 63 || k == g.n[2];
+    T ezl[L][2];
+    if (zload) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            const int cu = c + u * sx;
+            ezl[u][0] = i0 + u <= i1 ? gld(Ey + cu + okp) : (T)0;
+            ezl[u][1] = i0 + u <= i1 ? gld(Ex + cu + okp) : (T)0;
+        }
+    }
+    const T ex0 = gld(Ex + c), ey0 = gld(Ey + c), ez0 = gld(Ez + c);
+    double ex = ex0, ey = ey0, ez = ez0;
     bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
         in_faces(a.face_by_dim, a.face, 1, j, iny);
         in_faces(a.face_by_dim, a.face, 2, k, inz);
     }
-    const double kdy = a.ikdH[1][j], kdz = a.ikdH[2][k];
+    const double kdy = gld(a.ikdH[1] + j), kdz = gld(a.ikdH[2] + k);
     // the E and B values of all L planes are read before the first store, so that they leave
     // together (per plane: E at x+1, E_y and E_x at z+1, E_z and E_x at y+1, B)
     T pe[L][7], pb[L][3];
@@ -869,16 +1000,22 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
         if (i > i1) break;
         const int cu = c + u * sx;
         const int cip = (i == g.n[0] && a.periodic[0]) ? cu - (g.n[0] - 1) * sx : cu + sx;
-        pe[u][0] = Ex[cip];
-        pe[u][1] = Ey[cip];
-        pe[u][2] = Ez[cip];
-        pe[u][3] = Ey[cu + okp];
-        pe[u][4] = Ex[cu + okp];
-        pe[u][5] = Ez[cu + ojp];
-        pe[u][6] = Ex[cu + ojp];
-        pb[u][0] = Bx[cu];
-        pb[u][1] = By[cu];
-        pb[u][2] = Bz[cu];
+        pe[u][0] = gld(Ex + cip);
+        pe[u][1] = gld(Ey + cip);
+        pe[u][2] = gld(Ez + cip);
+        pe[u][5] = gld(Ez + cu + ojp);
+        pe[u][6] = gld(Ex + cu + ojp);
+        pb[u][0] = load_once(&Bx[cu]);
+        pb[u][1] = load_once(&By[cu]);
+        pb[u][2] = load_once(&Bz[cu]);
+    }
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+        if (i0 + u > i1) break;
+        // E at z+1: the next lane's centre value of the same plane
+        const T sy = lane_next(u == 0 ? ey0 : pe[u - 1][1]), sx_ = lane_next(u == 0 ? ex0 : pe[u - 1][0]);
+        pe[u][3] = zload ? ezl[u][0] : sy;
+        pe[u][4] = zload ? ezl[u][1] : sx_;
     }
 #pragma unroll
     for (int u = 0; u < L; ++u, c += sx) {
@@ -925,10 +1062,10 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
                 }
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
-        const double kdx = a.ikdH[0][i];
-        Bx[c] = (T)((double)pb[u][0] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx));
-        By[c] = (T)((double)pb[u][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my));
-        Bz[c] = (T)((double)pb[u][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz));
+        const double kdx = gld(a.ikdH[0] + i);
+        store_once(&Bx[c], (T)((double)pb[u][0] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx)));
+        store_once(&By[c], (T)((double)pb[u][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my)));
+        store_once(&Bz[c], (T)((double)pb[u][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz)));
         ex = ex1;
         ey = ey1;
         ez = ez1;
@@ -965,10 +1102,7 @@ template <typename T, int NP, bool NONLIN, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_E(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    if (pa->order)
-        xcd_tile(nbz, nbx, nby, tbz, tbx, tby);
-    else
-        xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    fused_tile_of(pa->order, pa->band, nbx, nby, nbz, tbx, tby, tbz);
     fused_E_tile<T, NP, NONLIN, CPML, L>(*pa, tbx, tby, tbz);
 }
 
@@ -976,10 +1110,7 @@ template <typename T, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_B(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    if (pa->order)
-        xcd_tile(nbz, nbx, nby, tbz, tbx, tby);
-    else
-        xcd_tile(nbx, nby, nbz, tbx, tby, tbz);
+    fused_tile_of(pa->order, pa->band, nbx, nby, nbz, tbx, tby, tbz);
     fused_B_tile<T, CPML, L>(*pa, tbx, tby, tbz);
 }
 
@@ -1458,7 +1589,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
 #pragma unroll
             for (int n = 0; n < NP; ++n)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = a.psio[n][q][c];
+                for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = load_once(&a.psio[n][q][c]);
         }
     };
 #pragma unroll

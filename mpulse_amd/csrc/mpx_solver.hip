@@ -145,8 +145,12 @@ struct mpx_solver {
 
     double *d_part = nullptr, *d_energy = nullptr;
     int time = 0;
-    int march_e = 1, march_b = 2;  // x planes per thread in the fused kernels (1, 2 or 4)
-    int tile_order = 0;            // fused kernels: block order, 0 = z tiles fastest, 1 = x chunks fastest
+    // x planes per thread in the fused kernels (1, 2 or 4); with the pole accumulators and the
+    // cell's own field streamed nontemporally, E: 1 and B: 2 are fastest in both precisions
+    int march_e = 1, march_b = 2;
+    int tile_order = 2;            // fused kernels: block order, 0 = z tiles fastest, 1 = x chunks fastest,
+                                   // 2 = bands of tile_band rows walked plane by plane (fastest)
+    int tile_band = 0;             // 0: chosen from the grid by auto_band
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
     double2 *psi2[3][3] = {};
@@ -577,7 +581,6 @@ static int build(mpx_solver *s, const mpx_desc *desc)
     s->plrc = (s->solver == MPX_SOLVER_PLRC_NONLIN || s->solver == MPX_SOLVER_PLRC_LIN);
     s->nonlin = (s->solver == MPX_SOLVER_PLRC_NONLIN);
     s->esz = (s->prec == MPX_PREC_FP32_BULK) ? 4 : 8;
-    if (s->esz == 8) s->march_e = 4;  // float64 E side: four x planes per thread measured fastest
     s->delta[0] = desc->dx;
     s->delta[1] = desc->dy;
     s->delta[2] = desc->dz;
@@ -1000,6 +1003,20 @@ static int build(mpx_solver *s, const mpx_desc *desc)
 // stepping
 // ---------------------------------------------------------------------------------------
 
+// Band height of the banded block order: the largest power of two that keeps a band-plane
+// near 512 tiles (E side) or 256 tiles (B side) of 64 z cells; on 512^3 that is 64 and 32
+// rows, the fastest of the heights measured (power-of-two bands keep the XCD shares of the
+// block order aligned to whole bands).
+static int auto_band(const mpx::GridDims &g, const int *blo, bool hside)
+{
+    const int lo1 = blo ? blo[1] : 1, lo2 = blo ? blo[2] : 1;
+    const int nzt = std::max(1, (g.n[2] - lo2 + 1) / 64), nby = (g.n[1] - lo1 + 1 + 3) / 4;
+    const int target = std::max(1, (hside ? 256 : 512) / nzt);
+    int band = 1;
+    while (band * 2 <= target && band * 2 <= nby) band *= 2;
+    return band;
+}
+
 template <typename T>
 static StepArgs<T> step_args(mpx_solver *s, bool hside)
 {
@@ -1062,6 +1079,7 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.haveJ = 0;
     a.march = hside ? s->march_b : s->march_e;
     a.order = s->tile_order;
+    a.band = s->tile_band > 0 ? s->tile_band : auto_band(s->g, hside ? a.blo : nullptr, hside);
     return a;
 }
 
@@ -1907,8 +1925,11 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
     else if (!std::strcmp(key, "march_b"))
         s->march_b = value;
     else if (!std::strcmp(key, "tile_order")) {
-        if (value != 0 && value != 1) return fail(MPX_ERR_INVALID, "tile_order must be 0 or 1");
+        if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "tile_order must be 0, 1 or 2");
         s->tile_order = value;
+    } else if (!std::strcmp(key, "tile_band")) {
+        if (value < 0) return fail(MPX_ERR_INVALID, "tile_band must be >= 0 (0: chosen from the grid)");
+        s->tile_band = value;
     } else if (!std::strcmp(key, "sweep_seg")) {
         if (value < 1) return fail(MPX_ERR_INVALID, "sweep_seg must be >= 1");
         s->sweep_seg = value < s->g.n[0] ? value : s->g.n[0];

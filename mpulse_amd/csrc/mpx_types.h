@@ -127,7 +127,9 @@ struct StepArgs {
     int haveJ;
     int blo[3];                 // first updated B index per dimension (fused path)
     int march;                  // fused path: x planes walked by one thread
-    int order;                  // fused path: 0 = z tiles fastest in the block order, 1 = x chunks fastest
+    int order;                  // fused path: 0 = z tiles fastest in the block order, 1 = x chunks fastest,
+                                // 2 = bands of `band` tile rows, walked plane by plane
+    int band;
 };
 
 // Single-pass sweep path: "o" = buffers of step n (read), "n" = buffers of step n+1 (written).
