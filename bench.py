@@ -8,9 +8,12 @@ A step is one full Yee step (E side then B side) of the whole grid; value = cell
 float64 reference-structured path are timed beside it, and the CPU oracle (OpenMP) is timed
 on a bounded sample of the same workload.  The headline form is the faster of the two
 FP32-bulk kernel structures (two fused kernels per step, or the single-pass sweep).  The
-64-run batched parameter sweep is measured beside it (see batch_sweep).
+64-run batched parameter sweep (see batch_sweep), the adjoint-gradient workload (see
+adjoint_gradient) and the FP32-bulk field error against float64 after 1000 steps (see
+accuracy_fp32_bulk) are measured beside it.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64] [--no-batch]
+                [--no-adjoint] [--no-accuracy]
 """
 import argparse
 import json
@@ -84,30 +87,16 @@ def cpu_baseline(n=512, n_sample_steps=3):
 
 def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     """Uneven-size parameter sweep: nruns independent cubes (48^3..160^3, one pole + Kerr,
-    10-cell CPML, 500 steps each) stepped together by the batched persistent sweep replayed
-    from captured graphs with autotuned tiles, against the same runs launched one after
-    another on the float64 plain (reference-structured) path and on the float64 fused path.
-    Times cover the steps only (set-up excluded), wall clock around each set."""
+    10-cell CPML, 500 steps each) stepped together by the batched persistent kernels replayed
+    from captured graphs with autotuned tiles, in float64 and in FP32-bulk, against the same
+    runs launched one after another on the float64 plain (reference-structured) path and on
+    the fused path of the same precision.  Times cover the steps only (set-up excluded), wall
+    clock around each set."""
     import torch
 
     from mpulse_amd import abi, configs
-    runs = configs.config_sweep(nruns=nruns, steps=steps)
-    descs = [d for d, _ in runs]
-    cells = sum(d.nx * d.ny * d.nz for d in descs)
-    out = {"runs": nruns, "steps": steps, "cells_per_step": cells, "precision": "fp64",
-           "sizes": [d.nx for d in descs]}
-    b = abi.Batch(descs)
-    tune = b.autotune(autotune_steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b.step(steps)
-    b.sync()
-    t_batch = time.perf_counter() - t0
-    b.close()
-    torch.cuda.empty_cache()
-    out["batched"] = {"s": t_batch, "cell_updates_per_s": cells * steps / t_batch, "tuning": tune}
-    print(f"# batch: {t_batch:.3f} s for {nruns} runs x {steps} steps, tuning {tune}", file=sys.stderr, flush=True)
-    for name, path in (("sequential_plain", abi.PATH_REFERENCE), ("sequential_fused", abi.PATH_FUSED)):
+
+    def sequential(descs, path):
         total = 0.0
         for d in descs:
             d.path = path
@@ -119,10 +108,66 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
             total += time.perf_counter() - t0
             s.close()
         torch.cuda.empty_cache()
-        out[name] = {"s": total, "cell_updates_per_s": cells * steps / total}
-        out[f"speedup_vs_{name}"] = total / t_batch
-        print(f"# {name}: {total:.3f} s, batched speed-up {total / t_batch:.2f}x", file=sys.stderr, flush=True)
+        return total
+
+    def batched(descs):
+        b = abi.Batch(descs)
+        tune = b.autotune(autotune_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        t = time.perf_counter() - t0
+        b.close()
+        torch.cuda.empty_cache()
+        return t, tune
+
+    descs64 = [d for d, _ in configs.config_sweep(nruns=nruns, steps=steps, precision=abi.PREC_FP64)]
+    cells = sum(d.nx * d.ny * d.nz for d in descs64)
+    out = {"runs": nruns, "steps": steps, "cells_per_step": cells, "sizes": [d.nx for d in descs64]}
+    t_plain = sequential(descs64, abi.PATH_REFERENCE)
+    out["sequential_plain"] = {"precision": "fp64", "s": t_plain, "cell_updates_per_s": cells * steps / t_plain}
+    print(f"# sequential_plain (float64): {t_plain:.3f} s", file=sys.stderr, flush=True)
+    for name, prec in (("fp64", abi.PREC_FP64), ("fp32_bulk", abi.PREC_FP32_BULK)):
+        descs = [d for d, _ in configs.config_sweep(nruns=nruns, steps=steps, precision=prec)]
+        t_b, tune = batched(descs)
+        t_f = sequential(descs, abi.PATH_FUSED)
+        out[name] = {"batched": {"s": t_b, "cell_updates_per_s": cells * steps / t_b, "tuning": tune},
+                     "sequential_fused": {"s": t_f, "cell_updates_per_s": cells * steps / t_f},
+                     "speedup_vs_sequential_plain": t_plain / t_b,
+                     "speedup_vs_sequential_fused": t_f / t_b}
+        print(f"# batch {name}: {t_b:.3f} s for {nruns} runs x {steps} steps, tuning {tune}; "
+              f"{t_plain / t_b:.2f}x sequential float64 plain, {t_f / t_b:.2f}x sequential {name} fused "
+              f"({t_f:.3f} s)", file=sys.stderr, flush=True)
     return out
+
+
+def accuracy_fp32_bulk(n=512, steps=1000):
+    """Relative L2 error of the FP32-bulk fields against the float64 fields after `steps` steps
+    of the same workload (both on the fused path): E and B as vector fields and per component."""
+    import numpy as np
+    import torch
+
+    from mpulse_amd import abi, configs
+    d64, _ = configs.config_roofline(precision=abi.PREC_FP64, n=n)
+    d32, _ = configs.config_roofline(precision=abi.PREC_FP32_BULK, n=n)
+    out, num, den = {}, {"E": 0.0, "B": 0.0}, {"E": 0.0, "B": 0.0}
+    with abi.Solver(d64) as a, abi.Solver(d32) as b:
+        a.step(steps)
+        b.step(steps)
+        for f in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+            x, y = b.get(f), a.get(f)
+            dd, yy = float(np.sum((x - y) ** 2)), float(np.sum(y ** 2))
+            out[f] = (dd / yy) ** 0.5
+            num[f[0]] += dd
+            den[f[0]] += yy
+            del x, y
+    torch.cuda.empty_cache()
+    for g in ("E", "B"):
+        out[g] = (num[g] / den[g]) ** 0.5
+    print(f"# fp32-bulk vs float64 after {steps} steps at {n}^3: rel-L2 E {out['E']:.2e}, B {out['B']:.2e}",
+          file=sys.stderr, flush=True)
+    return {"grid": [n, n, n], "steps": steps, "rel_l2": out, "bar": 1e-5}
 
 
 def adjoint_gradient(fd=True):
@@ -212,6 +257,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the 64-run batched sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint-gradient workload")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the 1000-step FP32-bulk against float64 field comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -294,6 +341,8 @@ def main():
     for ref in ("fp64_reference", "fp64_fused", "fp64_sweep"):
         if ref in results:
             out[f"fp32_bulk_over_{ref}"] = main_res["cell_updates_per_s"] / results[ref]["cell_updates_per_s"]
+    if not args.no_accuracy:
+        out["accuracy_fp32_bulk_vs_fp64"] = accuracy_fp32_bulk(n)
     if not args.no_batch:
         out["batch_sweep"] = batch_sweep()
     if not args.no_adjoint:

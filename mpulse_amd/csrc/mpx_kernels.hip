@@ -1076,7 +1076,9 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
 // fastest, then y, then x), the 8 XCDs take contiguous eighths of the list so that tiles
 // sharing halo cells meet in one L2, and each block strides its XCD's eighth by
 // gridDim.x / 8 (gridDim.x is a multiple of 8; the hardware deals blocks to the XCDs round
-// robin).  f(run, tz, ty, tx) processes one tile.
+// robin).  Each XCD starts its eighth at a different point (XCD x at x/8 of it, wrapping
+// round), so that the XCDs do not walk equal runs at equal offsets in lock step; that
+// measured 2-5 % faster on batches of equal-size runs.  f(run, tz, ty, tx) processes one tile.
 template <typename F>
 __device__ __forceinline__ void batch_tiles(const BatchRun *__restrict__ runs, int nruns, int total, F &&f)
 {
@@ -1084,8 +1086,14 @@ __device__ __forceinline__ void batch_tiles(const BatchRun *__restrict__ runs, i
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int chunk = (total + 7) >> 3;
     const int t0 = xcd * chunk, t1 = min(total, t0 + chunk);
+    const int len = max(0, t1 - t0);
+    const int rot = (int)(((long long)len * xcd) >> 3);
     int r = 0;
-    for (int t = t0 + slot; t < t1; t += per_xcd) {
+    for (int u = slot; u < len; u += per_xcd) {
+        int v = u + rot;
+        if (v >= len) v -= len;
+        const int t = t0 + v;
+        if (t < runs[r].first) r = 0;  // wrapped round to the start of the eighth
         while (r + 1 < nruns && t >= runs[r + 1].first) ++r;
         const int ntz = runs[r].ntz, nty = runs[r].nty;
         int local = t - runs[r].first;
