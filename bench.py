@@ -9,11 +9,11 @@ float64 reference-structured path are timed beside it, and the CPU oracle (OpenM
 on a bounded sample of the same workload.  The headline form is the faster of the two
 FP32-bulk kernel structures (two fused kernels per step, or the single-pass sweep).  The
 64-run batched parameter sweep (see batch_sweep), the adjoint-gradient workload (see
-adjoint_gradient) and the FP32-bulk field error against float64 after 1000 steps (see
-accuracy_fp32_bulk) are measured beside it.
+adjoint_gradient), the focused-pulse workload (see focused_pulse) and the FP32-bulk field
+error against float64 after 1000 steps (see accuracy_fp32_bulk) are measured beside it.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64] [--no-batch]
-                [--no-adjoint] [--no-accuracy]
+                [--no-adjoint] [--no-accuracy] [--no-focused]
 """
 import argparse
 import json
@@ -142,6 +142,38 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     return out
 
 
+def focused_pulse():
+    """The focused-pulse self-focusing workload: 384^2 x 768, two Lorentz poles, Kerr from n2,
+    focused pulse (NA 0.1, 30 fs, 5e16 W/m^2), 16-cell CPML, FP32-bulk on the fused path, its
+    full 2000 steps timed after 10 untimed ones; peak |E| on the focal plane at the end."""
+    import numpy as np
+    import torch
+
+    from mpulse_amd import abi, configs
+    d, meta = configs.config_focus(precision=abi.PREC_FP32_BULK)
+    steps = meta["steps"]
+    cells = d.nx * d.ny * d.nz
+    with abi.Solver(d) as s:
+        s.step(10)
+        s.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t = time.perf_counter() - t0
+        prof = s.time_steps(20)
+        kz = d.nz // 2
+        ex = s.get("Ex")[:, :, kz]
+    torch.cuda.empty_cache()
+    out = {"grid": [d.nx, d.ny, d.nz], "poles": 2, "steps": steps, "s": t, "ms_per_step": t / steps * 1e3,
+           "cell_updates_per_s": cells * steps / t, "kernels_ms": {k: v[0] for k, v in prof["kernels"].items()},
+           "peak_abs_Ex_focal_plane": float(np.max(np.abs(ex)))}
+    print(f"# focused pulse {d.nx}^2 x {d.nz}, 2 poles, FP32-bulk: {t:.2f} s for {steps} steps, "
+          f"{out['ms_per_step']:.3f} ms/step, {out['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def accuracy_fp32_bulk(n=512, steps=1000):
     """Relative L2 error of the FP32-bulk fields against the float64 fields after `steps` steps
     of the same workload (both on the fused path): E and B as vector fields and per component."""
@@ -257,6 +289,7 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the 64-run batched sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint-gradient workload")
+    ap.add_argument("--no-focused", action="store_true", help="skip the focused-pulse workload")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the 1000-step FP32-bulk against float64 field comparison")
     args = ap.parse_args()
@@ -343,6 +376,8 @@ def main():
             out[f"fp32_bulk_over_{ref}"] = main_res["cell_updates_per_s"] / results[ref]["cell_updates_per_s"]
     if not args.no_accuracy:
         out["accuracy_fp32_bulk_vs_fp64"] = accuracy_fp32_bulk(n)
+    if not args.no_focused:
+        out["focused_pulse"] = focused_pulse()
     if not args.no_batch:
         out["batch_sweep"] = batch_sweep()
     if not args.no_adjoint:
