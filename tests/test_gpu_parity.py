@@ -166,3 +166,36 @@ def test_plasma_against_oracle(path):
         err = rel_l2(g[n], r[n])
         print(n, err)
         assert err < 1e-12, (n, err)
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK])
+def test_fused_block_orders_identical(prec):
+    """Every block order and band height of the fused kernels, and every x-march, gives the
+    same fields bit for bit (each cell is computed by one thread with the same arithmetic);
+    120^2 x 70 has 30 (E) / 31 (B) tile rows, so the automatic band height (16) leaves a short
+    last band, and a partial z tile (70 = 64 + 6).  float64 is also held against the oracle."""
+    d, _ = configs.plrc_kerr_pulse(120, 8, 10.0, 10, 1e9, 30, d_src=10, nz=70, precision=prec)
+    d.factories[1].tshift = 12.0
+    shapes = [dict(tile_order=0), dict(tile_order=1), dict(tile_order=2, tile_band=0),
+              dict(tile_order=2, tile_band=3), dict(tile_order=2, tile_band=7),
+              dict(tile_order=2, tile_band=1000), dict(tile_order=2, tile_band=0, march_e=2, march_b=4)]
+    ref = None
+    for sh in shapes:
+        print(f"case: prec {prec} launch shape {sh}", flush=True)
+        with abi.Solver(d) as s:
+            for k, v in sh.items():
+                s.tune(k, v)
+            s.step(30)
+            got = {n: s.get(n) for n in FIELDS}
+        if ref is None:
+            ref = got
+            continue
+        for n in FIELDS:
+            assert np.array_equal(got[n], ref[n]), (sh, n, float(np.max(np.abs(got[n] - ref[n]))))
+    if prec == abi.PREC_FP64:
+        o = Oracle(d)
+        o.step(30)
+        for n in FIELDS:
+            err = rel_l2(ref[n], o.get(n))
+            assert err < 1e-12, (n, err)
+        o.close()
