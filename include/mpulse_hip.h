@@ -177,6 +177,24 @@ int  mpx_set_time(mpx_solver *s, int time);
    Buffer: (Nx+2)*(Ny+2)*(Nz+2) float64, z fastest. */
 int  mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count);
 int  mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count);
+/* Field output.  FieldDiag hands its HDF5 stream the grid as Storage holds it: the storage box
+   0..N+1 in every dimension, x fastest (src/fielddiag.cpp:17-24, src/hdfstream.t:88-187).
+   mpx_fetch_field queues a copy of grid `name` into `host` in that order, as float64, behind
+   every step queued before it, and returns at once with a handle: a device kernel packs the
+   grid (strips the padding, transposes to x fastest, widens float32) into a staging buffer,
+   then the copy runs on the solver's stream, so steps queued afterwards run on while the copy
+   lands and the caller writes its file.  mpx_fetch_wait -- callable from any host thread --
+   returns when the data is in `host` and releases the handle; `host` stays untouched by the
+   caller until then.  Page-locked memory from mpx_host_alloc lets the copy overlap the
+   steps.  Grids assembled on the host (PLRC accumulator parts, plasma grids, an unset Eps)
+   are read synchronously.  Replaces the synchronous grid read of FieldDiag::write; the
+   asynchronous handle has no counterpart in the reference. */
+typedef struct mpx_fetch mpx_fetch;
+int  mpx_fetch_field(mpx_solver *s, const char *name, double *host, size_t count, mpx_fetch **handle);
+int  mpx_fetch_wait(mpx_fetch *handle);
+int  mpx_host_alloc(size_t bytes, void **ptr);
+int  mpx_host_free(void *ptr);
+
 /* CPML border layers by reference name and Direction (0 north,1 south,2 west,3 east,4 up,5 down;
    src/mpulse.h:29): "CPMLPsiEx".."CPMLPsiBz".  Buffer is the layer's inclusive box
    lo..hi (returned through lo/hi), z fastest. */

@@ -141,6 +141,10 @@ def load():
         "mpx_set_time": (C.c_int, [P, C.c_int]),
         "mpx_get_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]),
         "mpx_set_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]),
+        "mpx_fetch_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t, C.POINTER(P)]),
+        "mpx_fetch_wait": (C.c_int, [P]),
+        "mpx_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(P)]),
+        "mpx_host_free": (C.c_int, [P]),
         "mpx_get_layer": (C.c_int, [P, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_size_t,
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "mpx_set_probes": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
@@ -180,6 +184,45 @@ def load():
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class FetchHandle:
+    """An mpx_fetch_field copy in flight; wait() returns once the data has landed."""
+
+    def __init__(self, solver, h):
+        self.solver, self.h = solver, h
+
+    def wait(self):
+        if self.h is not None:
+            rc = self.solver.lib.mpx_fetch_wait(self.h)
+            self.h = None
+            self.solver._check(rc)
+
+
+class PinnedArray:
+    """A float64 array in page-locked host memory (mpx_host_alloc), freed by close()."""
+
+    def __init__(self, shape):
+        self.lib = load()
+        n = int(np.prod(shape))
+        p = C.c_void_p()
+        rc = self.lib.mpx_host_alloc(max(1, n) * 8, C.byref(p))
+        if rc != OK:
+            raise MpxError(rc, self.lib.mpx_last_error().decode())
+        self.p = p
+        self.array = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n,)).reshape(shape)
+
+    def close(self):
+        if self.p is not None:
+            self.array = None
+            self.lib.mpx_host_free(self.p)
+            self.p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Solver:
@@ -242,6 +285,22 @@ class Solver:
     def get(self, name):
         out = np.empty(self.shape, dtype=np.float64)
         self._check(self.lib.mpx_get_field(self.h, name.encode(), _dptr(out), out.size))
+        return out
+
+    def fetch_async(self, name, out):
+        """Queue a copy of grid `name` into `out` (float64, shape (nz+2, ny+2, nx+2): the
+        reference's Grid order, x fastest) behind the steps queued so far; returns a handle
+        whose wait() returns once the data is in `out`."""
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.shape != self.shape[::-1]:
+            raise ValueError(f"expected a C-contiguous float64 array of shape {self.shape[::-1]}")
+        h = C.c_void_p()
+        self._check(self.lib.mpx_fetch_field(self.h, name.encode(), _dptr(out), out.size, C.byref(h)))
+        return FetchHandle(self, h)
+
+    def fetch(self, name):
+        """Grid `name` in the reference's Grid order (x fastest), shape (nz+2, ny+2, nx+2)."""
+        out = np.empty(self.shape[::-1], dtype=np.float64)
+        self.fetch_async(name, out).wait()
         return out
 
     def set(self, name, arr):
@@ -420,8 +479,61 @@ def load_host():
     h.mph_destroy.argtypes = [P]
     h.mph_last_error.restype = C.c_char_p
     h.mph_last_error.argtypes = []
+    for name, args in (("mph_h5_open", [C.c_char_p, C.c_int, C.POINTER(P)]),
+                       ("mph_h5_set_block_name", [P, C.c_char_p]),
+                       ("mph_h5_set_mtime", [P, C.c_uint]),
+                       ("mph_h5_write", [P, C.c_int, C.POINTER(C.c_ulonglong), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int), C.c_int]),
+                       ("mph_h5_close", [P])):
+        fn = getattr(h, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
     _host = h
     return h
+
+
+class HDFostream:
+    """MPulse's HDFostream (src/hdfstream.{h,t,cpp}) through the host library: float64
+    datasets named data0, data1, ... (or block name, name1, ...), written in the reference's
+    on-disk form without the HDF5 library."""
+
+    def __init__(self, path, capacity=64):
+        self.h = load_host()
+        p = C.c_void_p()
+        rc = self.h.mph_h5_open(os.fsencode(path), capacity, C.byref(p))
+        if rc != OK:
+            raise MpxError(rc, self.h.mph_last_error().decode())
+        self.p = p
+
+    def _check(self, rc):
+        if rc != OK:
+            raise MpxError(rc, self.h.mph_last_error().decode())
+
+    def set_block_name(self, name):
+        self._check(self.h.mph_h5_set_block_name(self.p, name.encode()))
+
+    def set_mtime(self, t):
+        self._check(self.h.mph_h5_set_mtime(self.p, int(t)))
+
+    def write(self, array, extent=None):
+        """One dataset: the HDF5 dims are array.shape, the data array in C order."""
+        a = np.ascontiguousarray(array, dtype=np.float64)
+        dims = (C.c_ulonglong * a.ndim)(*a.shape)
+        ext = None if extent is None else np.ascontiguousarray(extent, dtype=np.int32)
+        self._check(self.h.mph_h5_write(self.p, a.ndim, dims, _dptr(a),
+                                        None if ext is None else ext.ctypes.data_as(C.POINTER(C.c_int)),
+                                        0 if ext is None else ext.size))
+
+    def close(self):
+        if self.p is not None:
+            self._check(self.h.mph_h5_close(self.p))
+            self.p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def parse_setup(text):

@@ -45,14 +45,15 @@ def build(verbose=False):
 HOST_LIB = os.path.join(HERE, "libmpulse_host.so")
 HOST_EXE = os.path.join(HERE, "mpulse_hip_run")
 CXX = os.environ.get("CXX", "g++")
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include")]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", "-I", os.path.join(HERE, "..", "include")]
 LINK = ["-L", HERE, "-lmpulse_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
 
 
 def build_host(verbose=False):
     """The C++ host layer (setup front-end, FieldSolver plugins, driver) and its CLI."""
     host = os.path.join(HERE, "host")
-    cmd = [CXX, *HOST_FLAGS, "-shared", os.path.join(host, "mpulse_host.cpp"), "-o", HOST_LIB + ".tmp", *LINK]
+    cmd = [CXX, *HOST_FLAGS, "-shared", os.path.join(host, "mpulse_host.cpp"), os.path.join(host, "mpx_hdf5.cpp"),
+           "-o", HOST_LIB + ".tmp", *LINK]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

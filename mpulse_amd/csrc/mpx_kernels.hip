@@ -1495,6 +1495,36 @@ void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
                        (double2 *)dst, n);
 }
 
+// A grid over the storage box 0..N+1 in the order of the reference's Grid (x fastest), as
+// float64: dst[(k (ny+2) + j) (nx+2) + i] = src(i, j, k), the form FieldDiag hands to its HDF5
+// stream.  A block transposes a 32 (z) x 32 (x) tile of one y row through LDS, so that both
+// the reads (z contiguous) and the writes (x contiguous) are coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_xfast(GridDims g, const T *__restrict__ src, double *__restrict__ dst)
+{
+    __shared__ double tile[32][33];
+    const int nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+    const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32, j = blockIdx.z;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int i = i0 + r, k = k0 + threadIdx.x;
+        if (i < nx2 && k < nz2) tile[r][threadIdx.x] = (double)src[g.at(i, j, k)];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        const int k = k0 + r, i = i0 + threadIdx.x;
+        if (i < nx2 && k < nz2) dst[((size_t)k * ny2 + j) * nx2 + i] = tile[threadIdx.x][r];
+    }
+}
+
+template <typename T>
+void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s)
+{
+    dim3 grd((g.n[2] + 2 + 31) / 32, (g.n[0] + 2 + 31) / 32, g.n[1] + 2);
+    hipLaunchKernelGGL(k_pack_xfast<T>, grd, dim3(32, 8), 0, s, g, src, dst);
+}
+template void launch_pack_xfast<float>(const GridDims &, const float *, double *, hipStream_t);
+template void launch_pack_xfast<double>(const GridDims &, const double *, double *, hipStream_t);
+
 }  // namespace mpx
 
 namespace mpx {

@@ -81,6 +81,8 @@ void launch_probe(const ProbeArgs<T> &a, double *row, hipStream_t s);
 size_t energy_partials(const GridDims &g);
 void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 template <typename T>
+void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s);
+template <typename T>
 void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s);
 
 }  // namespace mpx

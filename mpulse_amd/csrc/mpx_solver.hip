@@ -151,6 +151,7 @@ struct mpx_solver {
     int tile_order = 2;            // fused kernels: block order, 0 = z tiles fastest, 1 = x chunks fastest,
                                    // 2 = bands of tile_band rows walked plane by plane (fastest)
     int tile_band = 0;             // 0: chosen from the grid by auto_band
+    double *d_stage = nullptr;     // x-fastest float64 copy of one grid for mpx_fetch_field
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
     double2 *psi2[3][3] = {};
@@ -1686,6 +1687,88 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
         else if (ptr == s->d_jp[0] || ptr == s->d_jp[1] || ptr == s->d_jp[2])
             plasma_ghosts(s, host, 2);
     }
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+struct mpx_fetch {
+    hipEvent_t done = nullptr;  // null: the data was already in place when the handle was made
+};
+
+extern "C" {
+
+int mpx_fetch_field(mpx_solver *s, const char *name, double *host, size_t count, mpx_fetch **handle)
+{
+    if (!s || !name || !host || !handle) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_OWNER_SYNC(s);
+    const GridDims &g = s->g;
+    const size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+    if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
+    void *ptr;
+    size_t elem;
+    bool isPsi;
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
+    if (rc) return fail(rc, std::string("unknown grid ") + name);
+    const bool plasma = ptr && (ptr == s->d_rho || ptr == s->d_sigma || ptr == s->d_jp[0] || ptr == s->d_jp[1] ||
+                                ptr == s->d_jp[2]);
+    if (!ptr || isPsi || plasma) {
+        // grids whose host view is assembled on the host: read synchronously and reorder
+        std::vector<double> tmp(count);
+        if ((rc = mpx_get_field(s, name, tmp.data(), count))) return rc;
+        for (size_t i = 0; i < nx2; ++i)
+            for (size_t j = 0; j < ny2; ++j)
+                for (size_t k = 0; k < nz2; ++k) host[(k * ny2 + j) * nx2 + i] = tmp[(i * ny2 + j) * nz2 + k];
+        *handle = new mpx_fetch;
+        return MPX_OK;
+    }
+    if (!s->d_stage) {
+        if (!(s->d_stage = s->dalloc<double>(count, rc))) return rc;
+    }
+    if ((rc = materialise_ghosts(s))) return rc;
+    if (elem == 8)
+        mpx::launch_pack_xfast<double>(g, (const double *)ptr, s->d_stage, s->stream);
+    else
+        mpx::launch_pack_xfast<float>(g, (const float *)ptr, s->d_stage, s->stream);
+    MPX_HIP(hipGetLastError());
+    MPX_HIP(hipMemcpyAsync(host, s->d_stage, count * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    mpx_fetch *f = new mpx_fetch;
+    if (hipEventCreateWithFlags(&f->done, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(f->done, s->stream) != hipSuccess) {
+        if (f->done) (void)hipEventDestroy(f->done);
+        delete f;
+        return fail(MPX_ERR_HIP, "event for the field copy failed");
+    }
+    *handle = f;
+    return MPX_OK;
+}
+
+int mpx_fetch_wait(mpx_fetch *f)
+{
+    if (!f) return fail(MPX_ERR_INVALID, "null handle");
+    int rc = MPX_OK;
+    if (f->done) {
+        if (hipEventSynchronize(f->done) != hipSuccess) rc = fail(MPX_ERR_HIP, "waiting for the field copy failed");
+        (void)hipEventDestroy(f->done);
+    }
+    delete f;
+    return rc;
+}
+
+int mpx_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return fail(MPX_ERR_INVALID, "null argument");
+    if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(MPX_ERR_NOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return MPX_OK;
+}
+
+int mpx_host_free(void *ptr)
+{
+    if (ptr && hipHostFree(ptr) != hipSuccess) return fail(MPX_ERR_HIP, "hipHostFree failed");
     return MPX_OK;
 }
 

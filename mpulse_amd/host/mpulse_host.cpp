@@ -1,12 +1,15 @@
 // C++ host layer over the C ABI: setup-file front-end, FieldSolver plugins that keep their
 // state on the GPU, and the FieldSimulation / Process driver.
 #include "mpulse_host.h"
+#include "mpx_hdf5.h"
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <sstream>
+#include <mutex>
+#include <thread>
 
 namespace mpulse_hip {
 
@@ -70,6 +73,9 @@ std::string Rebuildable::Rebuild(std::istream &in)
             if (!(in >> tok)) tok.clear();
         } else if (p.kind == Param::DOUBLE) {
             in >> *p.dp;
+            if (!(in >> tok)) tok.clear();
+        } else if (p.kind == Param::STRING) {
+            in >> *p.sp;
             if (!(in >> tok)) tok.clear();
         } else {
             Rebuildable *child = p.make();
@@ -503,6 +509,11 @@ double Storage::energy() const
 // FieldSimulation / Process
 // ---------------------------------------------------------------------------------------
 
+static void add_fielddiag(void *owner, Rebuildable *r)
+{
+    static_cast<FieldSimulation *>(owner)->fieldDiags.push_back(static_cast<FieldDiag *>(r));
+}
+
 static void set_solver(void *owner, Rebuildable *r)
 {
     auto *fs = static_cast<FieldSimulation *>(owner);
@@ -548,12 +559,143 @@ void FieldSimulation::MakeParamMap(ParamMap &pm)
     // the MPI boundaries wrap the same periodic domain; on one GPU the whole domain is local
     pm["mpi_slice_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
     pm["mpi_block_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
+    // diagnostics registry, src/fieldsim.cpp:169-171
+    pm["fielddiag"] = obj(&make_new<FieldDiag>, add_fielddiag);
 }
 
 FieldSimulation::~FieldSimulation()
 {
+    for (FieldDiag *d : fieldDiags) delete d;  // each finishes its writes first
     delete solver;
     delete boundary;
+}
+
+void FieldSimulation::executeDiagnostics()
+{
+    for (FieldDiag *d : fieldDiags) d->execute();
+}
+
+void FieldSimulation::finishDiagnostics()
+{
+    for (FieldDiag *d : fieldDiags) {
+        d->finish();
+        if (!d->error.empty()) throw SetupError(d->error);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// FieldDiag
+// ---------------------------------------------------------------------------------------
+
+struct FieldDiag::Impl {
+    mpx_solver *gpu = nullptr;
+    int *time = nullptr;
+    uint64_t nx2 = 0, ny2 = 0, nz2 = 0;
+    double *buf[2] = {nullptr, nullptr};  // page-locked, alternating between outputs
+    int next = 0;
+    std::thread writer;
+    HDFostream out;  // the open file of an appending diagnostic
+    std::mutex errorMutex;
+};
+
+void FieldDiag::fail(const std::string &m)
+{
+    std::lock_guard<std::mutex> lock(impl->errorMutex);
+    if (error.empty()) error = m;
+}
+
+FieldDiag::~FieldDiag()
+{
+    finish();
+    if (impl) {
+        impl->out.close();
+        for (double *b : impl->buf)
+            if (b) mpx_host_free(b);
+        delete impl;
+    }
+}
+
+void FieldDiag::MakeParamMap(ParamMap &pm)
+{
+    // FieldDiag::MakeParamMap (src/fielddiag.cpp:9-15), DiagnosticInterface::MakeParamMap
+    // (src/diagnostic.cpp:34-47)
+    pm["field"] = pstring(&fieldId, "");
+    pm["file"] = pstring(&fname, "");
+    pm["append"] = pstring(&append, "n");
+    pm["interval"] = pint(&interval, 100);
+}
+
+void FieldDiag::init(FieldSolver *solver, Globals *g, int *time)
+{
+    if (!impl) impl = new Impl;
+    impl->gpu = solver->handle();
+    impl->time = time;
+    impl->nx2 = g->GridX + 2;
+    impl->ny2 = g->GridY + 2;
+    impl->nz2 = g->GridZ + 2;
+    if (interval < 1) throw SetupError("FieldDiag: interval must be positive");
+}
+
+std::string FieldDiag::parsedFileName() const
+{
+    // DiagnosticInterface::parsedFileName, src/diagnostic.cpp:53-72 (one process: id 0)
+    std::string parsed = fname;
+    size_t pos = parsed.find("#p");
+    if (pos != std::string::npos) parsed.replace(pos, 2, "0");
+    pos = parsed.find("#t");
+    if (pos != std::string::npos) parsed.replace(pos, 2, std::to_string(impl && impl->time ? *impl->time : 0));
+    return parsed;
+}
+
+void FieldDiag::execute()
+{
+    // DiagnosticInterface::execute, src/diagnostic.cpp:12-31
+    if (!impl || !impl->gpu) return;
+    const bool appending = !append.empty() && append[0] == 'y';
+    if (t == 0 && appending) {
+        if (!impl->out.open(parsedFileName(), 256)) fail("FieldDiag: " + impl->out.error());
+    }
+    if (t % interval == 0) {
+        const size_t count = impl->nx2 * impl->ny2 * impl->nz2;
+        double *&b = impl->buf[impl->next];
+        if (!b && mpx_host_alloc(count * sizeof(double), (void **)&b) != MPX_OK) {
+            fail(std::string("FieldDiag: ") + mpx_last_error());
+            ++t;
+            return;
+        }
+        // queue the copy behind this step, then hand the previous output's write over
+        mpx_fetch *h = nullptr;
+        if (mpx_fetch_field(impl->gpu, fieldId.c_str(), b, count, &h) != MPX_OK) {
+            fail(std::string("FieldDiag: ") + mpx_last_error());
+            ++t;
+            return;
+        }
+        if (impl->writer.joinable()) impl->writer.join();
+        const std::string path = appending ? std::string() : parsedFileName();
+        const std::vector<uint64_t> dims = {impl->nz2, impl->ny2, impl->nx2};
+        const double *data = b;
+        impl->writer = std::thread([this, h, path, dims, data, appending]() {
+            if (mpx_fetch_wait(h) != MPX_OK) {
+                fail(std::string("FieldDiag: ") + mpx_last_error());
+                return;
+            }
+            if (appending) {
+                if (impl->out.good() && !impl->out.write(dims, data))
+                    fail("FieldDiag: " + impl->out.error());
+                return;
+            }
+            HDFostream o;
+            if (!o.open(path, 1) || !o.write(dims, data)) fail("FieldDiag: " + o.error());
+            o.close();
+        });
+        impl->next ^= 1;
+    }
+    ++t;
+}
+
+void FieldDiag::finish()
+{
+    if (impl && impl->writer.joinable()) impl->writer.join();
 }
 
 void FieldSimulation::init(Globals *g, int *time)
@@ -567,6 +709,7 @@ void FieldSimulation::init(Globals *g, int *time)
     solver->time = time;
     storage.reset(new Storage(solver));
     solver->initStorage(storage.get());
+    for (FieldDiag *d : fieldDiags) d->init(solver, g, time);  // FieldDiag::fetchField
 }
 
 void FieldSimulation::execute() { solver->stepScheme(dt); }
@@ -622,7 +765,9 @@ void Process::run(int steps, bool verbose)
     for (; time < T; ++time) {
         if (verbose) std::cout << "Cycle " << time << std::endl << std::flush;
         fieldSim->execute();
+        fieldSim->executeDiagnostics();
     }
+    fieldSim->finishDiagnostics();  // every file written so far is complete on return
 }
 
 }  // namespace mpulse_hip
@@ -705,3 +850,58 @@ int mph_destroy(mph_sim *sim)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// HDFostream entry points
+// ---------------------------------------------------------------------------------------
+
+struct mph_h5 {
+    HDFostream out;
+};
+
+int mph_h5_open(const char *path, int capacity, mph_h5 **out)
+{
+    if (!path || !out) return mph_fail("null argument");
+    mph_h5 *h = new mph_h5;
+    if (!h->out.open(path, capacity)) {
+        std::string e = h->out.error();
+        delete h;
+        return mph_fail(e);
+    }
+    *out = h;
+    return MPX_OK;
+}
+
+int mph_h5_set_block_name(mph_h5 *h, const char *name)
+{
+    if (!h || !name) return mph_fail("null argument");
+    h->out.setBlockName(name);
+    return MPX_OK;
+}
+
+int mph_h5_set_mtime(mph_h5 *h, unsigned int mtime)
+{
+    if (!h) return mph_fail("null argument");
+    h->out.setModificationTime(mtime);
+    return MPX_OK;
+}
+
+int mph_h5_write(mph_h5 *h, int rank, const unsigned long long *dims, const double *data, const int *extent,
+                 int n_extent)
+{
+    if (!h || !dims || (!data && rank > 0)) return mph_fail("null argument");
+    if (rank < 1 || rank > 3) return mph_fail("dataset rank must be 1..3");
+    std::vector<uint64_t> d(dims, dims + rank);
+    std::vector<int32_t> e;
+    if (extent && n_extent > 0) e.assign(extent, extent + n_extent);
+    if (!h->out.write(d, data, e)) return mph_fail(h->out.error());
+    return MPX_OK;
+}
+
+int mph_h5_close(mph_h5 *h)
+{
+    if (!h) return mph_fail("null argument");
+    h->out.close();
+    delete h;
+    return MPX_OK;
+}

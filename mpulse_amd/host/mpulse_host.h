@@ -27,9 +27,10 @@ class Rebuildable;
 
 // Reads one value of a parameter, or builds one child object.
 struct Param {
-    enum Kind { INT, DOUBLE, OBJECT } kind;
+    enum Kind { INT, DOUBLE, STRING, OBJECT } kind;
     int *ip = nullptr;
     double *dp = nullptr;
+    std::string *sp = nullptr;
     // OBJECT: factory making the child; the child is handed to `sink`
     Rebuildable *(*make)() = nullptr;
     void (*sink)(void *owner, Rebuildable *child) = nullptr;
@@ -61,6 +62,14 @@ class Rebuildable {
         Param q;
         q.kind = Param::DOUBLE;
         q.dp = p;
+        return q;
+    }
+    static Param pstring(std::string *p, const std::string &def)
+    {
+        *p = def;
+        Param q;
+        q.kind = Param::STRING;
+        q.sp = p;
         return q;
     }
 };
@@ -275,14 +284,52 @@ class FDTD_SimpleHIP : public FieldSolver {
 // driver
 // ---------------------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------------------
+
+class HDFostream;
+
+// FieldDiag (src/fielddiag.cpp:9-24) with the keys of DiagnosticInterface
+// (src/diagnostic.cpp:34-68): "field", "file" (#p -> process id, #t -> time step), "append"
+// (y/n), "interval".  execute() follows DiagnosticInterface::execute: output at its 0th call
+// and every `interval` calls after; without append one file per output, with append one file
+// whose datasets are data0, data1, ...  The grid goes out as the reference writes it: the
+// storage box 0..N+1, x fastest, HDF5 dims (Nz+2, Ny+2, Nx+2), float64.  The copy is queued
+// behind the step (mpx_fetch_field) into page-locked memory and a writer thread writes the
+// file while the GPU steps on; finish() waits for the writes in flight.
+class FieldDiag : public Rebuildable {
+  public:
+    ~FieldDiag() override;
+    std::string fieldId, fname, append;
+    int interval = 100;
+    void init(FieldSolver *solver, Globals *g, int *time);
+    void execute();
+    void finish();
+    std::string parsedFileName() const;  // DiagnosticInterface::parsedFileName
+    std::string error;                   // first failure of a write, empty when all went out
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+
+  private:
+    struct Impl;
+    Impl *impl = nullptr;
+    int t = 0;
+    void fail(const std::string &m);  // keeps the first failure (called from the writer thread too)
+};
+
 class FieldSimulation : public Rebuildable {  // src/fieldsim.cpp:31-194
   public:
     ~FieldSimulation() override;
     FieldSolver *solver = nullptr;
     Boundary *boundary = nullptr;
     std::unique_ptr<Storage> storage;
+    std::vector<FieldDiag *> fieldDiags;  // "fielddiag" blocks, in setup order
     void init(Globals *g, int *time);
     void execute();
+    void executeDiagnostics();  // DiagnosticManager::execute for the field diagnostics
+    void finishDiagnostics();
     double dt = 0;
 
   protected:
@@ -323,4 +370,16 @@ int mph_run(mph_sim *sim, int steps);
 mpx_solver *mph_solver(mph_sim *sim);
 int mph_destroy(mph_sim *sim);
 const char *mph_last_error(void);
+
+// HDFostream (src/hdfstream.{h,t,cpp}) without the HDF5 library: open a file for up to
+// `capacity` datasets, write float64 datasets of rank 1..3 (HDF5 dims slowest first, data in
+// C order over them) named as HDFstream::getNextBlockName names them, optionally with the
+// int32 "extent" attribute (n_extent > 0).  mtime 0: the time of each write.
+typedef struct mph_h5 mph_h5;
+int mph_h5_open(const char *path, int capacity, mph_h5 **out);
+int mph_h5_set_block_name(mph_h5 *h, const char *name);
+int mph_h5_set_mtime(mph_h5 *h, unsigned int mtime);
+int mph_h5_write(mph_h5 *h, int rank, const unsigned long long *dims, const double *data, const int *extent,
+                 int n_extent);
+int mph_h5_close(mph_h5 *h);
 }
