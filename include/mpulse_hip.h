@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define MPX_ABI_VERSION 2
+#define MPX_ABI_VERSION 3
 #define MPX_MAX_FACTORIES 8
 #define MPX_MAX_POLES 3
 
@@ -76,6 +76,10 @@ extern "C" {
 #define MPX_FACTORY_PLANE_WAVE    4  /* "wave_inject"          src/sources.cpp:8-173         */
 #define MPX_FACTORY_PLANE_GAUSS   5  /* "plane_gauss_inject"   src/sources.cpp:179-331       */
 #define MPX_FACTORY_PLASMA_CURRENT 7 /* "plasma_current"     src/plasmacurrent.cpp:6-74   */
+#define MPX_FACTORY_GAUSS_INJECT  8  /* "gauss_inject"         src/gaussinject.cpp:33-49: a Gaussian
+                                        pulse built in (kx, ky, omega) space and brought to the
+                                        injection plane by three Fourier transforms (hipFFT) in
+                                        blocks of time steps, injected on the down face */
 #define MPX_FACTORY_FOCUSED_PULSE 6  /* "focused_pulse_inject" keys of src/focusedpulseinject.cpp:38-56;
                                         the field model is this library's own (the reference's does
                                         not compile): x-polarised Gaussian pupil of waist "width",
@@ -110,6 +114,13 @@ typedef struct mpx_factory {
     double em;            /* "em"     (1)   e/m of the carriers */
     double gamma;         /* "gamma"  (0.1) friction           */
     double rho_bg;        /* "rho_bg" (0)   background density */
+    /* gauss_inject keys (also "length" (1), "width" (1), "amp" (1), "eps" (1)),
+       src/gaussinject.cpp:37-47 */
+    int    gsize_t;       /* "SizeT"  (4096) time samples of the spectrum */
+    int    toff;          /* "toff"   (0)    first step of the pulse      */
+    double k0;            /* "k0"     (2 pi) centre wave number           */
+    double gshift;        /* "Shift"  (0)    focus shift along z          */
+    double gtime;         /* "Time"   (0)    time offset                  */
 } mpx_factory;
 
 typedef struct mpx_desc {

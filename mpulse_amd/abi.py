@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmpulse_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_FACTORIES = 8
 
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0, -1, -2, -3, -4, -5, -6
@@ -23,6 +23,7 @@ PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
 FACTORY_FOCUSED_PULSE = 6
 FACTORY_PLASMA_CURRENT = 7
+FACTORY_GAUSS_INJECT = 8
 
 # Direction enum of the reference (src/mpulse.h:28)
 NORTH, SOUTH, WEST, EAST, UP, DOWN = 0, 1, 2, 3, 4, 5
@@ -41,6 +42,8 @@ class Factory(C.Structure):
         ("k", C.c_double * 3), ("h", C.c_double * 3), ("h_bg", C.c_double * 3), ("e_bg", C.c_double * 3),
         ("ramp", C.c_double), ("gwidth", C.c_double), ("goffset", C.c_double),
         ("em", C.c_double), ("gamma", C.c_double), ("rho_bg", C.c_double),
+        ("gsize_t", C.c_int), ("toff", C.c_int),
+        ("k0", C.c_double), ("gshift", C.c_double), ("gtime", C.c_double),
     ]
 
 
@@ -98,6 +101,7 @@ def factory_defaults(kind):
         f.k[2] = 1.0
         f.ramp, f.gwidth, f.goffset = 0.5, 10.0, 40.0
     f.em, f.gamma, f.rho_bg = 1.0, 0.1, 0.0  # src/plasmacurrent.cpp:10-12
+    f.gsize_t, f.toff, f.k0, f.gshift, f.gtime = 4096, 0, 2 * np.pi, 0.0, 0.0  # src/gaussinject.cpp:37-47
     return f
 
 

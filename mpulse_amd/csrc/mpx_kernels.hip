@@ -86,6 +86,19 @@ __device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const Fadde
             double posTime = time * P.DT - P.TShift;
             f[0] = P.amp * pulse_efunc(P, ft, posxh, posyo, poszo, posTime).re;
         }
+    } else if (a.kind == 4) {
+        // GaussInjectSourceFunc::getHField / getEField, src/gaussinject.cpp:203-224
+        const GaussSheetParams &G = a.gauss;
+        if (G.active) {
+            const size_t q = ((size_t)(time - G.blockToff) * a.nx2 + i) * a.ny2 + j;
+            if (!a.isH) {
+                f[0] = G.sqeps * G.amp * G.F1[q];
+            } else {
+                f[0] = G.amp * G.F1[q];
+                f[1] = G.amp * G.F2[q];
+            }
+        }
+        (void)z;
     } else if (a.kind == 3) {
         // amp Re[env(t - z/v) U exp(-i(om0 t - phase))], env(s) = exp(-s^2/T^2)
         const FocusedParams &F = a.focused;

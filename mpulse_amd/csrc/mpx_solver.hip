@@ -10,9 +10,11 @@
 #include <string>
 #include <algorithm>
 #include <array>
+#include <memory>
 #include <vector>
 
 #include "../../include/mpulse_hip.h"
+#include "mpx_gauss.h"
 #include "mpx_launch.h"
 
 using mpx::CpmlFace;
@@ -70,6 +72,7 @@ struct Current {
     int face = -1;    // CPML: index into the side's face list
     // source evaluation
     mpx::SheetEval ev{};
+    std::shared_ptr<mpx::GaussInjectDev> gauss;  // GaussInject: block schedule and samples
 };
 
 struct Timer {
@@ -511,6 +514,11 @@ void mpx_factory_defaults(mpx_factory *f, int kind)
     f->em = 1.0;  // src/plasmacurrent.cpp:10-12
     f->gamma = 0.1;
     f->rho_bg = 0.0;
+    f->gsize_t = 4096;  // src/gaussinject.cpp:37-47
+    f->toff = 0;
+    f->k0 = 2 * M_PI;
+    f->gshift = 0;
+    f->gtime = 0;
 }
 
 void mpx_desc_defaults(mpx_desc *d)
@@ -677,7 +685,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                 else
                     cpml_kappa(s, f.d, f.kappa_max);
             } else if (f.kind == MPX_FACTORY_SHORT_PULSE || f.kind == MPX_FACTORY_PLANE_WAVE ||
-                       f.kind == MPX_FACTORY_PLANE_GAUSS || f.kind == MPX_FACTORY_FOCUSED_PULSE) {
+                       f.kind == MPX_FACTORY_PLANE_GAUSS || f.kind == MPX_FACTORY_FOCUSED_PULSE ||
+                       f.kind == MPX_FACTORY_GAUSS_INJECT) {
                 if (ncpml == 0) s->sheets_first = 1;
                 // needCurrent(down) only: src/shortpulseinject.cpp:28-31, src/sources.cpp:8-11
                 for (int h = 0; h < 2; ++h) {
@@ -792,7 +801,17 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     ev.ny2 = g.n[1] + 2;
                     ev.sheet.kp = s->layers[l0].lo[2];
                     for (int w = 0; w < 3; ++w) ev.sheet.J[w] = (double *)s->layers[lc[w]].comp[0];
-                    if (f.kind == MPX_FACTORY_FOCUSED_PULSE) {
+                    if (f.kind == MPX_FACTORY_GAUSS_INJECT) {
+                        ev.kind = 4;
+                        if (f.gsize_t < 2 || !(f.eps > 0))
+                            return fail(MPX_ERR_INVALID, "gauss_inject: SizeT must be >= 2 and eps > 0");
+                        c.gauss = std::make_shared<mpx::GaussInjectDev>();
+                        int grc = c.gauss->setup(f, isH ? 1 : 0, g.n[0], g.n[1], desc->dx, desc->dy, desc->dt, 0,
+                                                 g.n[2] + 1, s->stream);
+                        if (grc == 1) return fail(MPX_ERR_NOMEM, "gauss_inject: device memory for the transforms");
+                        if (grc) return fail(MPX_ERR_HIP, "gauss_inject: hipFFT plan creation failed");
+                        c.gauss->params(ev.gauss);
+                    } else if (f.kind == MPX_FACTORY_FOCUSED_PULSE) {
                         ev.kind = 3;
                         if (!(f.eps > 0) || !(f.width > 0) || !(f.length != 0) || !(f.om0 > 0))
                             return fail(MPX_ERR_INVALID, "focused_pulse_inject: eps, width, om0 must be > 0, length != 0");
@@ -1120,11 +1139,27 @@ static mpx::SumList sum_list(mpx_solver *s, bool hside)
     return L;
 }
 
+// the step index of a sheet evaluation; a GaussInject sheet first runs its block schedule
+// (GaussInjectSourceFunc::setTime, called by IncidentSource*Current::stepScheme)
+static int sheet_time(mpx_solver *s, Current &c)
+{
+    c.ev.time = s->time;
+    if (c.gauss) {
+        int rc = c.gauss->setTime(s->time);
+        if (rc) return fail(MPX_ERR_HIP, "gauss_inject: transform of a sample block failed");
+        c.gauss->params(c.ev.gauss);
+        if (c.ev.gauss.active && (s->time - c.ev.gauss.blockToff < 0 || s->time - c.ev.gauss.blockToff >= c.gauss->Nt))
+            return fail(MPX_ERR_INVALID, "gauss_inject: step outside the sample block");
+    }
+    return MPX_OK;
+}
+
 static int eval_sheets(mpx_solver *s, std::vector<Current> &list, int type)
 {
     for (Current &c : list)
         if (c.type == type) {
-            c.ev.time = s->time;
+            int rc = sheet_time(s, c);
+            if (rc) return rc;
             s->tick(K_SHEET);
             mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
             s->tock();
@@ -1328,7 +1363,7 @@ static int step_once(mpx_solver *s)
                 mpx::launch_ref_plasma_current(ae, s->stream);
                 MPX_LAUNCHED("plasma current");
             } else {
-                c.ev.time = s->time;
+                if ((rc = sheet_time(s, c))) return rc;
                 s->tick(K_SHEET);
                 mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
                 s->tock();
@@ -1357,7 +1392,7 @@ static int step_once(mpx_solver *s)
                 s->tock();
                 MPX_LAUNCHED("cpml H");
             } else {
-                c.ev.time = s->time;
+                if ((rc = sheet_time(s, c))) return rc;
                 s->tick(K_SHEET);
                 mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
                 s->tock();
@@ -1500,7 +1535,7 @@ int mpx_step_init(mpx_solver *s)
             mpx::launch_ref_cpml(ah, c.face, true, s->stream);
             MPX_LAUNCHED("cpml H");
         } else {
-            c.ev.time = s->time;
+            if ((rc = sheet_time(s, c))) return rc;
             mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
             MPX_LAUNCHED("sheet H");
         }
@@ -2298,6 +2333,9 @@ int mpx_batch_create(const mpx_desc *descs, int nruns, mpx_batch **out)
         for (int q = 0; q < d.n_factories; ++q)
             if (d.factories[q].kind == MPX_FACTORY_PLASMA_CURRENT)
                 return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take a plasma current"));
+            else if (d.factories[q].kind == MPX_FACTORY_GAUSS_INJECT)
+                return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take gauss_inject (its sample blocks "
+                                                      "are scheduled from the host)"));
         mpx_solver *s = nullptr;
         int rc = mpx_create(&d, &s);
         if (rc) return bail(rc);
@@ -2579,6 +2617,11 @@ int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K,
                                          "Lorentz poles or plasma");
     if (!s->periodic[0] || !s->periodic[1] || !s->periodic[2])
         return fail(MPX_ERR_UNSUPPORTED, "the adjoint needs the periodic boundary");
+    for (const std::vector<Current> *list : {&s->ecur, &s->hcur})
+        for (const Current &c : *list)
+            if (c.gauss)
+                return fail(MPX_ERR_UNSUPPORTED, "the adjoint does not take gauss_inject (its sample blocks advance "
+                                                 "with the forward steps only)");
     if (obj->plane_k < 1 || obj->plane_k > s->g.n[2]) return fail(MPX_ERR_INVALID, "objective plane out of range");
     const bool grad = dJ_deps || dJ_dchi;
     hipStream_t st = s->stream;

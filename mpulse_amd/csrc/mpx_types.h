@@ -70,8 +70,16 @@ struct FocusedParams {
     double zE, zB, T, om0, v, TShift, phase, amp, DT;
 };
 
+// GaussInject samples of the block in force (GaussInjectSourceFunc::getEField/getHField,
+// src/gaussinject.cpp:203-224): F[(t - blockToff) (nx+2) + i) (ny+2) + j]
+struct GaussSheetParams {
+    const double *F1, *F2;
+    int active, blockToff;
+    double amp, sqeps;  // amp, sqrt(eps)
+};
+
 struct SheetEval {
-    int kind;        // 1 short pulse, 2 plane wave / gauss, 3 focused pulse
+    int kind;        // 1 short pulse, 2 plane wave / gauss, 3 focused pulse, 4 GaussInject
     int isH;         // 0: E current (samples H), 1: H current (samples E)
     int off;         // offset of the sampled cell along z (IncidentSource*Current::stepScheme)
     int t1, t2;      // transverse components
@@ -82,6 +90,7 @@ struct SheetEval {
     PulseParams pulse;
     PlaneParams plane;
     FocusedParams focused;
+    GaussSheetParams gauss;
 };
 
 // PLRC pole constants (PLRCData, src/fdtd_plrc.h:71-78) of the active poles, precomputed on the

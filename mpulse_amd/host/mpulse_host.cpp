@@ -177,6 +177,37 @@ mpx_factory ShortPulseInject::describe() const
     return f;
 }
 
+void GaussInject::MakeParamMap(ParamMap &pm)
+{
+    IncidentSource::MakeParamMap(pm);
+    pm["SizeT"] = pint(&SizeT, 4096);
+    pm["length"] = pdouble(&length, 1.);
+    pm["width"] = pdouble(&width, 1.);
+    pm["k0"] = pdouble(&k0, 2 * 3.14159265358979323846);
+    pm["Shift"] = pdouble(&Shift, 0);
+    pm["Time"] = pdouble(&Time, 0);
+    pm["amp"] = pdouble(&amp, 1);
+    pm["eps"] = pdouble(&eps, 1);
+    pm["toff"] = pint(&toff, 0);
+}
+
+mpx_factory GaussInject::describe() const
+{
+    mpx_factory f;
+    mpx_factory_defaults(&f, MPX_FACTORY_GAUSS_INJECT);
+    f.d = distance;
+    f.gsize_t = SizeT;
+    f.length = length;
+    f.width = width;
+    f.k0 = k0;
+    f.gshift = Shift;
+    f.gtime = Time;
+    f.amp = amp;
+    f.eps = eps;
+    f.toff = toff;
+    return f;
+}
+
 void FocusedPulseInject::MakeParamMap(ParamMap &pm)
 {
     IncidentSource::MakeParamMap(pm);
@@ -290,7 +321,6 @@ struct Unsupported : CurrentFactory {
     void MakeParamMap(ParamMap &) override {}
 };
 static const char kSideInject[] = "side_inject";
-static const char kGaussInject[] = "gauss_inject";
 
 static void push_factory(void *owner, Rebuildable *r)
 {
@@ -353,7 +383,7 @@ void FDTD_PLRCSolverHIP::MakeParamMap(ParamMap &pm)
     pm["plasma"] = pobject<FDTD_PLRCSolverHIP, Rebuildable, PlasmaDensityBlock>(this, set_plasma);
     pm["plasma_current"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, PlasmaCurrentFactory>(this, push_factory);
     pm["side_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kSideInject>>(this, push_factory);
-    pm["gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, Unsupported<kGaussInject>>(this, push_factory);
+    pm["gauss_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, GaussInject>(this, push_factory);
     pm["focused_pulse_inject"] = pobject<FDTD_PLRCSolverHIP, CurrentFactory, FocusedPulseInject>(this, push_factory);
 }
 

@@ -207,6 +207,16 @@ class PlaneWaveSource : public IncidentSource {  // src/sources.cpp:58-82
     void MakeParamMap(ParamMap &pm) override;
 };
 
+class GaussInject : public IncidentSource {  // src/gaussinject.cpp:33-49
+  public:
+    int SizeT = 4096, toff = 0;
+    double length = 1, width = 1, k0 = 6.283185307179586, Shift = 0, Time = 0, amp = 1, eps = 1;
+    mpx_factory describe() const override;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+};
+
 class PlaneGaussSource : public IncidentSource {  // src/sources.cpp:225-242
   public:
     double kx = 0, ky = 0, kz = 1, Hx = 0, Hy = 0, Hz = 0, width = 10, offset = 40, eps = 1;

@@ -526,6 +526,182 @@ static void plane_getE(const plane_t *W, int i, int j, int l, int time, double o
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* GaussInject source function, src/gaussinject.cpp:56-482 (FFTW transforms restated as   */
+/* direct discrete Fourier sums with the same sign conventions and index shifts)          */
+/* ------------------------------------------------------------------------------------ */
+
+enum { GAUSS_EX = 0, GAUSS_EY = 1, GAUSS_BX = 2 };
+
+typedef struct {
+    int isH;
+    int SizeX, SizeY, SizeT, SizeXH, SizeYH, SizeTH, Nt;
+    double BoxSizeX, BoxSizeY, BoxSizeT, DX, DY, DT;
+    double length, width, k0, Shift, Time, amp, eps;
+    int toff, blockCount, blockToff, active;
+    double sx, sy, sz; /* GaussFunc::setShift */
+    double *F1, *F2;   /* [kh][i][j]: Nt x SizeX x SizeY */
+} gauss_t;
+
+/* GaussInjectSourceFunc::setParam src/gaussinject.cpp:56-106, initSourceFunc :108-150
+   (Nt from the storage extent along z, lowz..highz) */
+static int gauss_setup(gauss_t *G, const mpx_factory *f, int isH, int nx, int ny, double dx, double dy,
+                       double dt, int lowz, int highz)
+{
+    memset(G, 0, sizeof(*G));
+    G->isH = isH;
+    G->SizeT = f->gsize_t;
+    G->length = f->length;
+    G->width = f->width;
+    G->k0 = f->k0;
+    G->Shift = f->gshift;
+    G->Time = f->gtime;
+    G->amp = f->amp;
+    G->eps = f->eps;
+    G->toff = f->toff;
+    double lightspeed = sqrt(1 / G->eps);
+    G->SizeX = nx + 2;
+    G->SizeY = ny + 2;
+    G->SizeTH = G->SizeT / 2;
+    G->SizeXH = G->SizeX / 2;
+    G->SizeYH = G->SizeY / 2;
+    G->DX = dx;
+    G->DY = dy;
+    G->DT = dt * lightspeed;
+    G->BoxSizeX = G->SizeX * G->DX;
+    G->BoxSizeY = G->SizeY * G->DY;
+    G->BoxSizeT = G->SizeT * G->DT;
+    G->Nt = 1 << (int)round(log((double)(highz - lowz)) / log(2.0) + 1e-6);
+    G->blockToff = G->toff;
+    size_t n = (size_t)G->Nt * G->SizeX * G->SizeY;
+    G->F1 = (double *)calloc(n, sizeof(double));
+    G->F2 = (double *)calloc(n, sizeof(double));
+    return G->F1 && G->F2;
+}
+
+/* GaussFunc, src/gaussinject.h:150-200 */
+static double gauss_comp(int comp, double x, double y, double z)
+{
+    double kp = sqrt(x * x + z * z), k = sqrt(x * x + y * y + z * z);
+    if (comp == GAUSS_EX) {
+        double D = k * kp;
+        return D == 0 ? 0 : (-x * y / D);
+    }
+    if (comp == GAUSS_EY) return k == 0 ? 0 : (kp / k);
+    return kp == 0 ? 0 : (-z / kp);
+}
+
+/* direct sum  sum_m in[m] exp(sgn 2 pi i m n / N)  with exact twiddle indices (m n mod N) */
+static cplx gauss_dft(const cplx *in, int N, int n, int sgn, const cplx *tw)
+{
+    cplx acc = 0;
+    for (int m = 0; m < N; ++m) {
+        long long q = ((long long)m * n) % N;
+        acc += in[m] * (sgn > 0 ? tw[q] : conj(tw[q]));
+    }
+    return acc;
+}
+
+/* fillEx / fillEy / fillBx with transformT, transformY, transformX, src/gaussinject.cpp:235-482 */
+static void gauss_fill(gauss_t *G, int comp, double *F)
+{
+    const int SX = G->SizeX, SY = G->SizeY, ST = G->SizeT, Nt = G->Nt;
+    const int blockmin = G->blockCount * Nt;
+    const double ft = 2 * M_PI * G->length / G->BoxSizeT, fy = G->width / G->BoxSizeY, fx = G->width / G->BoxSizeX;
+    int NM = ST > SX ? ST : SX;
+    if (SY > NM) NM = SY;
+    cplx *tw = (cplx *)malloc(sizeof(cplx) * NM), *A = (cplx *)malloc(sizeof(cplx) * ST);
+    cplx *uo = (cplx *)malloc(sizeof(cplx) * ST), *row = (cplx *)malloc(sizeof(cplx) * NM);
+    cplx *TY = (cplx *)malloc(sizeof(cplx) * (size_t)SY * Nt);
+    cplx *TX = (cplx *)malloc(sizeof(cplx) * (size_t)SX * SY * Nt);
+    for (int i = 0; i < SX; ++i) {
+        for (int j = 0; j < SY; ++j) {
+            for (int k = 0; k < ST; ++k) {
+                double kx = 2 * M_PI * (i - G->SizeXH) / G->BoxSizeX;
+                double ky = 2 * M_PI * (j - G->SizeYH) / G->BoxSizeY;
+                double om = 2 * M_PI * (k - G->SizeTH) / G->BoxSizeT;
+                double K = om * om - ky * ky - kx * kx;
+                if (fabs(K) < 1e-9) K = 0;
+                if (K < 0)
+                    A[k] = 0;
+                else {
+                    double kz = sqrt(K) * (om > 0 ? 1 : (om < 0 ? -1 : 0));
+                    double zoff = kz - G->k0;
+                    double g = exp(-0.25 * zoff * zoff * G->length * G->length -
+                                   0.25 * (kx * kx + ky * ky) * G->width * G->width);
+                    double a = gauss_comp(comp, kx, ky, kz) * g;
+                    double ph = -(kx * G->sx + ky * G->sy + kz * G->sz) + 0.5 * M_PI;
+                    double th = ph + G->Time * om; /* a exp(-i th) */
+                    A[k] = a * CX(cos(th), -sin(th));
+                }
+            }
+            /* transformT: backward transform over omega, keep this block's Nt samples */
+            for (int k = 0; k < ST; ++k) uo[k] = A[(k + G->SizeTH) % ST];
+            for (int q = 0; q < ST; ++q) tw[q] = CX(cos(2 * M_PI * q / ST), sin(2 * M_PI * q / ST));
+            for (int kh = 0; kh < Nt; ++kh) {
+                int n = (kh + blockmin + G->SizeTH) % ST;
+                TY[(size_t)j * Nt + kh] = ft * gauss_dft(uo, ST, n, +1, tw);
+            }
+        }
+        /* transformY(i): forward transform over y */
+        for (int q = 0; q < SY; ++q) tw[q] = CX(cos(2 * M_PI * q / SY), sin(2 * M_PI * q / SY));
+        for (int kh = 0; kh < Nt; ++kh) {
+            for (int j = 0; j < SY; ++j) row[(j + G->SizeYH) % SY] = TY[(size_t)j * Nt + kh];
+            for (int j = 0; j < SY; ++j) {
+                int jh = (j + G->SizeYH) % SY;
+                TX[((size_t)i * SY + j) * Nt + kh] = fy * gauss_dft(row, SY, jh, -1, tw);
+            }
+        }
+    }
+    /* transformX: forward transform over x, real part */
+    for (int q = 0; q < SX; ++q) tw[q] = CX(cos(2 * M_PI * q / SX), sin(2 * M_PI * q / SX));
+    for (int j = 0; j < SY; ++j)
+        for (int kh = 0; kh < Nt; ++kh) {
+            for (int i = 0; i < SX; ++i) row[(i + G->SizeXH) % SX] = TX[((size_t)i * SY + j) * Nt + kh];
+            for (int i = 0; i < SX; ++i) {
+                int ih = (i + G->SizeXH) % SX;
+                F[((size_t)kh * SX + i) * SY + j] = fx * creal(gauss_dft(row, SX, ih, -1, tw));
+            }
+        }
+    free(tw);
+    free(A);
+    free(uo);
+    free(row);
+    free(TY);
+    free(TX);
+}
+
+/* GaussInjectSourceFunc::setTime, src/gaussinject.cpp:152-182 */
+static void gauss_settime(gauss_t *G, int time)
+{
+    if (time < G->toff + G->blockCount * G->Nt) return;
+    if (time >= G->toff + G->SizeT) {
+        G->active = 0;
+        return;
+    }
+    if (!G->isH) {
+        G->sx = 0, G->sy = 0.5 * G->DX, G->sz = G->Shift + 0.5 * G->DX;
+        gauss_fill(G, GAUSS_BX, G->F1);
+    } else {
+        G->Time -= 0.5 * G->DT;
+        G->sx = 0.5 * G->DX, G->sy = 0, G->sz = G->Shift;
+        gauss_fill(G, GAUSS_EX, G->F1);
+        G->sx = 0, G->sy = 0.5 * G->DX, G->sz = G->Shift;
+        gauss_fill(G, GAUSS_EY, G->F2);
+        G->Time += 0.5 * G->DT;
+    }
+    G->blockToff = G->toff + G->blockCount * G->Nt;
+    G->blockCount++;
+    G->active = 1;
+}
+
+/* getHField / getEField, src/gaussinject.cpp:203-224 */
+static double gauss_at(const gauss_t *G, const double *F, int i, int j, int time)
+{
+    int t = time - G->blockToff;
+    return F[((size_t)t * G->SizeX + i) * G->SizeY + j];
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* Focused pulse (focused_pulse_inject keys; the field model is the library's own design)  */
 /* ------------------------------------------------------------------------------------ */
 
@@ -625,6 +801,25 @@ static double focused_value(const focused_t *F, const double *U, int i, int j, d
     return F->amp * env * (U[q] * cos(ph) - U[q + 1] * sin(ph));
 }
 
+int or_gauss_block(const mpx_factory *f, int isH, int nx, int ny, double dx, double dy, double dt, int lowz,
+                   int highz, int time, double *F1, double *F2, int *block_toff)
+{
+    gauss_t G;
+    if (f->gsize_t < 2 || !gauss_setup(&G, f, isH, nx, ny, dx, dy, dt, lowz, highz)) return 0;
+    /* the sheet calls setTime every step: replay the block boundaries up to `time` */
+    for (int t = 0; t <= time; ++t) gauss_settime(&G, t);
+    int nt = G.active ? G.Nt : 0;
+    if (nt) {
+        size_t n = (size_t)G.Nt * G.SizeX * G.SizeY;
+        if (F1) memcpy(F1, G.F1, n * sizeof(double));
+        if (F2) memcpy(F2, G.F2, n * sizeof(double));
+        if (block_toff) *block_toff = G.blockToff;
+    }
+    free(G.F1);
+    free(G.F2);
+    return nt;
+}
+
 void or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy, double z, int comp, double sx,
                       double sy, double *out)
 {
@@ -636,7 +831,7 @@ void or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy
 /* ------------------------------------------------------------------------------------ */
 
 enum { CUR_CPML_E = 1, CUR_CPML_H = 2, CUR_INC_E = 3, CUR_INC_H = 4, CUR_PLASMA = 5 };
-enum { SRC_PULSE = 1, SRC_PLANE = 2, SRC_FOCUSED = 3 };
+enum { SRC_PULSE = 1, SRC_PLANE = 2, SRC_FOCUSED = 3, SRC_GAUSS = 4 };
 
 typedef struct {
     char name[24];
@@ -666,6 +861,7 @@ typedef struct {
     pulse_t pulse;
     plane_t plane;
     focused_t focused;
+    gauss_t gauss;
     /* plasma current */
     double em, gamma, rho_bg;
 } current_t;
@@ -1062,7 +1258,12 @@ static void inc_init_storage(or_sim *s, current_t *c, int isH)
 
 static void src_getH(const current_t *c, int i, int j, int k, int time, double out[3])
 {
-    if (c->src_kind == SRC_FOCUSED) {
+    if (c->src_kind == SRC_GAUSS) {
+        const gauss_t *G = &c->gauss;
+        out[0] = G->active ? sqrt(G->eps) * G->amp * gauss_at(G, G->F1, i, j, time) : 0;
+        out[1] = out[2] = 0;
+        (void)k;
+    } else if (c->src_kind == SRC_FOCUSED) {
         const focused_t *F = &c->focused;
         double t = (time - 0.5) * F->DT - F->TShift;
         out[0] = focused_value(F, F->bx, i, j, t, F->zB);
@@ -1077,7 +1278,13 @@ static void src_getH(const current_t *c, int i, int j, int k, int time, double o
 
 static void src_getE(const current_t *c, int i, int j, int k, int time, double out[3])
 {
-    if (c->src_kind == SRC_FOCUSED) {
+    if (c->src_kind == SRC_GAUSS) {
+        const gauss_t *G = &c->gauss;
+        out[0] = G->active ? G->amp * gauss_at(G, G->F1, i, j, time) : 0;
+        out[1] = G->active ? G->amp * gauss_at(G, G->F2, i, j, time) : 0;
+        out[2] = 0;
+        (void)k;
+    } else if (c->src_kind == SRC_FOCUSED) {
         const focused_t *F = &c->focused;
         double t = time * F->DT - F->TShift;
         out[0] = focused_value(F, F->ex, i, j, t, F->zE);
@@ -1095,6 +1302,7 @@ static void inc_step_E(or_sim *s, current_t *c)
 {
     grid_t *J0 = c->J[c->t1], *J1 = c->J[c->t2];
     int Time = s->time;
+    if (c->src_kind == SRC_GAUSS) gauss_settime(&c->gauss, Time);
     int off[3] = {0, 0, 0};
     off[c->dim] = -1;
     double DX = c->dX[2];
@@ -1114,6 +1322,7 @@ static void inc_step_H(or_sim *s, current_t *c)
 {
     grid_t *J0 = c->J[c->t1], *J1 = c->J[c->t2];
     int Time = s->time;
+    if (c->src_kind == SRC_GAUSS) gauss_settime(&c->gauss, Time);
     int off[3] = {0, 0, 0};
     off[c->dim] = c->reverse ? +1 : 0;
     double DX = c->dX[2];
@@ -1736,7 +1945,8 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
                 else
                     cpml_init_kappa(s, f->d, f->kappa_max);
             } else if (f->kind == MPX_FACTORY_SHORT_PULSE || f->kind == MPX_FACTORY_PLANE_WAVE ||
-                       f->kind == MPX_FACTORY_PLANE_GAUSS || f->kind == MPX_FACTORY_FOCUSED_PULSE) {
+                       f->kind == MPX_FACTORY_PLANE_GAUSS || f->kind == MPX_FACTORY_FOCUSED_PULSE ||
+                       f->kind == MPX_FACTORY_GAUSS_INJECT) {
                 /* needCurrent: down face only, src/shortpulseinject.cpp:28-31, src/sources.cpp:8-11 */
                 int dir = DOWN;
                 for (int h = 0; h < 2; ++h) {
@@ -1748,7 +1958,15 @@ or_sim *or_create(const mpx_desc *desc, char *err, size_t errlen)
                     }
                     c->distance = f->d;
                     inc_dims(c, s->dx, s->dy, s->dz);
-                    if (f->kind == MPX_FACTORY_FOCUSED_PULSE) {
+                    if (f->kind == MPX_FACTORY_GAUSS_INJECT) {
+                        c->src_kind = SRC_GAUSS;
+                        if (f->gsize_t < 2 || !gauss_setup(&c->gauss, f, h, s->N[0], s->N[1], s->dx, s->dy, s->dt,
+                                                           s->low[2], s->high[2])) {
+                            seterr(err, errlen, "gauss_inject: SizeT must be >= 2 (or out of memory)");
+                            or_destroy(s);
+                            return NULL;
+                        }
+                    } else if (f->kind == MPX_FACTORY_FOCUSED_PULSE) {
                         c->src_kind = SRC_FOCUSED;
                         if (!focused_setup(&c->focused, f, s->N[0], s->N[1], s->N[2], s->dx, s->dy, s->dz, s->dt)) {
                             seterr(err, errlen, "out of memory");
@@ -1894,6 +2112,8 @@ void or_destroy(or_sim *s)
         free(s->cur[q].focused.ex);
         free(s->cur[q].focused.bx);
         free(s->cur[q].focused.by);
+        free(s->cur[q].gauss.F1);
+        free(s->cur[q].gauss.F2);
     }
     free(s);
 }

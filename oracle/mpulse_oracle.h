@@ -44,6 +44,11 @@ void    or_pulse_bfunc(const double params[8], int normalise, double x, double y
 double  or_pulse_amp(const double params[8]);
 /* focused-pulse plane: comp 0 = E_x, 1 = B_x, 2 = B_y at distance z from the focus, Yee
    offsets (sx, sy); out = (nx+2)(ny+2) interleaved (re, im) */
+/* The GaussInject sample block in force at step `time` for the E (isH 0) or H (isH 1) sheet:
+   F1, F2 receive Nt x (nx+2) x (ny+2) values ([kh][i][j]); returns Nt, or 0 when the source is
+   inactive at that step or on a bad argument (F1/F2 may be NULL to ask for Nt only). */
+int     or_gauss_block(const mpx_factory *f, int isH, int nx, int ny, double dx, double dy, double dt, int lowz,
+                       int highz, int time, double *F1, double *F2, int *block_toff);
 void    or_focused_plane(const mpx_factory *f, int nx, int ny, double dx, double dy, double z, int comp,
                          double sx, double sy, double *out);
 /* CPML coefficient table of one face current (isH, dir) as built by makeCoeff */

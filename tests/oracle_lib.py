@@ -48,6 +48,8 @@ def load(omp=False):
         "or_pulse_bfunc": (None, [D, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, D]),
         "or_pulse_amp": (C.c_double, [D]),
         "or_cpml_coeff": (C.c_int, [P, C.c_int, C.c_int, D, D, I3, I3]),
+        "or_gauss_block": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, D, D, C.POINTER(C.c_int)]),
         "or_focused_plane": (None, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_double, D]),
     }
@@ -192,3 +194,17 @@ def focused_plane(fac, nx, ny, dx, dy, z, comp, sx, sy):
     lib.or_focused_plane(C.byref(fac), nx, ny, dx, dy, z, comp, sx, sy, _dp(out))
     v = out[0::2] + 1j * out[1::2]
     return v.reshape(nx + 2, ny + 2)
+
+
+def gauss_block(fac, isH, nx, ny, dx, dy, dt, lowz, highz, time):
+    """GaussInject sample block in force at step `time`: (F1, F2 of shape (Nt, nx+2, ny+2),
+    first step of the block), or None when the source is inactive then."""
+    lib = load()
+    nt = lib.or_gauss_block(C.byref(fac), isH, nx, ny, dx, dy, dt, lowz, highz, time, None, None, None)
+    if nt == 0:
+        return None
+    F1 = np.empty((nt, nx + 2, ny + 2))
+    F2 = np.empty((nt, nx + 2, ny + 2))
+    bt = C.c_int()
+    lib.or_gauss_block(C.byref(fac), isH, nx, ny, dx, dy, dt, lowz, highz, time, _dp(F1), _dp(F2), C.byref(bt))
+    return F1, F2, bt.value

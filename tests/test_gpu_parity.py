@@ -199,3 +199,23 @@ def test_fused_block_orders_identical(prec):
             err = rel_l2(ref[n], o.get(n))
             assert err < 1e-12, (n, err)
         o.close()
+
+
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+def test_gauss_inject_against_oracle(path):
+    """gauss_inject (src/gaussinject.cpp) into the one-pole Kerr medium: 14^2 x 20 cells give
+    blocks of Nt = 16 steps; from toff = 2 the sheets take the samples of three blocks (the last
+    reaching past SizeT = 48 steps, i.e. wrapping round the periodic transform) and fall silent
+    at step 50.  The hipFFT transforms against the oracle's direct sums: fields and accumulators
+    to 1e-12."""
+    d, _ = configs.plrc_kerr_pulse(14, 4, 10.0, 4, 1e9, 60, d_src=6, nz=20, path=path)
+    f = abi.factory_defaults(abi.FACTORY_GAUSS_INJECT)
+    f.d, f.gsize_t, f.length, f.width, f.k0, f.gshift, f.gtime, f.amp, f.eps, f.toff = \
+        6, 48, 1.2, 1.1, 2 * np.pi, 0.4, 0.3, 0.8, 1.3, 2
+    d.factories[1] = f
+    g, r, _, _, _ = run_pair(d, 60, FIELDS + ("PsiR1x", "PsiI1y"))
+    assert np.abs(g["Ex"]).max() > 1e-3  # the pulse went in
+    for n in g:
+        err = rel_l2(g[n], r[n])
+        print(n, err)
+        assert err < 1e-12, (n, err)

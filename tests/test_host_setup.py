@@ -60,8 +60,24 @@ def test_setup_errors_follow_the_reference():
         abi.parse_setup("fieldsim { fdtd-plain { } }")
     with pytest.raises(abi.MpxError, match="No field simulation specified"):
         abi.parse_setup("globals { grid-x 8 }")
-    with pytest.raises(abi.MpxError, match="gauss_inject is not provided"):
-        abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { gauss_inject { } } }")
+    with pytest.raises(abi.MpxError, match="side_inject is not provided"):
+        abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { side_inject { } } }")
+
+
+def test_gauss_inject_keys():
+    """gauss_inject takes the keys of src/gaussinject.cpp:33-49 with their defaults and becomes
+    one GaussInject factory."""
+    d, _ = abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { gauss_inject { } } }")
+    f = d.factories[0]
+    assert d.n_factories == 1 and f.kind == abi.FACTORY_GAUSS_INJECT
+    assert (f.d, f.gsize_t, f.length, f.width, f.gshift, f.gtime, f.amp, f.eps, f.toff) == \
+        (15, 4096, 1.0, 1.0, 0.0, 0.0, 1.0, 1.0, 0)
+    assert abs(f.k0 - 2 * math.pi) < 1e-15
+    d, _ = abi.parse_setup("globals { } fieldsim { fdtd-plrc-nonlin { gauss_inject { d 9 SizeT 512 length 2.5 "
+                           "width 3.5 k0 6 Shift 1.25 Time 0.5 amp 0.7 eps 2.25 toff 4 } } }")
+    f = d.factories[0]
+    assert (f.d, f.gsize_t, f.length, f.width, f.k0, f.gshift, f.gtime, f.amp, f.eps, f.toff) == \
+        (9, 512, 2.5, 3.5, 6.0, 1.25, 0.5, 0.7, 2.25, 4)
 
 
 def test_focused_pulse_keys():
