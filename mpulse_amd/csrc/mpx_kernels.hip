@@ -1119,8 +1119,17 @@ __device__ __forceinline__ void batch_tiles(const BatchRun *__restrict__ runs, i
 // The argument block is read from device memory (the solver keeps a copy there): the face
 // tables are indexed at run time, which a by-value kernel argument would have to copy into
 // per-lane scratch.
+// Waves per SIMD the E kernel is compiled for: 6 (at most 80 registers) for the float32 Kerr
+// kernel with one pole and one plane per thread, measured 0.3 % faster than the 5 its registers
+// allow; no constraint otherwise (other variants would spill).
+template <typename T, int NP, bool NONLIN, int L>
+struct FusedEWaves {
+    static constexpr int v = (sizeof(T) == 4 && NP <= 1 && NONLIN && L == 1) ? 6 : 1;
+};
+
 template <typename T, int NP, bool NONLIN, bool CPML, int L>
-__global__ __launch_bounds__(256) void k_fused_E(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
+__global__ __attribute__((amdgpu_waves_per_eu(FusedEWaves<T, NP, NONLIN, L>::v))) __launch_bounds__(256) void k_fused_E(
+    const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
     fused_tile_of(pa->order, pa->band, nbx, nby, nbz, tbx, tby, tbz);

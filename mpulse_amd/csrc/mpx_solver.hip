@@ -970,6 +970,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     if (rc) return rc;
                 }
             }
+        // the 64x16 sweep block (at most 128 registers) spills with two or three poles; 64x8 does not
+        if (s->np >= 2) s->sweep_variant = 0;
         double sigma = 0.5 * 0.0 * desc->dt;  // no OptField writes Sigma on this path
         if (s->nonlin)
             s->denom = desc->eps + s->sumChi0 - s->sumXi0 + sigma;
