@@ -544,6 +544,11 @@ static void add_fielddiag(void *owner, Rebuildable *r)
     static_cast<FieldSimulation *>(owner)->fieldDiags.push_back(static_cast<FieldDiag *>(r));
 }
 
+static void add_otherdiag(void *owner, Rebuildable *r)
+{
+    static_cast<FieldSimulation *>(owner)->otherDiags.push_back(static_cast<SimpleFieldDiag *>(r));
+}
+
 static void set_solver(void *owner, Rebuildable *r)
 {
     auto *fs = static_cast<FieldSimulation *>(owner);
@@ -591,18 +596,25 @@ void FieldSimulation::MakeParamMap(ParamMap &pm)
     pm["mpi_block_periodic"] = obj(&make_new<SinglePeriodicBoundary>, set_boundary);
     // diagnostics registry, src/fieldsim.cpp:169-171
     pm["fielddiag"] = obj(&make_new<FieldDiag>, add_fielddiag);
+    pm["slicediag"] = obj(&make_new<FieldSliceDiag>, add_otherdiag);
+    pm["linediag"] = obj(&make_new<FieldLineDiag>, add_otherdiag);
+    pm["energy"] = obj(&make_new<FieldEnergyDiag>, add_otherdiag);
 }
 
 FieldSimulation::~FieldSimulation()
 {
     for (FieldDiag *d : fieldDiags) delete d;  // each finishes its writes first
+    for (SimpleFieldDiag *d : otherDiags) delete d;
     delete solver;
     delete boundary;
 }
 
 void FieldSimulation::executeDiagnostics()
 {
+    // DiagnosticManager::execute (src/process.cpp:29); each diagnostic writes its own files,
+    // so the order among them does not show in the outputs
     for (FieldDiag *d : fieldDiags) d->execute();
+    for (SimpleFieldDiag *d : otherDiags) d->execute();
 }
 
 void FieldSimulation::finishDiagnostics()
@@ -611,6 +623,205 @@ void FieldSimulation::finishDiagnostics()
         d->finish();
         if (!d->error.empty()) throw SetupError(d->error);
     }
+    for (SimpleFieldDiag *d : otherDiags) {
+        d->finish();
+        if (!d->error.empty()) throw SetupError(d->error);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// text and slice diagnostics
+// ---------------------------------------------------------------------------------------
+
+SimpleFieldDiag::~SimpleFieldDiag() { close(); }
+
+void SimpleFieldDiag::MakeParamMap(ParamMap &pm)
+{
+    // DiagnosticInterface::MakeParamMap, src/diagnostic.cpp:34-47
+    pm["file"] = pstring(&fname, "");
+    pm["append"] = pstring(&append, "n");
+    pm["interval"] = pint(&interval, 100);
+}
+
+void SimpleFieldDiag::init(FieldSolver *solver, Globals *g, int *time_)
+{
+    gpu = solver->handle();
+    globals = g;
+    time = time_;
+    if (interval < 1) throw SetupError("Diagnostic: interval must be positive");
+}
+
+std::string SimpleFieldDiag::parsedFileName() const
+{
+    // DiagnosticInterface::parsedFileName, src/diagnostic.cpp:53-72 (one process: id 0)
+    std::string parsed = fname;
+    size_t pos = parsed.find("#p");
+    if (pos != std::string::npos) parsed.replace(pos, 2, "0");
+    pos = parsed.find("#t");
+    if (pos != std::string::npos) parsed.replace(pos, 2, std::to_string(time ? *time : 0));
+    return parsed;
+}
+
+bool SimpleFieldDiag::open(const std::string &path)
+{
+    close();
+    out = new std::ofstream(path.c_str());
+    if (!*out) {
+        if (error.empty()) error = "Diagnostic: cannot open " + path;
+        return false;
+    }
+    return true;
+}
+
+void SimpleFieldDiag::close()
+{
+    if (out) {
+        out->close();
+        delete out;
+        out = nullptr;
+    }
+}
+
+void SimpleFieldDiag::finish()
+{
+    if (out) out->flush();
+}
+
+std::vector<double> SimpleFieldDiag::grid(const std::string &name)
+{
+    std::vector<double> v((size_t)(globals->GridX + 2) * (globals->GridY + 2) * (globals->GridZ + 2));
+    if (mpx_get_field(gpu, name.c_str(), v.data(), v.size()) != MPX_OK && error.empty())
+        error = std::string("Diagnostic: ") + mpx_last_error();
+    return v;
+}
+
+void SimpleFieldDiag::execute()
+{
+    // DiagnosticInterface::execute, src/diagnostic.cpp:12-31
+    if (!gpu) return;
+    const bool appending = !append.empty() && append[0] == 'y';
+    if (t == 0 && appending && active()) open(parsedFileName());
+    if (t % interval == 0 && active()) {
+        bool ok = appending || open(parsedFileName());
+        if (ok) write();
+        if (!appending) close();
+    }
+    ++t;
+}
+
+void FieldEnergyDiag::write()
+{
+    // FieldEnergyDiag::calculate / write, src/fielddiag.cpp:244-286
+    double energy = 0;
+    if (mpx_energy(gpu, &energy) != MPX_OK && error.empty()) error = std::string("Diagnostic: ") + mpx_last_error();
+    if (out) *out << *time << " " << energy << std::endl;
+}
+
+void FieldLineDiag::MakeParamMap(ParamMap &pm)
+{
+    // FieldLineDiag::MakeParamMap, src/fielddiag.cpp:141-153
+    SimpleFieldDiag::MakeParamMap(pm);
+    pm["field"] = pstring(&fieldId, "");
+    pm["dim"] = pstring(&direction, "x");
+    pm["posx"] = pint(&posx, 5);
+    pm["posy"] = pint(&posy, 5);
+    pm["posz"] = pint(&posz, 5);
+    pm["format"] = pstring(&format, "lines");
+}
+
+static int line_dim(const std::string &d) { return d.empty() ? 0 : (d[0] == 'y' ? 1 : (d[0] == 'z' ? 2 : 0)); }
+
+bool FieldLineDiag::active() const
+{
+    // FieldLineDiag::fetchField, src/fielddiag.cpp:155-192: the line lies strictly inside the
+    // storage box in both transverse dimensions
+    const int dim = line_dim(direction), t1 = dim == 0 ? 1 : 0, t2 = dim == 2 ? 1 : 2;
+    const int pos[3] = {posx, posy, posz}, high[3] = {globals->GridX + 1, globals->GridY + 1, globals->GridZ + 1};
+    return pos[t1] > 0 && pos[t1] < high[t1] && pos[t2] > 0 && pos[t2] < high[t2];
+}
+
+bool FieldLineDiag::open(const std::string &path)
+{
+    if (!SimpleFieldDiag::open(path)) return false;
+    out->precision(10);
+    return true;
+}
+
+void FieldLineDiag::write()
+{
+    // FieldLineDiag::write, src/fielddiag.cpp:194-217
+    if (!out) return;
+    const int dim = line_dim(direction);
+    const int ny2 = globals->GridY + 2, nz2 = globals->GridZ + 2, high = (dim == 0 ? globals->GridX : dim == 1 ? globals->GridY : globals->GridZ) + 1;
+    std::vector<double> f = grid(fieldId);
+    int i[3] = {posx, posy, posz};
+    if (format != "gnu") {
+        std::string sep;
+        for (i[dim] = 0; i[dim] <= high; ++i[dim]) {
+            *out << sep << f[((size_t)i[0] * ny2 + i[1]) * nz2 + i[2]];
+            sep = " ";
+        }
+        *out << std::endl;
+    } else {
+        for (i[dim] = 0; i[dim] <= high; ++i[dim]) *out << i[dim] << ' ' << f[((size_t)i[0] * ny2 + i[1]) * nz2 + i[2]] << std::endl;
+        *out << std::endl;
+    }
+}
+
+FieldSliceDiag::~FieldSliceDiag() { close(); }
+
+void FieldSliceDiag::MakeParamMap(ParamMap &pm)
+{
+    // FieldSliceDiag::MakeParamMap, src/fielddiag.cpp:45-53
+    SimpleFieldDiag::MakeParamMap(pm);
+    pm["field"] = pstring(&fieldId, "");
+    pm["plane"] = pstring(&plane, "x");
+    pm["pos"] = pint(&pos, 5);
+}
+
+bool FieldSliceDiag::active() const
+{
+    // FieldSliceDiag::fetchField, src/fielddiag.cpp:55-93: the plane lies strictly inside
+    const int normal = line_dim(plane);
+    const int high = (normal == 0 ? globals->GridX : normal == 1 ? globals->GridY : globals->GridZ) + 1;
+    return pos > 0 && pos < high;
+}
+
+bool FieldSliceDiag::open(const std::string &path)
+{
+    close();
+    h5 = new HDFostream;
+    if (!h5->open(path, 256)) {
+        if (error.empty()) error = "Diagnostic: " + h5->error();
+        delete h5;
+        h5 = nullptr;
+        return false;
+    }
+    return true;
+}
+
+void FieldSliceDiag::close()
+{
+    if (h5) {
+        h5->close();
+        delete h5;
+        h5 = nullptr;
+    }
+}
+
+void FieldSliceDiag::write()
+{
+    // FieldSliceDiag::write, src/fielddiag.cpp:95-118: slice(d1, d2) over the storage box of
+    // the two in-plane dimensions (d1 < d2), streamed with d1 fastest
+    if (!h5) return;
+    const int normal = line_dim(plane), d1 = normal == 0 ? 1 : 0, d2 = normal == 2 ? 1 : 2;
+    const int n[3] = {globals->GridX + 2, globals->GridY + 2, globals->GridZ + 2};
+    std::vector<double> f = grid(fieldId), s((size_t)n[d1] * n[d2]);
+    int i[3];
+    i[normal] = pos;
+    for (i[d2] = 0; i[d2] < n[d2]; ++i[d2])
+        for (i[d1] = 0; i[d1] < n[d1]; ++i[d1]) s[(size_t)i[d2] * n[d1] + i[d1]] = f[((size_t)i[0] * n[1] + i[1]) * n[2] + i[2]];
+    if (!h5->write({(uint64_t)n[d2], (uint64_t)n[d1]}, s.data()) && error.empty()) error = "Diagnostic: " + h5->error();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -740,6 +951,7 @@ void FieldSimulation::init(Globals *g, int *time)
     storage.reset(new Storage(solver));
     solver->initStorage(storage.get());
     for (FieldDiag *d : fieldDiags) d->init(solver, g, time);  // FieldDiag::fetchField
+    for (SimpleFieldDiag *d : otherDiags) d->init(solver, g, time);
 }
 
 void FieldSimulation::execute() { solver->stepScheme(dt); }

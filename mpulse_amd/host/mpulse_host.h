@@ -8,6 +8,7 @@
 // defaults.  The solvers keep their state on the GPU; stepScheme calls mpx_step.
 #pragma once
 
+#include <fstream>
 #include <iosfwd>
 #include <list>
 #include <map>
@@ -329,6 +330,74 @@ class FieldDiag : public Rebuildable {
     void fail(const std::string &m);  // keeps the first failure (called from the writer thread too)
 };
 
+// Text and slice diagnostics with DiagnosticInterface's keys and rules (src/diagnostic.cpp:
+// 12-72): "file" (#p, #t), "append", "interval"; output at the 0th call and every `interval`
+// calls after; one file per output, or one file when appending.  The grids are read when an
+// output is due (the reference computes every step and writes at the outputs: same values).
+class SimpleFieldDiag : public Rebuildable {
+  public:
+    ~SimpleFieldDiag() override;
+    std::string fname, append;
+    int interval = 100;
+    void init(FieldSolver *solver, Globals *g, int *time);
+    void execute();
+    void finish();
+    std::string parsedFileName() const;
+    std::string error;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    virtual bool active() const { return true; }
+    virtual bool open(const std::string &path);  // text stream by default
+    virtual void write() = 0;
+    virtual void close();
+    std::vector<double> grid(const std::string &name);  // storage box, z fastest
+    mpx_solver *gpu = nullptr;
+    Globals *globals = nullptr;
+    int *time = nullptr;
+    std::ofstream *out = nullptr;
+    int t = 0;
+};
+
+// FieldEnergyDiag (src/fielddiag.cpp:219-286): "time energy" lines, energy = sum over the
+// interior of E^2 + B^2 times dx dy dz
+class FieldEnergyDiag : public SimpleFieldDiag {
+  protected:
+    void write() override;
+};
+
+// FieldLineDiag (src/fielddiag.cpp:122-217): keys "field", "dim", "posx", "posy", "posz",
+// "format" (lines | gnu); precision 10
+class FieldLineDiag : public SimpleFieldDiag {
+  public:
+    std::string fieldId, direction, format;
+    int posx = 5, posy = 5, posz = 5;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    bool active() const override;
+    bool open(const std::string &path) override;
+    void write() override;
+};
+
+// FieldSliceDiag (src/fielddiag.cpp:26-118): keys "field", "plane" (x | y | z), "pos"; one
+// float64 dataset per output through HDFostream, HDF5 dims (N2+2, N1+2) of the two in-plane
+// dimensions d1 < d2, d1 fastest
+class FieldSliceDiag : public SimpleFieldDiag {
+  public:
+    ~FieldSliceDiag() override;
+    std::string fieldId, plane;
+    int pos = 5;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    bool active() const override;
+    bool open(const std::string &path) override;
+    void write() override;
+    void close() override;
+    HDFostream *h5 = nullptr;
+};
+
 class FieldSimulation : public Rebuildable {  // src/fieldsim.cpp:31-194
   public:
     ~FieldSimulation() override;
@@ -336,6 +405,7 @@ class FieldSimulation : public Rebuildable {  // src/fieldsim.cpp:31-194
     Boundary *boundary = nullptr;
     std::unique_ptr<Storage> storage;
     std::vector<FieldDiag *> fieldDiags;  // "fielddiag" blocks, in setup order
+    std::vector<SimpleFieldDiag *> otherDiags;  // "slicediag", "linediag", "energy" blocks
     void init(Globals *g, int *time);
     void execute();
     void executeDiagnostics();  // DiagnosticManager::execute for the field diagnostics

@@ -102,3 +102,7 @@ def test_fielddiag_keys_parse():
     d1, t1 = abi.parse_setup(base % "fielddiag { field Ex file out_#p_#t.h5 append n interval 2 }")
     assert t0 == t1 == 3
     assert bytes(d0) == bytes(d1)
+    d2, _ = abi.parse_setup(base % ("energy { file e.dat interval 5 append y } "
+                                    "linediag { field Ey file l.dat dim y posx 3 posz 4 format gnu } "
+                                    "slicediag { field Bz file s_#t.h5 plane z pos 2 }"))
+    assert bytes(d0) == bytes(d2)
