@@ -232,7 +232,9 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
    tile row per thread, the last row and lane of a tile being its halo).  "tile_order" = block order
    of the fused kernels (0: z tiles fastest, 1: x chunks fastest, 2, the default: bands of
    "tile_band" tile rows walked x plane by x plane; "tile_band" 0, the default, chooses the
-   height from the grid).  No counterpart in the reference: the hook of
+   height from the grid); "b_reverse" 1, the default, walks that order backwards in the B-side
+   kernel, so it starts on the tiles the E-side kernel wrote last (0: forwards).  No
+   counterpart in the reference: the hook of
    the tile autotuner. */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
 

@@ -761,14 +761,16 @@ __device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int
 // Banded order: the tile rows are cut into bands of R rows; inside a band the z tiles run
 // fastest, then the rows of the band, then the x chunks, so the x-neighbour plane of a tile
 // was read a band-plane (nbx * R tiles) earlier and is still in the XCD's L2.  Each XCD
-// takes a contiguous share of the order as in xcd_tile.
-__device__ __forceinline__ void band_tile(int nbx, int nby, int nbz, int R, int &bx, int &by, int &bz)
+// takes a contiguous share of the order as in xcd_tile.  With rev the order runs from its end,
+// which the B-side kernel uses to begin on the tiles the E-side kernel finished last.
+__device__ __forceinline__ void band_tile(int nbx, int nby, int nbz, int R, int rev, int &bx, int &by, int &bz)
 {
     unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     unsigned total = (unsigned)nbx * nby * nbz;
     unsigned per = total / 8u;
     unsigned xcd = lin & 7u, idx = lin >> 3;
     unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
+    if (rev) t = total - 1u - t;
     const unsigned full = (unsigned)nbx * R * nbz;  // tiles of a full band; only the last is shorter
     const unsigned b = t / full;
     unsigned rem = t - b * full;
@@ -779,11 +781,11 @@ __device__ __forceinline__ void band_tile(int nbx, int nby, int nbz, int R, int 
     bz = rem / rows;
 }
 
-__device__ __forceinline__ void fused_tile_of(int order, int band, int nbx, int nby, int nbz, int &bx, int &by,
-                                              int &bz)
+__device__ __forceinline__ void fused_tile_of(int order, int band, int rev, int nbx, int nby, int nbz, int &bx,
+                                              int &by, int &bz)
 {
     if (order == 2)
-        band_tile(nbx, nby, nbz, band, bx, by, bz);
+        band_tile(nbx, nby, nbz, band, rev, bx, by, bz);
     else if (order == 1)
         xcd_tile(nbz, nbx, nby, bz, bx, by);
     else
@@ -1132,7 +1134,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(FusedEWaves<T, NP, NONLIN, L>::v))
     const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    fused_tile_of(pa->order, pa->band, nbx, nby, nbz, tbx, tby, tbz);
+    fused_tile_of(pa->order, pa->band, pa->rev, nbx, nby, nbz, tbx, tby, tbz);
     fused_E_tile<T, NP, NONLIN, CPML, L>(*pa, tbx, tby, tbz);
 }
 
@@ -1140,7 +1142,7 @@ template <typename T, bool CPML, int L>
 __global__ __launch_bounds__(256) void k_fused_B(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
-    fused_tile_of(pa->order, pa->band, nbx, nby, nbz, tbx, tby, tbz);
+    fused_tile_of(pa->order, pa->band, pa->rev, nbx, nby, nbz, tbx, tby, tbz);
     fused_B_tile<T, CPML, L>(*pa, tbx, tby, tbz);
 }
 

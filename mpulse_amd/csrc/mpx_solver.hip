@@ -154,6 +154,7 @@ struct mpx_solver {
     int tile_order = 2;            // fused kernels: block order, 0 = z tiles fastest, 1 = x chunks fastest,
                                    // 2 = bands of tile_band rows walked plane by plane (fastest)
     int tile_band = 0;             // 0: chosen from the grid by auto_band
+    int b_reverse = 1;             // B kernel walks the banded order backwards
     double *d_stage = nullptr;     // x-fastest float64 copy of one grid for mpx_fetch_field
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
@@ -1102,6 +1103,7 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
     a.march = hside ? s->march_b : s->march_e;
     a.order = s->tile_order;
     a.band = s->tile_band > 0 ? s->tile_band : auto_band(s->g, hside ? a.blo : nullptr, hside);
+    a.rev = hside ? s->b_reverse : 0;
     return a;
 }
 
@@ -2047,6 +2049,9 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
     else if (!std::strcmp(key, "tile_order")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "tile_order must be 0, 1 or 2");
         s->tile_order = value;
+    } else if (!std::strcmp(key, "b_reverse")) {
+        if (value != 0 && value != 1) return fail(MPX_ERR_INVALID, "b_reverse must be 0 or 1");
+        s->b_reverse = value;
     } else if (!std::strcmp(key, "tile_band")) {
         if (value < 0) return fail(MPX_ERR_INVALID, "tile_band must be >= 0 (0: chosen from the grid)");
         s->tile_band = value;

@@ -139,6 +139,7 @@ struct StepArgs {
     int order;                  // fused path: 0 = z tiles fastest in the block order, 1 = x chunks fastest,
                                 // 2 = bands of `band` tile rows, walked plane by plane
     int band;
+    int rev;                    // fused path: walk the block order backwards (B side)
 };
 
 // Single-pass sweep path: "o" = buffers of step n (read), "n" = buffers of step n+1 (written).

@@ -667,7 +667,11 @@ bool SimpleFieldDiag::open(const std::string &path)
     close();
     out = new std::ofstream(path.c_str());
     if (!*out) {
-        if (error.empty()) error = "Diagnostic: cannot open " + path;
+        // the reference's std::ofstream stays in its failed state and drops the writes (an
+        // empty "file", the default, gives no output); warn once instead of failing the run
+        if (!warned) std::fprintf(stderr, "Diagnostic: cannot open '%s'; its output is dropped\n", path.c_str());
+        warned = true;
+        close();
         return false;
     }
     return true;

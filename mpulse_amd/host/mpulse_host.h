@@ -357,6 +357,7 @@ class SimpleFieldDiag : public Rebuildable {
     int *time = nullptr;
     std::ofstream *out = nullptr;
     int t = 0;
+    bool warned = false;
 };
 
 // FieldEnergyDiag (src/fielddiag.cpp:219-286): "time energy" lines, energy = sum over the

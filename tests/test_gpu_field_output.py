@@ -106,6 +106,8 @@ fieldsim {
   linediag { field Ex file DIR/line_#t.dat dim z posx 6 posy 7 interval 4 }
   linediag { field By file DIR/gnu.dat dim x posy 7 posz 6 format gnu interval 7 append y }
   linediag { field Ez file DIR/never.dat dim y posx 0 posz 3 interval 1 }
+  energy { file DIR/missing_dir/e.dat interval 2 }
+  energy { interval 2 }
   slicediag { field Ex file DIR/slice_#t.h5 plane y pos 7 interval 4 }
   slicediag { field Bz file DIR/slices.h5 plane z pos 6 interval 3 append y }
 }
@@ -114,7 +116,8 @@ fieldsim {
 
 def test_energy_line_and_slice_diagnostics(tmp_path):
     """energy (src/fielddiag.cpp:219-286), linediag (:122-217) and slicediag (:26-118) through the
-    host driver: files per output or appended, '#t' names, inactive lines write nothing;
+    host driver: files per output or appended, '#t' names, inactive lines write nothing, a text
+    output that cannot be opened (no "file", a missing directory) writes nothing and the run goes on;
     contents equal the grids after those steps (cycles 0, 3, 6 / 0, 4 / 0, 7)."""
     print("case: energy / line / slice diagnostics", flush=True)
     text = SETUP2.replace("DIR", str(tmp_path))

@@ -178,7 +178,8 @@ def test_fused_block_orders_identical(prec):
     d.factories[1].tshift = 12.0
     shapes = [dict(tile_order=0), dict(tile_order=1), dict(tile_order=2, tile_band=0),
               dict(tile_order=2, tile_band=3), dict(tile_order=2, tile_band=7),
-              dict(tile_order=2, tile_band=1000), dict(tile_order=2, tile_band=0, march_e=2, march_b=4)]
+              dict(tile_order=2, tile_band=1000), dict(tile_order=2, tile_band=0, march_e=2, march_b=4),
+              dict(tile_order=2, tile_band=0, b_reverse=0), dict(tile_order=0, b_reverse=1)]
     ref = None
     for sh in shapes:
         print(f"case: prec {prec} launch shape {sh}", flush=True)
