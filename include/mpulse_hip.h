@@ -19,8 +19,10 @@
  *   mpx_get_field     Storage::getGrid / getBorderLayer     src/storage.cpp:48-90
  *   mpx_set_field     (same grids, written by initialisers, e.g. src/waveinit.cpp:9-60)
  *   mpx_energy        FieldEnergyDiag::calculate            src/fielddiag.cpp:255-286
- *   mpx_set_probes /  point histories (the reference has no point-probe diagnostic;
- *   mpx_get_probes    FieldLineDiag src/fielddiag.cpp:122-223 is the nearest)
+ *   mpx_set_probes /  interior point histories of E and B, recorded inside every step
+ *   mpx_get_probes    (no reference counterpart)
+ *   mpx_sampler_*     the per-step DataGrid reads of FrequencyDiag::calculate and
+ *                     FollowMax::calculate     src/freqdiag.cpp:45-70, src/followmax.cpp:73-113
  *   mpx_last_error    std::cerr + exit(-1) error reporting  (e.g. src/fdtd_plrc.t:20-24)
  *
  * Grid convention (src/globals.cpp:44-45): interior cells are 1..N in each dimension,
@@ -216,6 +218,23 @@ int  mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_
    comp: 0..2 = Ex,Ey,Ez, 3..5 = Bx,By,Bz.  Capacity = max steps recorded. */
 int  mpx_set_probes(mpx_solver *s, int nprobes, const int *ijk, const int *comp, int capacity);
 int  mpx_get_probes(mpx_solver *s, double *host, int *nrecorded);
+
+/* Samplers: the per-step reads of FrequencyDiag::calculate (one cell, src/freqdiag.cpp:45-70)
+   and FollowMax::calculate (a storage line, src/followmax.cpp:73-113), which read
+   DataGrid(x, y, z) every step.  A sampler of grid `name` records the value at storage cell
+   pos (dim = -1) or the N[dim] + 2 values of the storage line through pos along dim (0..2),
+   including ghosts, once per mpx_sampler_record, in step order, without waiting for the GPU;
+   mpx_sampler_drain waits and copies the records made since the last drain (nrecords x
+   values per record, float64) and empties the sampler.  Recording into a full sampler
+   (capacity records pending) is an error.  pos must lie in the storage box 0..N+1.  E and B
+   are gathered on the device; other grids are read through mpx_get_field.  Destroy a
+   sampler before its solver. */
+typedef struct mpx_sampler mpx_sampler;
+int  mpx_sampler_create(mpx_solver *s, const char *name, int dim, const int pos[3], int capacity,
+                        mpx_sampler **out);
+int  mpx_sampler_record(mpx_sampler *p);
+int  mpx_sampler_drain(mpx_sampler *p, double *host, size_t count, int *nrecords);
+int  mpx_sampler_destroy(mpx_sampler *p);
 
 /* sum over interior cells of (E^2 + B^2) * dx*dy*dz  (src/fielddiag.cpp:255-286) */
 int  mpx_energy(mpx_solver *s, double *energy);

@@ -147,6 +147,10 @@ def load():
         "mpx_set_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t]),
         "mpx_fetch_field": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.c_size_t, C.POINTER(P)]),
         "mpx_fetch_wait": (C.c_int, [P]),
+        "mpx_sampler_create": (C.c_int, [P, C.c_char_p, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(P)]),
+        "mpx_sampler_record": (C.c_int, [P]),
+        "mpx_sampler_drain": (C.c_int, [P, C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_int)]),
+        "mpx_sampler_destroy": (C.c_int, [P]),
         "mpx_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(P)]),
         "mpx_host_free": (C.c_int, [P]),
         "mpx_get_layer": (C.c_int, [P, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_size_t,
@@ -201,6 +205,42 @@ class FetchHandle:
             rc = self.solver.lib.mpx_fetch_wait(self.h)
             self.h = None
             self.solver._check(rc)
+
+
+class Sampler:
+    """An mpx_sampler: records grid `name` at one storage cell (dim -1) or along the storage line
+    through `pos` in dimension dim, once per record(); drain() returns the records made since
+    the last drain, shape (records, values per record)."""
+
+    def __init__(self, solver, name, dim, pos, capacity):
+        self.solver, self.h = solver, None
+        self.len = 1 if dim < 0 else solver.shape[dim]
+        self.capacity = capacity
+        p = (C.c_int * 3)(*[int(v) for v in pos])
+        h = C.c_void_p()
+        solver._check(solver.lib.mpx_sampler_create(solver.h, name.encode(), dim, p, capacity, C.byref(h)))
+        self.h = h
+
+    def record(self):
+        self.solver._check(self.solver.lib.mpx_sampler_record(self.h))
+
+    def drain(self):
+        out = np.empty(self.capacity * self.len, dtype=np.float64)
+        n = C.c_int()
+        self.solver._check(self.solver.lib.mpx_sampler_drain(self.h, _dptr(out), out.size, C.byref(n)))
+        return out[: n.value * self.len].reshape(n.value, self.len)
+
+    def close(self):
+        if self.h is not None:
+            rc = self.solver.lib.mpx_sampler_destroy(self.h)
+            self.h = None
+            self.solver._check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class PinnedArray:
@@ -306,6 +346,10 @@ class Solver:
         out = np.empty(self.shape[::-1], dtype=np.float64)
         self.fetch_async(name, out).wait()
         return out
+
+    def sampler(self, name, dim, pos, capacity):
+        """A Sampler of grid `name` (close it before the solver)."""
+        return Sampler(self, name, dim, pos, capacity)
 
     def set(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)

@@ -1549,6 +1549,35 @@ void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t
 template void launch_pack_xfast<float>(const GridDims &, const float *, double *, hipStream_t);
 template void launch_pack_xfast<double>(const GridDims &, const double *, double *, hipStream_t);
 
+// One record of a sampler: the value at storage cell c (dim < 0) or the len = N[dim] + 2 values
+// of the storage line through c along dim, as float64.  A coordinate of a dimension in
+// wrapmask that names a ghost reads its periodic image (0 -> N, N + 1 -> 1), which is what the
+// ghost holds once the wrap has run; the caller checks 0 <= c[d] <= N[d] + 1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_gather(GridDims g, const T *__restrict__ src, int dim, int c0, int c1,
+                                                      int c2, int len, int wrapmask, double *__restrict__ dst)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= len) return;
+    int c[3] = {c0, c1, c2};
+    if (dim >= 0) c[dim] = q;
+    for (int d = 0; d < 3; ++d)
+        if ((wrapmask >> d) & 1) c[d] = c[d] == 0 ? g.n[d] : (c[d] == g.n[d] + 1 ? 1 : c[d]);
+    dst[q] = (double)src[g.at(c[0], c[1], c[2])];
+}
+
+template <typename T>
+void launch_sample_gather(const GridDims &g, const T *src, int dim, const int *c, int len, int wrapmask, double *dst,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sample_gather<T>, dim3((len + 255) / 256), dim3(256), 0, s, g, src, dim, c[0], c[1], c[2], len,
+                       wrapmask, dst);
+}
+template void launch_sample_gather<float>(const GridDims &, const float *, int, const int *, int, int, double *,
+                                         hipStream_t);
+template void launch_sample_gather<double>(const GridDims &, const double *, int, const int *, int, int, double *,
+                                          hipStream_t);
+
 }  // namespace mpx
 
 namespace mpx {

@@ -83,6 +83,9 @@ void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
 template <typename T>
 void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s);
 template <typename T>
+void launch_sample_gather(const GridDims &g, const T *src, int dim, const int *c, int len, int wrapmask, double *dst,
+                         hipStream_t s);
+template <typename T>
 void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s);
 
 }  // namespace mpx

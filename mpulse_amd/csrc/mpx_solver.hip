@@ -1924,6 +1924,128 @@ int mpx_get_probes(mpx_solver *s, double *host, int *nrecorded)
     return MPX_OK;
 }
 
+}  // extern "C"
+
+// A sampler: values of one grid at a storage cell or along a storage line, one record per
+// mpx_sampler_record call, kept in a device ring of `capacity` records until drained.  Records
+// of E and B are gathered on the device with the ghost images the wrap would give; other grids
+// go through mpx_get_field on the host (their ghosts are assembled there).
+struct mpx_sampler {
+    mpx_solver *s = nullptr;
+    std::string name;
+    int dim = -1, pos[3] = {0, 0, 0}, len = 1, capacity = 0, pending = 0;
+    bool device = false;         // a grid of E or B, gathered on the device
+    int wrapmask = 0;
+    double *ring = nullptr;      // device, capacity * len
+    std::vector<double> host;    // records of the host-read grids
+};
+
+extern "C" {
+
+int mpx_sampler_create(mpx_solver *s, const char *name, int dim, const int pos[3], int capacity, mpx_sampler **out)
+{
+    if (!s || !name || !pos || !out) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_NOT_IN_BATCH(s);
+    if (dim < -1 || dim > 2) return fail(MPX_ERR_INVALID, "sampler dim must be -1 (cell) or 0..2 (line)");
+    for (int d = 0; d < 3; ++d)
+        if (pos[d] < 0 || pos[d] > s->g.n[d] + 1) return fail(MPX_ERR_INVALID, "sampler cell outside the storage box 0..N+1");
+    const int len = dim < 0 ? 1 : s->g.n[dim] + 2;
+    if (capacity < 1 || (size_t)capacity * len > ((size_t)1 << 27))
+        return fail(MPX_ERR_INVALID, "sampler capacity must be 1.. with capacity * values per record <= 2^27");
+    void *ptr;
+    size_t elem;
+    bool isPsi;
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
+    if (rc) return fail(rc, std::string("unknown grid ") + name);
+    std::unique_ptr<mpx_sampler> p(new mpx_sampler);
+    p->s = s;
+    p->name = name;
+    p->dim = dim;
+    for (int d = 0; d < 3; ++d) p->pos[d] = pos[d];
+    p->len = len;
+    p->capacity = capacity;
+    for (int c = 0; c < 3 && ptr && !isPsi; ++c)
+        if (ptr == s->E[c] || ptr == s->B[c]) p->device = true;
+    if (p->device) {
+        // the fused and sweep paths of the PLRC solvers leave the ghosts of E and B stale
+        if (s->plrc && s->path != MPX_PATH_REFERENCE)
+            for (int d = 0; d < 3; ++d) p->wrapmask |= s->periodic[d] ? 1 << d : 0;
+        if (hipMalloc((void **)&p->ring, (size_t)capacity * len * sizeof(double)) != hipSuccess)
+            return fail(MPX_ERR_NOMEM, "hipMalloc of the sampler ring failed");
+    }
+    *out = p.release();
+    return MPX_OK;
+}
+
+int mpx_sampler_record(mpx_sampler *p)
+{
+    if (!p) return fail(MPX_ERR_INVALID, "null sampler");
+    mpx_solver *s = p->s;
+    MPX_NOT_IN_BATCH(s);
+    if (p->pending >= p->capacity) return fail(MPX_ERR_INVALID, "sampler full: drain it first");
+    if (p->device) {
+        // looked up at every record: the sweep path swaps its two sets of E and B after each step
+        void *grid;
+        size_t elem;
+        bool isPsi;
+        int part = 0;
+        if (find_grid(s, p->name.c_str(), &grid, &elem, &isPsi, &part) || !grid)
+            return fail(MPX_ERR_INVALID, "sampler grid is gone");
+        double *dst = p->ring + (size_t)p->pending * p->len;
+        if (elem == 8)
+            mpx::launch_sample_gather<double>(s->g, (const double *)grid, p->dim, p->pos, p->len, p->wrapmask, dst,
+                                              s->stream);
+        else
+            mpx::launch_sample_gather<float>(s->g, (const float *)grid, p->dim, p->pos, p->len, p->wrapmask, dst,
+                                             s->stream);
+        MPX_HIP(hipGetLastError());
+    } else {
+        const GridDims &g = s->g;
+        const size_t ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
+        std::vector<double> f((size_t)(g.n[0] + 2) * ny2 * nz2);
+        int rc = mpx_get_field(s, p->name.c_str(), f.data(), f.size());
+        if (rc) return rc;
+        int c[3] = {p->pos[0], p->pos[1], p->pos[2]};
+        for (int q = 0; q < p->len; ++q) {
+            if (p->dim >= 0) c[p->dim] = q;
+            p->host.push_back(f[((size_t)c[0] * ny2 + c[1]) * nz2 + c[2]]);
+        }
+    }
+    p->pending++;
+    return MPX_OK;
+}
+
+int mpx_sampler_drain(mpx_sampler *p, double *host, size_t count, int *nrecords)
+{
+    if (!p || !nrecords || (!host && p->pending)) return fail(MPX_ERR_INVALID, "null argument");
+    const size_t n = (size_t)p->pending * p->len;
+    if (count < n) return fail(MPX_ERR_INVALID, "buffer smaller than the pending records");
+    if (n && p->device) {
+        MPX_HIP(hipMemcpyAsync(host, p->ring, n * sizeof(double), hipMemcpyDeviceToHost, p->s->stream));
+        MPX_HIP(hipStreamSynchronize(p->s->stream));
+    } else if (n) {
+        std::memcpy(host, p->host.data(), n * sizeof(double));
+    }
+    p->host.clear();
+    *nrecords = p->pending;
+    p->pending = 0;
+    return MPX_OK;
+}
+
+int mpx_sampler_destroy(mpx_sampler *p)
+{
+    if (!p) return MPX_OK;
+    int rc = MPX_OK;
+    if (p->ring) {
+        // records still queued on the solver's stream write into the ring
+        if (hipStreamSynchronize(p->s->stream) != hipSuccess || hipFree(p->ring) != hipSuccess)
+            rc = fail(MPX_ERR_HIP, "freeing the sampler ring failed");
+    }
+    delete p;
+    return rc;
+}
+
 int mpx_energy(mpx_solver *s, double *energy)
 {
     if (!s || !energy) return fail(MPX_ERR_INVALID, "null argument");

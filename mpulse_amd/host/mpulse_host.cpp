@@ -3,6 +3,7 @@
 #include "mpulse_host.h"
 #include "mpx_hdf5.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -599,6 +600,8 @@ void FieldSimulation::MakeParamMap(ParamMap &pm)
     pm["slicediag"] = obj(&make_new<FieldSliceDiag>, add_otherdiag);
     pm["linediag"] = obj(&make_new<FieldLineDiag>, add_otherdiag);
     pm["energy"] = obj(&make_new<FieldEnergyDiag>, add_otherdiag);
+    pm["frequency"] = obj(&make_new<FrequencyDiag>, add_otherdiag);
+    pm["followmax"] = obj(&make_new<FollowMax>, add_otherdiag);
 }
 
 FieldSimulation::~FieldSimulation()
@@ -703,6 +706,7 @@ void SimpleFieldDiag::execute()
 {
     // DiagnosticInterface::execute, src/diagnostic.cpp:12-31
     if (!gpu) return;
+    calculate();
     const bool appending = !append.empty() && append[0] == 'y';
     if (t == 0 && appending && active()) open(parsedFileName());
     if (t % interval == 0 && active()) {
@@ -789,6 +793,167 @@ bool FieldSliceDiag::active() const
     const int normal = line_dim(plane);
     const int high = (normal == 0 ? globals->GridX : normal == 1 ? globals->GridY : globals->GridZ) + 1;
     return pos > 0 && pos < high;
+}
+
+// ---------------------------------------------------------------------------------------
+// diagnostics that read the grid after every step
+// ---------------------------------------------------------------------------------------
+
+SampledFieldDiag::~SampledFieldDiag()
+{
+    if (sampler) mpx_sampler_destroy(sampler);
+}
+
+void SampledFieldDiag::calculate()
+{
+    if (noGrid || !wanted()) return;
+    if (!sampler) {
+        int pos[3];
+        samplerPos(pos);
+        const int dim = samplerDim();
+        const int n[3] = {globals->GridX, globals->GridY, globals->GridZ};
+        len = dim < 0 ? 1 : n[dim] + 2;
+        capacity = std::max(1, std::min(4096, (1 << 20) / len));
+        const int rc = mpx_sampler_create(gpu, samplerField().c_str(), dim, pos, capacity, &sampler);
+        if (rc == MPX_ERR_NAME) {  // storage->hasGrid(field) is false
+            noGrid = true;
+            return;
+        }
+        if (rc) {
+            if (error.empty()) error = std::string("Diagnostic: ") + mpx_last_error();
+            noGrid = true;
+            return;
+        }
+        buf.resize((size_t)capacity * len);
+    }
+    if ((int)times.size() == capacity) catchUp();
+    if (mpx_sampler_record(sampler) != MPX_OK) {
+        if (error.empty()) error = std::string("Diagnostic: ") + mpx_last_error();
+        return;
+    }
+    times.push_back(*time);
+}
+
+void SampledFieldDiag::catchUp()
+{
+    if (!sampler || times.empty()) return;
+    int n = 0;
+    if (mpx_sampler_drain(sampler, buf.data(), buf.size(), &n) != MPX_OK || n != (int)times.size()) {
+        if (error.empty()) error = std::string("Diagnostic: ") + mpx_last_error();
+        times.clear();
+        return;
+    }
+    for (int r = 0; r < n; ++r) replay(times[r], buf.data() + (size_t)r * len);
+    times.clear();
+}
+
+void FrequencyDiag::MakeParamMap(ParamMap &pm)
+{
+    // FrequencyDiag::MakeParamMap, src/freqdiag.cpp:72-95
+    SimpleFieldDiag::MakeParamMap(pm);
+    pm["field"] = pstring(&field, "Ex");
+    pm["x"] = pint(&x, 1);
+    pm["y"] = pint(&y, 1);
+    pm["z"] = pint(&z, 1);
+}
+
+void FrequencyDiag::replay(int tstep, const double *values)
+{
+    // FrequencyDiag::calculate, src/freqdiag.cpp:45-70
+    const double val = 0.15 * values[0];
+    if (val * lastval <= 0 && std::fabs(val) * std::fabs(lastval) > 0) {
+        const double d = lastval / (lastval - val);
+        const double t = (tstep - 1 + d) * globals->DT;
+        if (count < 0) {
+            firstzero = t;
+            count = 0;
+        } else {
+            ++count;
+            frequency = (0.5 * count) / (t - firstzero);
+        }
+    }
+    lastval = val;
+}
+
+void FrequencyDiag::write()
+{
+    // FrequencyDiag::write, src/freqdiag.cpp:20-29
+    catchUp();
+    if (count > lastcount) {
+        lastcount = count;
+        if (out) *out << *time << " " << frequency << std::endl;
+    }
+}
+
+void FollowMax::MakeParamMap(ParamMap &pm)
+{
+    // FollowMax::MakeParamMap, src/followmax.cpp:115-125
+    SimpleFieldDiag::MakeParamMap(pm);
+    pm["field"] = pstring(&field, "Ex");
+    pm["dim"] = pstring(&direction, "x");
+    pm["posx"] = pint(&posx, 5);
+    pm["posy"] = pint(&posy, 5);
+    pm["posz"] = pint(&posz, 5);
+    pm["start"] = pint(&start, 0);
+}
+
+int FollowMax::samplerDim() const { return line_dim(direction); }
+
+bool FollowMax::open(const std::string &path)
+{
+    if (!SimpleFieldDiag::open(path)) return false;
+    out->precision(10);
+    return true;
+}
+
+bool FollowMax::wanted() const { return *time >= start; }
+
+void FollowMax::setup()
+{
+    // FollowMax::init, src/followmax.cpp:32-71: the walk starts at pos[dim]
+    if (ready) return;
+    const int pos[3] = {posx, posy, posz};
+    iposition = pos[samplerDim()];
+    position = iposition;
+    ready = true;
+}
+
+void FollowMax::replay(int, const double *line)
+{
+    // FollowMax::calculate, src/followmax.cpp:73-113, on the storage line low = 0 .. high = N + 1
+    setup();
+    const int high = samplerDim() == 0 ? globals->GridX + 1 : (samplerDim() == 1 ? globals->GridY + 1 : globals->GridZ + 1);
+    const int low = 0;
+    double mid, left, right;
+    bool maxfound;
+    do {
+        int pm = iposition - 1, pp = iposition + 1;
+        if (pm <= low) pm = high - 1;
+        if (pp >= high) pp = low + 1;
+        mid = line[iposition];
+        left = line[pm];
+        right = line[pp];
+        if (right > mid) {
+            iposition = pp;
+            maxfound = false;
+        } else if (left > mid) {
+            iposition = pm;
+            maxfound = false;
+        } else {
+            maxfound = true;
+        }
+    } while (!maxfound);
+    const double offset = (left - right) / (2 * (left - 2 * mid + right));
+    position = iposition + offset;
+}
+
+void FollowMax::write()
+{
+    // FollowMax::write, src/followmax.cpp:20-25
+    setup();
+    catchUp();
+    if (*time < start) return;
+    if (out) *out << *time << " " << position << std::endl;
 }
 
 bool FieldSliceDiag::open(const std::string &path)

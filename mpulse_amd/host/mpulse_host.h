@@ -348,6 +348,7 @@ class SimpleFieldDiag : public Rebuildable {
   protected:
     void MakeParamMap(ParamMap &pm) override;
     virtual bool active() const { return true; }
+    virtual void calculate() {}                  // every step, before the output check
     virtual bool open(const std::string &path);  // text stream by default
     virtual void write() = 0;
     virtual void close();
@@ -379,6 +380,77 @@ class FieldLineDiag : public SimpleFieldDiag {
     bool active() const override;
     bool open(const std::string &path) override;
     void write() override;
+};
+
+// Base of the diagnostics that read the grid after every step (FrequencyDiag, FollowMax):
+// the reads are queued on the GPU through an mpx_sampler and the reference's calculate()
+// runs over them, in step order, when an output is due or the sampler is full.
+class SampledFieldDiag : public SimpleFieldDiag {
+  public:
+    ~SampledFieldDiag() override;
+
+  protected:
+    void calculate() override;
+    void catchUp();  // replays the recorded steps
+    virtual bool wanted() const { return true; }                 // record at this step?
+    virtual void replay(int time, const double *values) = 0;    // one step's calculate()
+    virtual int samplerDim() const = 0;                         // -1: one cell
+    virtual void samplerPos(int pos[3]) const = 0;
+    virtual std::string samplerField() const = 0;
+    bool noGrid = false;  // the storage has no such grid: calculate() does nothing
+
+  private:
+    mpx_sampler *sampler = nullptr;
+    std::vector<int> times;  // step counter of each pending record
+    std::vector<double> buf;
+    int capacity = 0, len = 1;
+};
+
+// FrequencyDiag (src/freqdiag.cpp:5-95): keys "field" (Ex), "x", "y", "z" (1); the zero
+// crossings of 0.15 G(x, y, z) between steps, linearly interpolated; a line "time frequency"
+// at an output when a half period was counted since the last line
+class FrequencyDiag : public SampledFieldDiag {
+  public:
+    std::string field;
+    int x = 1, y = 1, z = 1;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    void write() override;
+    void replay(int time, const double *values) override;
+    int samplerDim() const override { return -1; }
+    void samplerPos(int pos[3]) const override { pos[0] = x, pos[1] = y, pos[2] = z; }
+    std::string samplerField() const override { return field; }
+
+  private:
+    double frequency = 0, lastval = 0, firstzero = 0;
+    int count = -1, lastcount = 0;
+};
+
+// FollowMax (src/followmax.cpp:5-125): keys "field" (Ex), "dim" (x), "posx", "posy", "posz" (5),
+// "start" (0); from step `start` on, walks along the line to a local maximum of the field and
+// writes "time position" (position refined by a quadratic through the maximum and its two
+// neighbours, the line's interior taken as periodic), precision 10
+class FollowMax : public SampledFieldDiag {
+  public:
+    std::string field, direction;
+    int posx = 5, posy = 5, posz = 5, start = 0;
+
+  protected:
+    void MakeParamMap(ParamMap &pm) override;
+    bool open(const std::string &path) override;
+    void write() override;
+    bool wanted() const override;
+    void replay(int time, const double *values) override;
+    int samplerDim() const override;
+    void samplerPos(int pos[3]) const override { pos[0] = posx, pos[1] = posy, pos[2] = posz; }
+    std::string samplerField() const override { return field; }
+
+  private:
+    void setup();        // FollowMax::init
+    bool ready = false;
+    int iposition = 0;
+    double position = 0;
 };
 
 // FieldSliceDiag (src/fielddiag.cpp:26-118): keys "field", "plane" (x | y | z), "pos"; one
