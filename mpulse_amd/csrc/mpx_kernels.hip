@@ -400,9 +400,10 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
                                                  double numer, double chi)
 {
     double Psix = 0, Psiy = 0, Psiz = 0;
+    const size_t pq = psi_at(c);
 #pragma unroll
     for (int n = 0; n < NP; ++n) {
-        double2 px = psi_in[n][0][c], py = psi_in[n][1][c], pz = psi_in[n][2][c];
+        double2 px = psi_in[n][0][pq], py = psi_in[n][1][pq], pz = psi_in[n][2][pq];
         const double2 D = pc.dd[n], X = pc.dxi0[n], C = pc.crec[n];
         double pxr = D.x * ex + px.x, pxi = D.y * ex + px.y;
         double pyr = D.x * ey + py.x, pyi = D.y * ey + py.y;
@@ -411,9 +412,9 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
         Psiy += pyr;
         Psiz += pzr;
         if (store) {
-            store_once(&psi[n][0][c], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
-            store_once(&psi[n][1][c], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
-            store_once(&psi[n][2][c], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
+            store_once(&psi[n][0][pq], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
+            store_once(&psi[n][1][pq], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
+            store_once(&psi[n][2][pq], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
         }
     }
     if (NONLIN) {
@@ -654,9 +655,9 @@ __global__ __launch_bounds__(256) void k_init_accumulator(StepArgs<double> a)
     for (int n = 0; n < NP; ++n)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            double2 p = a.psi[n][d][c];
+            double2 p = a.psi[n][d][psi_at(c)];
             const double2 X = a.pc.dxi0[n], C = a.pc.crec[n];
-            a.psi[n][d][c] = make_double2(X.x * e[d] + (C.x * p.x - C.y * p.y), X.y * e[d] + (C.x * p.y + C.y * p.x));
+            a.psi[n][d][psi_at(c)] = make_double2(X.x * e[d] + (C.x * p.x - C.y * p.y), X.y * e[d] + (C.x * p.y + C.y * p.x));
         }
 }
 
@@ -872,7 +873,7 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
 #pragma unroll
         for (int n = 0; n < NP; ++n)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][c]);
+            for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
         const double dBz_y = bz - (double)gld(Bz + c + ojm);
         const double dBx_y = bx - (double)gld(Bx + c + ojm);
         const T bys = lane_prev(byt), bxs = lane_prev(bxt);
@@ -1680,7 +1681,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
 #pragma unroll
             for (int n = 0; n < NP; ++n)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = load_once(&a.psio[n][q][c]);
+                for (int q = 0; q < 3; ++q) npsi[r].v[n][q] = load_once(&a.psio[n][q][psi_at(c)]);
         }
     };
 #pragma unroll

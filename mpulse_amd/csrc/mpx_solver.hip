@@ -966,10 +966,9 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                 s->pc.dd[a] = make_double2(D.real(), D.imag());
                 s->pc.dxi0[a] = make_double2(dxi0[n].real(), dxi0[n].imag());
                 s->pc.crec[a] = make_double2(Crec[n].real(), Crec[n].imag());
-                for (int c = 0; c < 3; ++c) {
-                    s->psi[a][c] = s->dalloc<double2>(s->cells, rc);
-                    if (rc) return rc;
-                }
+                double2 *base = s->dalloc<double2>(mpx::psi_block_elems(s->cells), rc);
+                if (rc) return rc;
+                for (int c = 0; c < 3; ++c) s->psi[a][c] = base + 64 * c;
             }
         // the 64x16 sweep block (at most 128 registers) spills with two or three poles; 64x8 does not
         if (s->np >= 2) s->sweep_variant = 0;
@@ -996,8 +995,12 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                     s->E2[c] = s->dalloc<float>(s->cells, rc);
                     s->B2[c] = s->dalloc<float>(s->cells, rc);
                 }
-                for (int a = 0; a < s->np; ++a) s->psi2[a][c] = s->dalloc<double2>(s->cells, rc);
                 if (rc) return rc;
+            }
+            for (int a = 0; a < s->np; ++a) {
+                double2 *base = s->dalloc<double2>(mpx::psi_block_elems(s->cells), rc);
+                if (rc) return rc;
+                for (int c = 0; c < 3; ++c) s->psi2[a][c] = base + 64 * c;
             }
             for (size_t f = 0; f < s->efaces.size(); ++f) {
                 const CpmlFace &F = s->efaces[f];
@@ -1702,8 +1705,8 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
         return MPX_OK;
     }
     if ((rc = materialise_ghosts(s))) return rc;
-    size_t stride = isPsi ? 16 : elem;
-    std::vector<unsigned char> buf(s->cells * stride);
+    // an accumulator component spans its pole's block from its own offset (psi_at)
+    std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     for (size_t i = 0; i < nx2; ++i)
@@ -1712,7 +1715,7 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
                 size_t src = g.at((int)i, (int)j, (int)k);
                 size_t dst = (i * ny2 + j) * nz2 + k;
                 if (isPsi)
-                    host[dst] = ((const double *)buf.data())[2 * src + part];
+                    host[dst] = ((const double *)buf.data())[2 * mpx::psi_at(src) + part];
                 else if (elem == 8)
                     host[dst] = ((const double *)buf.data())[src];
                 else
@@ -1833,8 +1836,8 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
     int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
     if (rc) return fail(rc, std::string("unknown grid ") + name);
     if (!ptr) return fail(MPX_ERR_UNSUPPORTED, std::string("grid ") + name + " is not stored (inactive pole or Sigma)");
-    size_t stride = isPsi ? 16 : elem;
-    std::vector<unsigned char> buf(s->cells * stride);
+    // an accumulator component spans its pole's block from its own offset (psi_at)
+    std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     for (size_t i = 0; i < nx2; ++i)
@@ -1843,7 +1846,7 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
                 size_t dst = g.at((int)i, (int)j, (int)k);
                 size_t src = (i * ny2 + j) * nz2 + k;
                 if (isPsi)
-                    ((double *)buf.data())[2 * dst + part] = host[src];
+                    ((double *)buf.data())[2 * mpx::psi_at(dst) + part] = host[src];
                 else if (elem == 8)
                     ((double *)buf.data())[dst] = host[src];
                 else
@@ -2118,7 +2121,7 @@ int mpx_state_bytes(mpx_solver *s, size_t *bytes)
     size_t b = 0;
     for (void *p : s->allocs) (void)p;
     b += 6 * s->cells * s->esz;
-    b += (size_t)s->np * 3 * s->cells * 16;
+    b += (size_t)s->np * mpx::psi_block_elems(s->cells) * 16;
     *bytes = b;
     return MPX_OK;
 }
@@ -2418,7 +2421,7 @@ static int reset_state(mpx_solver *s, hipStream_t st)
         rc |= z(s->E2[c], s->cells * s->esz) | z(s->B2[c], s->cells * s->esz);
         rc |= z(s->Jt[c], s->cells * 8) | z(s->Mt[c], s->cells * 8);
         for (int n = 0; n < 3; ++n) {
-            rc |= z(s->psi[n][c], s->cells * 16) | z(s->psi2[n][c], s->cells * 16);
+            if (c == 0) rc |= z(s->psi[n][0], mpx::psi_block_elems(s->cells) * 16) | z(s->psi2[n][0], mpx::psi_block_elems(s->cells) * 16);
             rc |= z(s->P[n][c], s->cells * 8) | z(s->Pp[n][c], s->cells * 8);
         }
     }

@@ -25,6 +25,20 @@ struct GridDims {
     }
 };
 
+// Layout of the PLRC accumulators: the three components of one pole's accumulator are stored
+// in blocks of 64 cells, [chunk][component][64] double2, so that the 64 z-consecutive cells a
+// wavefront updates form one contiguous 3 KB run for all three components.  Component q's
+// pointer is base + 64 q and cell c (a storage index of GridDims::at) sits at psi_at(c) from
+// it.  The shift of 32 starts a chunk at the interior cell k = 1 + 64 t of a row (storage
+// offset z0 + 1 = 32) when the row pitch is a multiple of 64.
+constexpr size_t PSI_SHIFT = 32;
+__host__ __device__ __forceinline__ size_t psi_at(size_t c)
+{
+    return ((c + PSI_SHIFT) >> 6) * 192 + ((c + PSI_SHIFT) & 63);
+}
+// double2 elements of one pole's block over `cells` storage cells
+inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
+
 // One CPML border current (a face of CPMLBorder{E,H}Current): its layer box, its two live
 // auxiliaries, and the b/c coefficient lines indexed by the normal coordinate.
 struct CpmlFace {
