@@ -187,7 +187,11 @@ int  mpx_set_time(mpx_solver *s, int time);
    "P1x".."P3zp" (fdtd-disp), "JxTotal".."MzTotal" (reference path only), and "Eps": a
    per-cell permittivity of the PLRC solvers (no reference counterpart; reads give the uniform
    eps until one is set; setting it needs the fused or reference-structured path).
-   Buffer: (Nx+2)*(Ny+2)*(Nz+2) float64, z fastest. */
+   Buffer: (Nx+2)*(Ny+2)*(Nz+2) float64, z fastest.  Ghosts: the reference-structured path and
+   the plain / Kerr-only / ADE solvers read the ghost values of E and B as set until their first
+   exchange, as the reference does; the fused and sweep paths of the PLRC solvers read a
+   periodic dimension's ghost as the periodic image of the interior (what the exchange puts
+   there), so a state whose ghosts differ from those images steps differently there. */
 int  mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count);
 int  mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t count);
 /* Field output.  FieldDiag hands its HDF5 stream the grid as Storage holds it: the storage box

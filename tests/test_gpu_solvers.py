@@ -1,0 +1,155 @@
+"""GPU parity against the oracle for the solvers, sources and boundaries besides the PLRC + Kerr
++ short-pulse workload: the linear PLRC core (src/fdtd_plrc.cpp:94-261), the plain, Kerr-only
+and ADE-Lorentz solvers (src/fdtd_plain.cpp, src/fdtd_nl.cpp, src/fdtd_disp.cpp), the
+xy-periodic boundary (src/periodic_bound.cpp:75-140), cpml_border_1d, the plane-wave and
+plane-Gauss sources (src/sources.cpp:8-331) and stepSchemeInit (src/fdtd_plrc.t:90-124).
+
+Float64, the same descriptor and initial state on both sides; relative L2 of every field and of
+the accumulators below 1e-12."""
+import numpy as np
+import pytest
+
+from mpulse_amd import abi, configs
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+PATHS = [abi.PATH_FUSED, abi.PATH_REFERENCE, abi.PATH_SWEEP]
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
+
+
+def compare(desc, steps, names=FIELDS, init=None, step_init=False, tol=1e-12):
+    print(f"case: grid {desc.nx}x{desc.ny}x{desc.nz} solver {desc.solver} boundary {desc.boundary} "
+          f"path {desc.path} factories {[desc.factories[q].kind for q in range(desc.n_factories)]} "
+          f"steps {steps} init {sorted(init) if init else None} step_init {step_init}", flush=True)
+    o = Oracle(desc)
+    with abi.Solver(desc) as s:
+        for n, a in (init or {}).items():
+            s.set(n, a)
+            o.set(n, a)
+        if step_init:
+            s.step_init()
+            o.step_init()
+        s.step(steps)
+        o.step(steps)
+        for n in names:
+            got, ref = s.get(n), o.get(n)
+            assert np.all(np.isfinite(got)), n
+            err = rel_l2(got, ref)
+            assert err < tol, (n, err)
+            assert np.linalg.norm(ref) > 0, n  # the case exercises this grid
+    o.close()
+
+
+def pulse(path, solver=abi.SOLVER_PLRC_NONLIN, boundary=abi.BOUNDARY_PERIODIC):
+    d, _ = configs.plrc_kerr_pulse(26, 6, 8.0, 6, 2e9, 0, d_src=9, nz=34, path=path, boundary=boundary)
+    d.solver = solver
+    d.factories[1].tshift = 10.0
+    return d
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_linear_plrc_core(path):
+    compare(pulse(path, solver=abi.SOLVER_PLRC_LIN), 60, FIELDS + ("PsiR1x", "PsiI1z"))
+
+
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+def test_xy_periodic_boundary(path):
+    """No ghost exchange along z (SingleXYPeriodicBoundary::exchangeZ is empty)."""
+    compare(pulse(path, boundary=abi.BOUNDARY_XY_PERIODIC), 60)
+
+
+def test_sweep_path_refuses_xy_periodic():
+    with pytest.raises(abi.MpxError) as e:
+        abi.Solver(pulse(abi.PATH_SWEEP, boundary=abi.BOUNDARY_XY_PERIODIC))
+    assert e.value.code == abi.ERR_UNSUPPORTED
+
+
+def column(path, src, cpml_kind, solver=abi.SOLVER_PLRC_NONLIN):
+    """A 10 x 12 x 60 box: a CPML (all faces, or the z faces only for cpml_border_1d), the
+    two-pole medium with Kerr, and a plane source at z = 16 travelling up."""
+    d = abi.desc_defaults()
+    d.nx, d.ny, d.nz = 10, 12, 60
+    d.dx = d.dy = d.dz = configs.DX
+    d.dt = configs.DT
+    d.solver = solver
+    d.path = path
+    d.eps, d.chi = 1.2, 0.05
+    d.l_eps[0], d.l_om2[0], d.l_delta[0] = 0.6, 3.0, 0.01
+    d.l_eps[1], d.l_om2[1], d.l_delta[1] = 0.3, 0.5, 0.002
+    cp = abi.factory_defaults(cpml_kind)
+    cp.d = 6
+    cp.sigma_max = configs.sigma_max(6)
+    abi.add_factory(d, cp)
+    f = abi.factory_defaults(src)
+    f.d = 16
+    f.h[0], f.h[1], f.h[2] = 0.3, 1.0, 0.0
+    f.k[0], f.k[1], f.k[2] = 2 * np.pi / (10 * configs.DX), 0.0, 1.5  # one period across x
+    f.h_bg[0], f.e_bg[1] = 0.01, 0.02
+    f.om0, f.ramp, f.gwidth, f.goffset, f.eps = 1.3, 0.5, 5.0, 14.0, 1.4
+    abi.add_factory(d, f)
+    return d
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("src", [abi.FACTORY_PLANE_WAVE, abi.FACTORY_PLANE_GAUSS])
+def test_plane_sources(path, src):
+    compare(column(path, src, abi.FACTORY_CPML), 150, FIELDS + ("PsiR2y",))
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_cpml_border_1d(path):
+    compare(column(path, abi.FACTORY_PLANE_GAUSS, abi.FACTORY_CPML_1D), 150)
+
+
+def random_state(shape, seed, amp=0.3):
+    rng = np.random.default_rng(seed)
+    return {n: amp * (rng.random(shape) + 0.2) * rng.choice([-1.0, 1.0], shape) for n in FIELDS}
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_PLAIN, abi.SOLVER_NL, abi.SOLVER_DISP])
+def test_simple_solvers(solver):
+    """Periodic-only solvers from a random non-zero state (no sources: they refuse currents);
+    the Kerr-only solver's Newton solve and the ADE polarisation with two oscillators."""
+    d = abi.desc_defaults()
+    d.nx, d.ny, d.nz = 11, 13, 17
+    d.dx, d.dy, d.dz = 0.3, 0.35, 0.25
+    d.dt = 0.1
+    d.solver = solver
+    d.eps, d.chi = 1.5, 0.2
+    d.d_eps2[0], d.d_om2[0] = 0.8, 2.0
+    d.d_eps2[1], d.d_om2[1] = 0.3, 0.7
+    compare(d, 40, init=random_state((13, 15, 19), seed=solver))
+
+
+def periodic_fill(a):
+    """ghost planes holding the periodic images, X then Y then Z (src/periodic_bound.cpp:10-71)"""
+    a[0], a[-1] = a[-2], a[1]
+    a[:, 0], a[:, -1] = a[:, -2], a[:, 1]
+    a[:, :, 0], a[:, :, -1] = a[:, :, -2], a[:, :, 1]
+    return a
+
+
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+def test_step_scheme_init(path):
+    """stepSchemeInit from a non-zero state: the accumulators seeded from E, B advanced half a
+    step, then the regular steps.  The reference-structured path reads the ghosts as set (as the
+    reference does before its first exchange); the fused path takes a periodic dimension's ghost
+    to be the interior's image, so its state is given consistent ghosts."""
+    d = pulse(path)
+    state = random_state((28, 28, 36), seed=7, amp=0.05)
+    if path == abi.PATH_FUSED:
+        state = {n: periodic_fill(a) for n, a in state.items()}
+    compare(d, 20, FIELDS + ("PsiR1y",), init=state, step_init=True)
+
+
+def test_simple_solver_step_init():
+    d = abi.desc_defaults()
+    d.nx, d.ny, d.nz = 9, 10, 12
+    d.solver = abi.SOLVER_PLAIN
+    compare(d, 10, init=random_state((11, 12, 14), seed=3), step_init=True)
