@@ -153,3 +153,42 @@ def test_simple_solver_step_init():
     d.nx, d.ny, d.nz = 9, 10, 12
     d.solver = abi.SOLVER_PLAIN
     compare(d, 10, init=random_state((11, 12, 14), seed=3), step_init=True)
+
+
+def combined_rel_l2(got, ref, names):
+    num = sum(np.sum((got[n] - ref[n]) ** 2) for n in names)
+    den = sum(np.sum(ref[n] ** 2) for n in names)
+    return float(np.sqrt(num / den))
+
+
+FP32_CASES = [
+    ("linear core", lambda p: pulse(p, solver=abi.SOLVER_PLRC_LIN), 60, [abi.PATH_FUSED, abi.PATH_SWEEP]),
+    ("xy-periodic", lambda p: pulse(p, boundary=abi.BOUNDARY_XY_PERIODIC), 60, [abi.PATH_FUSED]),
+    ("plane wave", lambda p: column(p, abi.FACTORY_PLANE_WAVE, abi.FACTORY_CPML), 150, [abi.PATH_FUSED, abi.PATH_SWEEP]),
+    ("plane gauss", lambda p: column(p, abi.FACTORY_PLANE_GAUSS, abi.FACTORY_CPML), 150, [abi.PATH_FUSED]),
+    ("cpml_border_1d", lambda p: column(p, abi.FACTORY_PLANE_GAUSS, abi.FACTORY_CPML_1D), 150,
+     [abi.PATH_FUSED, abi.PATH_SWEEP]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(FP32_CASES)))
+def test_fp32_bulk_cases_against_float64(case):
+    """FP32-bulk (float32 E, B and CPML auxiliaries; float64 accumulators and Kerr solve) against
+    the float64 run of the same case: relative L2 of the E vector field and of the B vector
+    field at most 1e-5, the bar stated for the FP32-bulk form."""
+    name, make, steps, paths = FP32_CASES[case]
+    ref_desc = make(abi.PATH_FUSED)
+    with abi.Solver(ref_desc) as s:
+        s.step(steps)
+        ref = {n: s.get(n) for n in FIELDS}
+    for path in paths:
+        d = make(path)
+        d.precision = abi.PREC_FP32_BULK
+        print(f"case: fp32-bulk {name} path {path} steps {steps}", flush=True)
+        with abi.Solver(d) as s:
+            s.step(steps)
+            got = {n: s.get(n) for n in FIELDS}
+        for group in (FIELDS[:3], FIELDS[3:]):
+            err = combined_rel_l2(got, ref, group)
+            print(f"  {name} path {path} {group[0][0]}: rel-L2 {err:.2e}", flush=True)
+            assert err <= 1e-5, (name, path, group, err)
