@@ -61,13 +61,16 @@ __device__ inline void plane_getE(const PlaneParams &W, int i, int j, int l, int
     }
 }
 
-// one sheet cell (i, j) at step `time`
+// one sheet cell (i, j) at step `time`; KIND > 0 fixes the source kind at compile time (each
+// kind then gets only the registers it needs), KIND = 0 reads it from the descriptor
+template <int KIND = 0>
 __device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const FaddeevaTables &ft, int i, int j, int time)
 {
+    const int kind = KIND > 0 ? KIND : a.kind;
     int z = a.sheet.kp + a.off;
     double f[3] = {0, 0, 0};
     const PulseParams &P = a.pulse;
-    if (a.kind == 1) {
+    if (kind == 1) {
         if (!a.isH) {
             // getHField, src/shortpulseinject.cpp:151-178
             double posxo = (i - P.centrex) * P.DX;
@@ -86,7 +89,7 @@ __device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const Fadde
             double posTime = time * P.DT - P.TShift;
             f[0] = P.amp * pulse_efunc(P, ft, posxh, posyo, poszo, posTime).re;
         }
-    } else if (a.kind == 4) {
+    } else if (kind == 4) {
         // GaussInjectSourceFunc::getHField / getEField, src/gaussinject.cpp:203-224
         const GaussSheetParams &G = a.gauss;
         if (G.active) {
@@ -99,7 +102,7 @@ __device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const Fadde
             }
         }
         (void)z;
-    } else if (a.kind == 3) {
+    } else if (kind == 3) {
         // amp Re[env(t - z/v) U exp(-i(om0 t - phase))], env(s) = exp(-s^2/T^2)
         const FocusedParams &F = a.focused;
         size_t p = (size_t)i * a.ny2 + j;
@@ -129,12 +132,13 @@ __device__ __forceinline__ void sheet_eval_point(const SheetEval &a, const Fadde
     a.sheet.J[a.t2][q] = f[a.t1] / a.DX;
 }
 
+template <int KIND>
 __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables ft)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     if (j >= a.ny2 || i >= a.nx2) return;
-    sheet_eval_point(a, ft, i, j, a.time);
+    sheet_eval_point<KIND>(a, ft, i, j, a.time);
 }
 
 // The sheets of many runs in one launch.  Sheet s owns blocks first[s] .. first[s+1]-1, one
@@ -178,7 +182,12 @@ void launch_sheet_eval(const SheetEval &a, const FaddeevaTables &ft, hipStream_t
 {
     dim3 blk(64);
     dim3 grd((a.ny2 + 63) / 64, a.nx2);
-    hipLaunchKernelGGL(k_sheet_eval, grd, blk, 0, s, a, ft);
+    switch (a.kind) {
+    case 1: hipLaunchKernelGGL(k_sheet_eval<1>, grd, blk, 0, s, a, ft); break;
+    case 3: hipLaunchKernelGGL(k_sheet_eval<3>, grd, blk, 0, s, a, ft); break;
+    case 4: hipLaunchKernelGGL(k_sheet_eval<4>, grd, blk, 0, s, a, ft); break;
+    default: hipLaunchKernelGGL(k_sheet_eval<2>, grd, blk, 0, s, a, ft); break;
+    }
 }
 
 // ------------------------------------------------------------------------------------
