@@ -2610,9 +2610,9 @@ int mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step)
     return rc;
 }
 
-// Autotuner: times every candidate launch shape -- the fused structure with 1 or 4 B planes
-// per thread and 4 or 8 blocks per CU, and the sweep with block shapes 64x16 / 64x8, x
-// segments of 16 / 32 planes and 1 or 2 blocks per CU -- for `steps_per_trial` steps on the
+// Autotuner: times every candidate launch shape -- the fused structure with 1, 2 or 4 B planes
+// per thread and 4 or 8 blocks per CU, and the sweep with block shapes 64x16 / 64x8 / 64x4, x
+// segments of 8 / 16 / 32 planes and 1 or 2 blocks per CU -- for `steps_per_trial` steps on the
 // batch itself, keeps the fastest, then clears every run back to its initial all-zero state
 // at time 0.  Allowed only before the first step.  result = {structure, a, b, c}: fused
 // {0, march_b, blocks_per_cu, 0}, sweep {1, variant, seg, blocks_per_cu}.
@@ -2624,10 +2624,10 @@ int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double 
         int structure, p0, p1, p2;
     };
     std::vector<Cand> cands;
-    for (int mb : {1, 4})
+    for (int mb : {1, 2, 4})
         for (int bpc : {4, 8}) cands.push_back({0, mb, bpc, 0});
-    for (int v : {2, 0})
-        for (int sg : {16, 32})
+    for (int v : {2, 0, 1})
+        for (int sg : {8, 16, 32})
             for (int bpc : {1, 2}) cands.push_back({1, v, sg, bpc});
     auto apply = [&](const Cand &c) {
         b->structure = c.structure;
