@@ -47,7 +47,7 @@ std::string Rebuildable::Rebuild(std::istream &in)
                 break;
             }
             if (tok == "{") {
-                warnings.push_back("Unknown Task " + name + ". Skipping... ");
+                warn("Unknown Task " + name + ". Skipping... ");
                 int level = 0;
                 do {
                     if (!(in >> tok)) {
@@ -58,13 +58,13 @@ std::string Rebuildable::Rebuild(std::istream &in)
                     if (tok == "}") level--;
                 } while (tok != "}" || level >= 0);
                 if (tok.empty()) {
-                    warnings.push_back("Unexpected end of file.");
+                    warn("Unexpected end of file.");
                     break;
                 }
                 if (!(in >> tok)) tok.clear();
             } else {
                 // the value is not consumed: it is read as the next key, as in the reference
-                warnings.push_back("Unknown Parameter " + name);
+                warn("Unknown Parameter " + name);
             }
             continue;
         }

@@ -10,6 +10,7 @@
 
 #include <fstream>
 #include <iosfwd>
+#include <iostream>
 #include <list>
 #include <map>
 #include <memory>
@@ -46,6 +47,12 @@ class Rebuildable {
     // reported and skipped (blocks included); returns the next token after the block.
     std::string Rebuild(std::istream &in);
     std::vector<std::string> warnings;
+    // kept, and printed on std::cerr as the reference's parser prints them (src/rebuild.cpp:34-50)
+    void warn(const std::string &m)
+    {
+        warnings.push_back(m);
+        std::cerr << m << std::endl;
+    }
 
   protected:
     virtual void MakeParamMap(ParamMap &pm) = 0;

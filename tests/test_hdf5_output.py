@@ -93,7 +93,7 @@ def test_writer_limits(tmp_path):
         H5File(b"not an hdf5 file")
 
 
-def test_fielddiag_keys_parse():
+def test_fielddiag_keys_parse(capfd):
     """The fielddiag block and its keys (src/fielddiag.cpp:9-15, src/diagnostic.cpp:34-47) are
     part of the setup grammar; they leave the solver descriptor unchanged."""
     base = ("globals { grid-x 8 grid-y 8 grid-z 8 T-total 3 } fieldsim { fdtd-plrc-nonlin { } single_periodic "
@@ -106,3 +106,12 @@ def test_fielddiag_keys_parse():
                                     "linediag { field Ey file l.dat dim y posx 3 posz 4 format gnu } "
                                     "slicediag { field Bz file s_#t.h5 plane z pos 2 }"))
     assert bytes(d0) == bytes(d2)
+    # FrequencyDiag (src/freqdiag.cpp:72-95) and FollowMax (src/followmax.cpp:115-125) keys
+    d3, _ = abi.parse_setup(base % ("frequency { field Ey file f.dat x 2 y 3 z 4 interval 5 } "
+                                    "followmax { field Ex file m.dat dim z posx 2 posy 3 posz 4 start 7 append y }"))
+    assert bytes(d0) == bytes(d3)
+    # an unknown key is reported and skipped, as the reference's parser does (src/rebuild.cpp:48-51)
+    capfd.readouterr()
+    d4, _ = abi.parse_setup(base % "followmax { field Ex nosuchkey start 3 }")
+    assert "Unknown Parameter nosuchkey" in capfd.readouterr().err
+    assert bytes(d0) == bytes(d4)
