@@ -94,12 +94,17 @@ def test_focused_pulse_keys():
 
 
 @pytest.mark.gpu
-def test_setup_file_runs_like_the_descriptor_path():
-    """The same workload through the C++ FieldSolver layer and through mpx_create directly."""
+@pytest.mark.parametrize("prec,path", [(abi.PREC_FP64, abi.PATH_FUSED), (abi.PREC_FP32_BULK, abi.PATH_FUSED),
+                                       (abi.PREC_FP32_BULK, abi.PATH_SWEEP), (abi.PREC_FP64, abi.PATH_REFERENCE)])
+def test_setup_file_runs_like_the_descriptor_path(prec, path):
+    """The same workload through the C++ FieldSolver layer and through mpx_create directly, for
+    the precision and kernel path options of mpulse_hip_run (--fp32, --sweep, --reference)."""
+    print(f"case: host driver prec {prec} path {path}", flush=True)
     text = SETUP_SMALL.replace("T-total 400", "T-total 40")
-    sim = abi.HostSimulation(text)
+    sim = abi.HostSimulation(text, precision=prec, path=path)
     sim.run()
     d, _ = abi.parse_setup(text)
+    d.precision, d.path = prec, path
     with abi.Solver(d) as s:
         s.step(40)
         for n in ("Ex", "By", "Ez"):
