@@ -202,7 +202,7 @@ def test_fused_block_orders_identical(prec):
         o.close()
 
 
-@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE])
+@pytest.mark.parametrize("path", [abi.PATH_FUSED, abi.PATH_REFERENCE, abi.PATH_SWEEP])
 def test_gauss_inject_against_oracle(path):
     """gauss_inject (src/gaussinject.cpp) into the one-pole Kerr medium: 14^2 x 20 cells give
     blocks of Nt = 16 steps; from toff = 2 the sheets take the samples of three blocks (the last
