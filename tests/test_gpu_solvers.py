@@ -161,6 +161,30 @@ def combined_rel_l2(got, ref, names):
     return float(np.sqrt(num / den))
 
 
+def plasma_case(path):
+    """multiphoton ionisation, resistivity and a plasma current in the one-pole Kerr medium"""
+    d, _ = configs.plrc_kerr_pulse(30, 6, 6.0, 6, 4e9, 0, d_src=9, path=path, poles=((1.1, 4.0, 0.005),))
+    d.factories[1].tshift = 12.0
+    f = abi.factory_defaults(abi.FACTORY_PLASMA_CURRENT)
+    f.em, f.gamma, f.rho_bg = 0.8, 0.2, 0.0
+    src = abi.Factory.from_buffer_copy(d.factories[1])
+    d.factories[1] = f
+    d.factories[2] = src
+    d.n_factories = 3
+    d.plasma, d.plasma_k, d.plasma_nu, d.plasma_mpa, d.plasma_wion = 1, 3, 0.05, 0.003, 0.5
+    return d
+
+
+def gauss_case(path):
+    """gauss_inject over three sample blocks"""
+    d, _ = configs.plrc_kerr_pulse(14, 4, 10.0, 4, 1e9, 60, d_src=6, nz=20, path=path)
+    f = abi.factory_defaults(abi.FACTORY_GAUSS_INJECT)
+    f.d, f.gsize_t, f.length, f.width, f.k0, f.gshift, f.gtime, f.amp, f.eps, f.toff = \
+        6, 48, 1.2, 1.1, 2 * np.pi, 0.4, 0.3, 0.8, 1.3, 2
+    d.factories[1] = f
+    return d
+
+
 FP32_CASES = [
     ("linear core", lambda p: pulse(p, solver=abi.SOLVER_PLRC_LIN), 60, [abi.PATH_FUSED, abi.PATH_SWEEP]),
     ("xy-periodic", lambda p: pulse(p, boundary=abi.BOUNDARY_XY_PERIODIC), 60, [abi.PATH_FUSED]),
@@ -168,14 +192,16 @@ FP32_CASES = [
     ("plane gauss", lambda p: column(p, abi.FACTORY_PLANE_GAUSS, abi.FACTORY_CPML), 150, [abi.PATH_FUSED]),
     ("cpml_border_1d", lambda p: column(p, abi.FACTORY_PLANE_GAUSS, abi.FACTORY_CPML_1D), 150,
      [abi.PATH_FUSED, abi.PATH_SWEEP]),
+    ("plasma", plasma_case, 90, [abi.PATH_FUSED]),
+    ("gauss_inject", gauss_case, 60, [abi.PATH_FUSED, abi.PATH_SWEEP]),
 ]
 
 
 @pytest.mark.parametrize("case", range(len(FP32_CASES)))
 def test_fp32_bulk_cases_against_float64(case):
-    """FP32-bulk (float32 E, B and CPML auxiliaries; float64 accumulators and Kerr solve) against
-    the float64 run of the same case: relative L2 of the E vector field and of the B vector
-    field at most 1e-5, the bar stated for the FP32-bulk form."""
+    """FP32-bulk (float32 E, B and CPML auxiliaries; float64 accumulators, Kerr solve and plasma
+    grids) against the float64 run of the same case: relative L2 of the E vector field and of
+    the B vector field at most 1e-5, the bar stated for the FP32-bulk form."""
     name, make, steps, paths = FP32_CASES[case]
     ref_desc = make(abi.PATH_FUSED)
     with abi.Solver(ref_desc) as s:
