@@ -195,15 +195,21 @@ def _dptr(a):
 
 
 class FetchHandle:
-    """An mpx_fetch_field copy in flight; wait() returns once the data has landed."""
+    """An mpx_fetch_field copy in flight; wait() returns once the data has landed.  The handle
+    holds the destination (array or PinnedArray) until then, so the copy never lands in freed
+    host memory."""
 
-    def __init__(self, solver, h):
-        self.solver, self.h = solver, h
+    def __init__(self, solver, h, out):
+        self.solver, self.h, self.out = solver, h, out
 
     def wait(self):
         if self.h is not None:
             rc = self.solver.lib.mpx_fetch_wait(self.h)
             self.h = None
+            self.solver._pending.discard(self)
+            if isinstance(self.out, PinnedArray):
+                self.out._pending.discard(self)
+            self.out = None
             self.solver._check(rc)
 
 
@@ -220,6 +226,7 @@ class Sampler:
         h = C.c_void_p()
         solver._check(solver.lib.mpx_sampler_create(solver.h, name.encode(), dim, p, capacity, C.byref(h)))
         self.h = h
+        solver._samplers.add(self)
 
     def record(self):
         self.solver._check(self.solver.lib.mpx_sampler_record(self.h))
@@ -234,6 +241,7 @@ class Sampler:
         if self.h is not None:
             rc = self.solver.lib.mpx_sampler_destroy(self.h)
             self.h = None
+            self.solver._samplers.discard(self)
             self.solver._check(rc)
 
     def __enter__(self):
@@ -255,9 +263,12 @@ class PinnedArray:
             raise MpxError(rc, self.lib.mpx_last_error().decode())
         self.p = p
         self.array = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(n,)).reshape(shape)
+        self._pending = set()  # fetches still copying into this buffer
 
     def close(self):
         if self.p is not None:
+            for f in list(self._pending):
+                f.wait()
             self.array = None
             self.lib.mpx_host_free(self.p)
             self.p = None
@@ -277,6 +288,7 @@ class Solver:
         self.desc = desc
         self.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
         self._own = True
+        self._samplers, self._pending = set(), set()
         h = C.c_void_p()
         self._check(self.lib.mpx_create(C.byref(desc), C.byref(h)))
         self.h = h
@@ -286,6 +298,7 @@ class Solver:
         """A non-owning handle on one run of a Batch (fields, energy; stepping is refused)."""
         v = cls.__new__(cls)
         v.lib, v.desc, v.h, v._own = lib, desc, handle, False
+        v._samplers, v._pending = set(), set()
         v.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
         return v
 
@@ -294,7 +307,18 @@ class Solver:
             raise MpxError(rc, self.lib.mpx_last_error().decode())
 
     def close(self):
+        """Wait for the fetches in flight and destroy the open samplers, then the solver."""
         if getattr(self, "h", None):
+            for f in list(getattr(self, "_pending", ())):
+                try:
+                    f.wait()
+                except MpxError:
+                    pass
+            for smp in list(getattr(self, "_samplers", ())):
+                try:
+                    smp.close()
+                except MpxError:
+                    pass
             if getattr(self, "_own", True):
                 self.lib.mpx_destroy(self.h)
             self.h = None
@@ -332,14 +356,20 @@ class Solver:
         return out
 
     def fetch_async(self, name, out):
-        """Queue a copy of grid `name` into `out` (float64, shape (nz+2, ny+2, nx+2): the
-        reference's Grid order, x fastest) behind the steps queued so far; returns a handle
-        whose wait() returns once the data is in `out`."""
-        if out.dtype != np.float64 or not out.flags.c_contiguous or out.shape != self.shape[::-1]:
+        """Queue a copy of grid `name` into `out` (a float64 array or a PinnedArray, shape
+        (nz+2, ny+2, nx+2): the reference's Grid order, x fastest) behind the steps queued so
+        far; returns a handle whose wait() returns once the data is in `out`.  The handle keeps
+        `out` alive until then; a PinnedArray's close() and this solver's close() wait for it."""
+        arr = out.array if isinstance(out, PinnedArray) else out
+        if arr is None or arr.dtype != np.float64 or not arr.flags.c_contiguous or arr.shape != self.shape[::-1]:
             raise ValueError(f"expected a C-contiguous float64 array of shape {self.shape[::-1]}")
         h = C.c_void_p()
-        self._check(self.lib.mpx_fetch_field(self.h, name.encode(), _dptr(out), out.size, C.byref(h)))
-        return FetchHandle(self, h)
+        self._check(self.lib.mpx_fetch_field(self.h, name.encode(), _dptr(arr), arr.size, C.byref(h)))
+        fh = FetchHandle(self, h, out)
+        self._pending.add(fh)
+        if isinstance(out, PinnedArray):
+            out._pending.add(fh)
+        return fh
 
     def fetch(self, name):
         """Grid `name` in the reference's Grid order (x fastest), shape (nz+2, ny+2, nx+2)."""

@@ -66,6 +66,32 @@ static void grid_fill(grid_t *g, double val)
     for (size_t q = 0; q < n; ++q) g->v[q] = val;
 }
 
+/* Storage rounding (not part of the reference): the value a field word holds after it is
+   stored in a narrower form.  Mode 1: float32 (round to nearest even).  Mode 2: float32 plus a
+   10-bit two's-complement residual q in units of 2^-10 of the float32 word's ulp,
+   x ~ hi + q 2^(max(e,1) - 160) for the biased exponent e of hi, |q| <= 511.  The checker uses
+   it to show what a storage form costs in accuracy against float64 storage. */
+static double store_round(double x, int mode)
+{
+    float hi = (float)x;
+    if (mode == 1 || !isfinite(hi)) return (double)hi;
+    union { float f; unsigned u; } b = {hi};
+    int e = (int)((b.u >> 23) & 0xffu);
+    if (e < 1) e = 1;
+    double q = rint(ldexp(x - (double)hi, 160 - e));
+    if (q > 511.0) q = 511.0;
+    if (q < -512.0) q = -512.0;
+    return (double)hi + ldexp(q, e - 160);
+}
+
+static void grid_round(grid_t *g, int mode)
+{
+    if (!mode) return;
+    size_t n = grid_count(g);
+#pragma omp parallel for schedule(static)
+    for (size_t q = 0; q < n; ++q) g->v[q] = store_round(g->v[q], mode);
+}
+
 typedef struct {
     int lo, hi;
     double *v;
@@ -886,6 +912,7 @@ struct or_sim {
     int haveJ, haveM;
     line_t kappaE[3], kappaH[3];
     int haveKappa;
+    int store_mode; /* storage rounding of E, B and the CPML auxiliaries (store_round) */
     /* PLRCData, src/fdtd_plrc.h:71-78 */
     cplx dchi0[3], dxi0[3], Crec[3];
     double sumChi0, sumXi0;
@@ -1380,8 +1407,16 @@ static void plasma_current_step(or_sim *s, current_t *c)
 static void current_step(or_sim *s, current_t *c)
 {
     switch (c->type) {
-    case CUR_CPML_E: cpml_step_E(s, c); break;
-    case CUR_CPML_H: cpml_step_H(s, c); break;
+    case CUR_CPML_E:
+        cpml_step_E(s, c);
+        grid_round(c->J[c->psi0], s->store_mode);
+        grid_round(c->J[c->psi1], s->store_mode);
+        break;
+    case CUR_CPML_H:
+        cpml_step_H(s, c);
+        grid_round(c->J[c->psi0], s->store_mode);
+        grid_round(c->J[c->psi1], s->store_mode);
+        break;
     case CUR_INC_E: inc_step_E(s, c); break;
     case CUR_INC_H: inc_step_H(s, c); break;
     case CUR_PLASMA: plasma_current_step(s, c); break;
@@ -1620,6 +1655,7 @@ static void plrc_stepD(or_sim *s, double dt)
                 }
                 plrc_step_d_cell(s, nonlin, dt, i, j, k, dx, dy, dz, jx, jy, jz);
             }
+    for (int c = 0; c < 3; ++c) grid_round(&s->E[c], s->store_mode);
     apply_boundary(s, s->E);
 }
 
@@ -1654,6 +1690,7 @@ static void plrc_stepB(or_sim *s, double dt)
                                   dt * ((AT(Ex, i, j + 1, k) - AT(Ex, i, j, k)) / kappaHdy -
                                         (AT(Ey, i + 1, j, k) - AT(Ey, i, j, k)) / kappaHdx + jz);
             }
+    for (int c = 0; c < 3; ++c) grid_round(&s->B[c], s->store_mode);
     apply_boundary(s, s->B);
 }
 
@@ -2158,6 +2195,15 @@ int or_step_init(or_sim *s)
 }
 
 int or_get_time(or_sim *s) { return s->time; }
+
+int or_set_storage(or_sim *s, int mode)
+{
+    if (mode < 0 || mode > 2) return -1;
+    s->store_mode = mode;
+    return 0;
+}
+
+double or_store_round(double x, int mode) { return store_round(x, mode); }
 void or_set_time(or_sim *s, int t) { s->time = t; }
 
 static grid_t *find_grid(or_sim *s, const char *name)

@@ -22,6 +22,10 @@ void    or_destroy(or_sim *s);
 int     or_step(or_sim *s, int nsteps);
 int     or_step_init(or_sim *s);
 int     or_get_time(or_sim *s);
+/* Storage rounding applied after every update of E, B and the CPML auxiliaries: 0 float64
+   (the reference), 1 float32, 2 float32 + 10-bit residual word (see store_round). */
+int     or_set_storage(or_sim *s, int mode);
+double  or_store_round(double x, int mode);
 void    or_set_time(or_sim *s, int t);
 int     or_get_grid(or_sim *s, const char *name, double *out, size_t count);
 int     or_set_grid(or_sim *s, const char *name, const double *in, size_t count);

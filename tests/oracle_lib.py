@@ -35,6 +35,8 @@ def load(omp=False):
         "or_step": (C.c_int, [P, C.c_int]),
         "or_step_init": (C.c_int, [P]),
         "or_get_time": (C.c_int, [P]),
+        "or_set_storage": (C.c_int, [P, C.c_int]),
+        "or_store_round": (C.c_double, [C.c_double, C.c_int]),
         "or_set_time": (None, [P, C.c_int]),
         "or_get_grid": (C.c_int, [P, C.c_char_p, D, C.c_size_t]),
         "or_set_grid": (C.c_int, [P, C.c_char_p, D, C.c_size_t]),
@@ -90,6 +92,12 @@ class Oracle:
 
     def step(self, n=1):
         self.lib.or_step(self.h, n)
+
+    def set_storage(self, mode):
+        """Round E, B and the CPML auxiliaries after every update: 0 float64 (the reference),
+        1 float32, 2 float32 + 10-bit residual word."""
+        if self.lib.or_set_storage(self.h, int(mode)):
+            raise ValueError(mode)
 
     def step_init(self):
         self.lib.or_step_init(self.h)
@@ -208,3 +216,8 @@ def gauss_block(fac, isH, nx, ny, dx, dy, dt, lowz, highz, time):
     bt = C.c_int()
     lib.or_gauss_block(C.byref(fac), isH, nx, ny, dx, dy, dt, lowz, highz, time, _dp(F1), _dp(F2), C.byref(bt))
     return F1, F2, bt.value
+
+
+def store_round(x, mode):
+    """The oracle's storage rounding of one value (0 float64, 1 float32, 2 float32 + residual)."""
+    return load().or_store_round(float(x), int(mode))

@@ -38,11 +38,15 @@ def test_fetch_overlaps_later_steps():
     d = small(abi.PREC_FP64)
     with abi.Solver(d) as a, abi.PinnedArray((32, 26, 26)) as buf:
         a.step(10)
-        h = a.fetch_async("Ey", buf.array)
+        h = a.fetch_async("Ey", buf)
         a.step(10)
         h.wait()
         got = buf.array.copy()
         later = a.fetch("Ey")
+        # a fetch still in flight when the buffer is closed: close() waits for it
+        h2 = a.fetch_async("Ex", buf)
+        buf.close()
+        assert h2.h is None
     with abi.Solver(d) as b:
         b.step(10)
         ref = b.get("Ey").transpose(2, 1, 0)
