@@ -65,6 +65,10 @@ extern "C" {
 /* storage precision of the bulk fields */
 #define MPX_PREC_FP64      0  /* every word float64 */
 #define MPX_PREC_FP32_BULK 1  /* E, B and CPML psi float32; PLRC accumulators and Kerr solve float64 */
+#define MPX_PREC_FP32_RES  2  /* as FP32_BULK, each E, B and CPML psi word backed by a 10-bit residual
+                                 (three of a cell's E or B components, or a face's two auxiliaries,
+                                 share one 32-bit word): stored values within 2^-10 float32 ulp,
+                                 about 1000x closer to float64 storage; fused path */
 
 /* kernel structure */
 #define MPX_PATH_FUSED     0  /* one fused E-side and one fused B-side kernel per step */
@@ -314,6 +318,11 @@ typedef struct mpx_objective {
 } mpx_objective;
 int  mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int checkpoint_every, double *J,
                       double *dJ_deps, double *dJ_dchi);
+
+/* The value a field word holds once stored in `precision` (MPX_PREC_*): x itself (float64),
+   its float32 rounding, or the float32 word plus residual; what mpx_set_field stores for x.
+   No counterpart in the reference. */
+double mpx_store_round(double x, int precision);
 
 const char *mpx_last_error(void);
 int  mpx_abi_version(void);

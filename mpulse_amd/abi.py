@@ -18,7 +18,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0,
 
 SOLVER_PLRC_NONLIN, SOLVER_PLRC_LIN, SOLVER_PLAIN, SOLVER_NL, SOLVER_DISP = 0, 1, 2, 3, 4
 BOUNDARY_PERIODIC, BOUNDARY_XY_PERIODIC = 0, 1
-PREC_FP64, PREC_FP32_BULK = 0, 1
+PREC_FP64, PREC_FP32_BULK, PREC_FP32_RES = 0, 1, 2
 PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
 FACTORY_FOCUSED_PULSE = 6
@@ -176,6 +176,7 @@ def load():
         "mpx_adjoint_grad": (C.c_int, [P, C.POINTER(Objective), C.c_int, C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "mpx_last_error": (C.c_char_p, []),
+        "mpx_store_round": (C.c_double, [C.c_double, C.c_int]),
         "mpx_abi_version": (C.c_int, []),
         "mpx_desc_defaults": (None, [C.POINTER(Desc)]),
         "mpx_factory_defaults": (None, [C.POINTER(Factory), C.c_int]),
@@ -661,3 +662,8 @@ class HostSimulation:
             self.close()
         except Exception:
             pass
+
+
+def store_round(x, precision):
+    """The value a field word holds once stored in `precision` (PREC_*)."""
+    return load().mpx_store_round(float(x), int(precision))

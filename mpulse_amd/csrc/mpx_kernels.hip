@@ -11,6 +11,7 @@
 // wavefront is one coalesced 256/512-byte segment.
 #include <hip/hip_runtime.h>
 
+#include "mpx_device.h"
 #include "mpx_launch.h"
 
 namespace mpx {
@@ -284,6 +285,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_wrap(GridDims g, T *f0, T *f1, T *f2, int dim)
 {
     T *f = blockIdx.z == 0 ? f0 : (blockIdx.z == 1 ? f1 : f2);
+    if (!f) return;  // a group of one array (the residual words of float32 + residual storage)
     int a = blockIdx.x * 64 + threadIdx.x;  // fastest remaining dimension
     int b = blockIdx.y * 4 + threadIdx.y;
     int N = g.n[dim];
@@ -313,209 +315,7 @@ void launch_wrap(const GridDims &g, T *const f[3], int dim, hipStream_t s)
 }
 template void launch_wrap<double>(const GridDims &, double *const[3], int, hipStream_t);
 template void launch_wrap<float>(const GridDims &, float *const[3], int, hipStream_t);
-
-// ------------------------------------------------------------------------------------
-// PLRC + Kerr E update core, shared by both paths:
-// FDTD_PLRCNonlinCore::plrcStepD src/fdtd_plrc.cpp:304-427,
-// FDTD_PLRCLinCore::plrcStepD    src/fdtd_plrc.cpp:94-261.
-// d* are the B differences in the reference's order, kd* the kappa*delta factors.
-// ------------------------------------------------------------------------------------
-// Every grid, table and layer pointer of the solver points into hipMalloc'd memory.  Kernels
-// that read their argument block from device memory see those pointers as generic ones, and
-// generic (flat) accesses also count against the scalar-load counter, so each wait for an
-// argument load would wait for the outstanding grid loads too.  gld / gst access through the
-// global address space instead.
-#define MPX_GLOBAL __attribute__((address_space(1)))
-template <typename T>
-__device__ __forceinline__ T gld(const T *p)
-{
-    return *(const MPX_GLOBAL T *)p;
-}
-template <typename T>
-__device__ __forceinline__ void gst(T *p, T v)
-{
-    *(MPX_GLOBAL T *)p = v;
-}
-
-// Words a launch touches exactly once (the pole accumulators, a cell's own E in the E kernel
-// and B in the B kernel) move with nontemporal loads and stores: they then do not evict from L2
-// the neighbour planes that other threads of the launch re-read.
-typedef double double2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 load_once(const double2 *p)
-{
-    const double2v v = __builtin_nontemporal_load((const MPX_GLOBAL double2v *)p);
-    return make_double2(v.x, v.y);
-}
-__device__ __forceinline__ void store_once(double2 *p, double2 v)
-{
-    double2v w;
-    w.x = v.x;
-    w.y = v.y;
-    __builtin_nontemporal_store(w, (MPX_GLOBAL double2v *)p);
-}
-template <typename T>
-__device__ __forceinline__ T load_once(const T *p)
-{
-    return __builtin_nontemporal_load((const MPX_GLOBAL T *)p);
-}
-template <typename T>
-__device__ __forceinline__ void store_once(T *p, T v)
-{
-    __builtin_nontemporal_store(v, (MPX_GLOBAL T *)p);
-}
-
-// The value a neighbouring lane of the wavefront holds, by DPP: lane_prev gives lane l the
-// value of lane l - 1 (wave_shr:1), lane_next that of lane l + 1 (wave_shl:1).  Lanes of a
-// fused-kernel wave hold consecutive z cells of one row, so the z neighbour of a cell is in
-// the next lane except at the ends of the wave and of the row, where it is loaded.
-__device__ __forceinline__ float lane_prev(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float lane_next(float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-__device__ __forceinline__ double lane_prev(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ double lane_next(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// quotient x / k, or x * k when k already holds the reciprocal (RECIP)
-template <bool RECIP>
-__device__ __forceinline__ double qt(double x, double k)
-{
-    return RECIP ? x * k : x / k;
-}
-
-// RECIP: kdx, kdy, kdz and denom hold reciprocals (1/(kappa delta), 1/denom) and the divisions
-// become products; otherwise the reference's quotients are formed.
-template <int NP, bool NONLIN, bool RECIP, typename PIN, typename POUT>
-__device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
-                                                 double dBx_z, double dBz_x, double dBy_x, double dBx_y,
-                                                 double kdx, double kdy, double kdz, double jx, double jy,
-                                                 double jz, const PIN &psi_in, const POUT &psi, size_t c,
-                                                 bool store, const PoleConsts &pc, double dt, double denom,
-                                                 double numer, double chi)
-{
-    double Psix = 0, Psiy = 0, Psiz = 0;
-    const size_t pq = psi_at(c);
-#pragma unroll
-    for (int n = 0; n < NP; ++n) {
-        double2 px = psi_in[n][0][pq], py = psi_in[n][1][pq], pz = psi_in[n][2][pq];
-        const double2 D = pc.dd[n], X = pc.dxi0[n], C = pc.crec[n];
-        double pxr = D.x * ex + px.x, pxi = D.y * ex + px.y;
-        double pyr = D.x * ey + py.x, pyi = D.y * ey + py.y;
-        double pzr = D.x * ez + pz.x, pzi = D.y * ez + pz.y;
-        Psix += pxr;
-        Psiy += pyr;
-        Psiz += pzr;
-        if (store) {
-            store_once(&psi[n][0][pq], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
-            store_once(&psi[n][1][pq], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
-            store_once(&psi[n][2][pq], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
-        }
-    }
-    if (NONLIN) {
-        double E2 = ex * ex + ey * ey + ez * ez;
-        double f = numer + chi * E2;
-        double cx = qt<RECIP>(f * ex + dt * ((qt<RECIP>(dBz_y, kdy) - qt<RECIP>(dBy_z, kdz)) + jx) + Psix, denom);
-        double cy = qt<RECIP>(f * ey + dt * ((qt<RECIP>(dBx_z, kdz) - qt<RECIP>(dBz_x, kdx)) + jy) + Psiy, denom);
-        double cz = qt<RECIP>(f * ez + dt * ((qt<RECIP>(dBy_x, kdx) - qt<RECIP>(dBx_y, kdy)) + jz) + Psiz, denom);
-        double E = sqrt(E2);
-        double A = qt<RECIP>(chi, denom);
-        double Cm = sqrt(cx * cx + cy * cy + cz * cz);
-        if (Cm > 0) {
-            // the reference's stopping rule |E - Eold| <= 1e-9; the cap only guards a cell that
-            // could never meet it (non-finite data, or fields far from O(1))
-            double Eold;
-            int it = 0;
-            do {
-                Eold = E;
-                double Q = E * E;
-                E = (2 * A * Q * E + Cm) / (3 * A * Q + 1);
-            } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
-            if (RECIP) {
-                const double sc = E / Cm;
-                ex = sc * cx;
-                ey = sc * cy;
-                ez = sc * cz;
-            } else {
-                ex = E * cx / Cm;
-                ey = E * cy / Cm;
-                ez = E * cz / Cm;
-            }
-        } else {
-            ex = 0.0;
-            ey = 0.0;
-            ez = 0.0;
-        }
-    } else {
-        double exn = qt<RECIP>(numer * ex + (dt * ((qt<RECIP>(dBz_y, kdy) - qt<RECIP>(dBy_z, kdz)) + jx) + Psix), denom);
-        double eyn = qt<RECIP>(numer * ey + (dt * ((qt<RECIP>(dBx_z, kdz) - qt<RECIP>(dBz_x, kdx)) + jy) + Psiy), denom);
-        double ezn = qt<RECIP>(numer * ez + (dt * ((qt<RECIP>(dBy_x, kdx) - qt<RECIP>(dBx_y, kdy)) + jz) + Psiz), denom);
-        ex = exn;
-        ey = eyn;
-        ez = ezn;
-    }
-}
-
-// Denominator and numerator of the E update at cell c: the uniform ones, or those of a
-// per-cell permittivity formed with the pole sums in the order the uniform ones are formed.
-// sigma = Sigma dt / 2 of the plasma resistivity (src/fdtd_plrc.cpp:356-359).
-template <bool NONLIN, typename A>
-__device__ __forceinline__ void cell_eps(const A &a, size_t c, double sigma, double &denom, double &numer)
-{
-    if (a.epsf || a.plasma) {
-        const double e = a.epsf ? gld(a.epsf + c) : a.eps;
-        denom = NONLIN ? e + a.sumChi0 - a.sumXi0 + sigma : e + (a.sumChi0 - a.sumXi0) + sigma;
-        numer = e - a.sumXi0 - sigma;
-    } else {
-        denom = a.denom;
-        numer = a.numer;
-    }
-}
-
-// PlasmaDensity::stepScheme at one cell (src/plasmadensity.cpp:40-53): rho <- (rho (1 + n I)
-// + m I^K) / (1 - n I) with I = |E|^2; returns I^(K-1), which Sigma receives times w.
-__device__ __forceinline__ double plasma_density_cell(double ex, double ey, double ez, double &rho, double n,
-                                                      double m, int Km)
-{
-    const double I = ex * ex + ey * ey + ez * ez;
-    double IKm = 1.0;
-    for (int q = 0; q < Km; ++q) IKm *= I;  // ipow, src/util.h:67-73
-    const double IK = I * IKm;
-    rho = (rho * (1 + n * I) + m * IK) / (1 - n * I);
-    return IKm;
-}
-
-// PlasmaCurrent::stepScheme at one cell (src/plasmacurrent.cpp:63-71)
-__device__ __forceinline__ double plasma_current_comp(double j, double e, double rho_t, double gdtn, double gdtd,
-                                                      double emdt)
-{
-    return (j * gdtn - emdt * e * rho_t) / gdtd;
-}
-
-template <int NP, bool NONLIN, bool RECIP>
-__device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez, double dBz_y, double dBy_z,
-                                              double dBx_z, double dBz_x, double dBy_x, double dBx_y,
-                                              double kdx, double kdy, double kdz, double jx, double jy, double jz,
-                                              double2 *const psi[3][3], size_t c, const PoleConsts &pc,
-                                              double dt, double denom, double numer, double chi)
-{
-    plrc_e_update_pp<NP, NONLIN, RECIP>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kdx, kdy, kdz, jx, jy, jz, psi,
-                                 psi, c, true, pc, dt, denom, numer, chi);
-}
+template void launch_wrap<unsigned>(const GridDims &, unsigned *const[3], int, hipStream_t);
 
 // FDTD_PLRCSolver::stepD loop, src/fdtd_plrc.t:221-233, reading the current totals
 template <int NP, bool NONLIN>
@@ -683,162 +483,13 @@ void launch_init_accumulator(const StepArgs<double> &a, int np, hipStream_t s)
     }
 }
 
-// ------------------------------------------------------------------------------------
-// fused path
-// ------------------------------------------------------------------------------------
-
-// accumulators of the active poles held in registers, indexable like the psi pointer table
-// (psi[n][component][cell]) so that the shared E update reads them in place
-template <int NP>
-struct PsiRegs {
-    double2 v[NP > 0 ? NP : 1][3];
-    struct Comp {
-        double2 x;
-        __device__ double2 operator[](size_t) const { return x; }
-    };
-    struct Pole {
-        const double2 *p;
-        __device__ Comp operator[](int q) const { return Comp{p[q]}; }
-    };
-    __device__ Pole operator[](int n) const { return Pole{v[n]}; }
-};
-
-
-// CPML auxiliaries of one face updated inside the fused kernels.  The face map is
-// x: (psi0 -> Jy from dBz, psi1 -> Jz from dBy), y: (psi0 -> Jz from dBx, psi1 -> Jx from dBz),
-// z: (psi0 -> Jx from dBy, psi1 -> Jy from dBx) (src/cpml_border.cpp:357-386, 456-485).
-// E side: CPMLBorderECurrent::stepScheme, src/cpml_border.cpp:398-425 (psi0 gets -c d1/dx);
-// H side: CPMLBorderHCurrent::stepScheme, src/cpml_border.cpp:499-527 (psi0 gets +c d1/dx).
-template <typename T, bool HSIDE>
-__device__ __forceinline__ void cpml_face(const CpmlFace &f, int coord, int l, double d1, double d0, double &v0,
-                                          double &v1)
-{
-    int q = coord - f.lo[f.dim];
-    T *p0 = (T *)f.psi0, *p1 = (T *)f.psi1;
-    double bq = gld(f.b + q), cq = gld(f.c + q);
-    T n0, n1;
-    // ordinary loads: a face row covers only part of a cache line (z faces: 16 cells), and the
-    // rest of the line is read by the neighbouring row
-    const double o0 = (double)gld(p0 + l), o1 = (double)gld(p1 + l);
-    if (HSIDE) {
-        n0 = (T)(bq * o0 + cq * d1 * f.idxn);
-        n1 = (T)(bq * o1 - cq * d0 * f.idxn);
-    } else {
-        n0 = (T)(bq * o0 - cq * d1 * f.idxn);
-        n1 = (T)(bq * o1 + cq * d0 * f.idxn);
-    }
-    gst(p0 + l, n0);
-    gst(p1 + l, n1);
-    v0 = (double)n0;
-    v1 = (double)n1;
-}
-
-// 32-bit layer index of cell (i, j, k) in a face box
-__device__ __forceinline__ int layer_index(const CpmlFace &f, int i, int j, int k)
-{
-    return ((i - f.lo[0]) * f.n1 + (j - f.lo[1])) * f.n2 + (k - f.lo[2]);
-}
-
-// add the current-sheet layers that contain cell (i, j, k), in list order
-template <typename T>
-__device__ __forceinline__ void add_sheets(const StepArgs<T> &a, int i, int j, int k, double &x, double &y,
-                                           double &z)
-{
-    for (int s = 0; s < a.nsheets; ++s)
-        if (k == a.sheet[s].kp) {
-            int q = i * a.g.py + j;
-            x += gld(a.sheet[s].J[0] + q);
-            y += gld(a.sheet[s].J[1] + q);
-            z += gld(a.sheet[s].J[2] + q);
-        }
-}
-
-// Map the linear block id to a tile so that each XCD (blocks are dealt round-robin over
-// the 8 XCDs) walks a contiguous slab of x planes and keeps the x-neighbour planes in its L2.
-__device__ __forceinline__ void xcd_tile(int nbx, int nby, int nbz, int &bx, int &by, int &bz)
-{
-    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    unsigned total = (unsigned)nbx * nby * nbz;
-    unsigned per = total / 8u;
-    unsigned xcd = lin & 7u, idx = lin >> 3;
-    unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
-    bx = t % nbx;
-    t /= nbx;
-    by = t % nby;
-    bz = t / nby;
-}
-
-// Banded order: the tile rows are cut into bands of R rows; inside a band the z tiles run
-// fastest, then the rows of the band, then the x chunks, so the x-neighbour plane of a tile
-// was read a band-plane (nbx * R tiles) earlier and is still in the XCD's L2.  Each XCD
-// takes a contiguous share of the order as in xcd_tile.  With rev the order runs from its end,
-// which the B-side kernel uses to begin on the tiles the E-side kernel finished last.
-__device__ __forceinline__ void band_tile(int nbx, int nby, int nbz, int R, int rev, int &bx, int &by, int &bz)
-{
-    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    unsigned total = (unsigned)nbx * nby * nbz;
-    unsigned per = total / 8u;
-    unsigned xcd = lin & 7u, idx = lin >> 3;
-    unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
-    if (rev) t = total - 1u - t;
-    const unsigned full = (unsigned)nbx * R * nbz;  // tiles of a full band; only the last is shorter
-    const unsigned b = t / full;
-    unsigned rem = t - b * full;
-    const unsigned rows = min((unsigned)R, (unsigned)nby - b * R);
-    bx = rem % nbx;
-    rem /= nbx;
-    by = b * R + rem % rows;
-    bz = rem / rows;
-}
-
-__device__ __forceinline__ void fused_tile_of(int order, int band, int rev, int nbx, int nby, int nbz, int &bx,
-                                              int &by, int &bz)
-{
-    if (order == 2)
-        band_tile(nbx, nby, nbz, band, rev, bx, by, bz);
-    else if (order == 1)
-        xcd_tile(nbz, nbx, nby, bz, bx, by);
-    else
-        xcd_tile(nbx, nby, nbz, bx, by, bz);
-}
-
-// Whether coordinate coord of dimension dim lies in the layer of the high-side ([0]) and
-// low-side ([1]) CPML face of that dimension.  The face index itself stays uniform across
-// the wave (fbd[dim][s]), so the face descriptors are read with scalar loads.
-__device__ __forceinline__ void in_faces(const int fbd[3][2], const CpmlFace *faces, int dim, int coord, bool out[2])
-{
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        int f = fbd[dim][s];
-        out[s] = f >= 0 && coord >= faces[f].lo[dim] && coord <= faces[f].hi[dim];
-    }
-}
 
 // Fused E side of one step.  A thread owns one (j, k) column segment of L cells along x and
 // walks it plane by plane, keeping B of the previous plane in registers for the x difference.
-// (j, k) of this thread in a tile: 64 z cells x 4 y rows, or (LIN) 256 consecutive cells of
-// the y-z plane taken row after row, which keeps every lane busy on grids whose z extent is
-// not a multiple of 64.  lo = first index of the range in y and z.
-template <bool LIN>
-__device__ __forceinline__ void tile_jk(const GridDims &g, const int lo[3], int tbx, int tby, int &j, int &k)
-{
-    if (LIN) {
-        const int nk = g.n[2] - lo[2] + 1;
-        const int lin = tbx * 256 + threadIdx.y * 64 + threadIdx.x;
-        const int q = lin / nk;
-        j = q + lo[1];
-        k = lin - q * nk + lo[2];
-    } else {
-        k = tbx * 64 + threadIdx.x + lo[2];
-        j = tby * 4 + threadIdx.y + lo[1];
-    }
-}
-
-// Fused E side of one step.  A thread owns one (j, k) column segment of L cells along x and
-// walks it plane by plane, keeping B of the previous plane in registers for the x difference.
-template <typename T, int NP, bool NONLIN, bool CPML, int L, bool LIN = false>
+template <typename T, int NP, bool NONLIN, bool CPML, int L, bool LIN = false, bool RES = false>
 __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int tby, int tbz)
 {
+    using V = typename Word<T, RES>::V;
     const GridDims &g = a.g;
     const int one[3] = {1, 1, 1};
     int j, k;
@@ -852,7 +503,8 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
     int c = i0 * sx + j * pz + k + g.z0;
     const T *Bx = a.B[0], *By = a.B[1], *Bz = a.B[2];
     const int cim = (i0 == 1 && a.periodic[0]) ? c + (g.n[0] - 1) * sx : c - sx;
-    double bz_im = (double)gld(Bz + cim), by_im = (double)gld(By + cim);
+    const unsigned wim = lo_ld<RES>(a.Blo, cim);
+    double bz_im = (double)rd<T, RES>(Bz, wim, 2, cim), by_im = (double)rd<T, RES>(By, wim, 1, cim);
     bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
         in_faces(a.face_by_dim, a.face, 1, j, iny);
@@ -867,25 +519,29 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
         // other lane takes them from the previous lane (the loads are issued first, so that one
         // wait covers them and the cell's own loads)
         const bool zload = threadIdx.x == 0 || k == 1;
-        T bym = (T)0, bxm = (T)0;
+        V bym = (V)0, bxm = (V)0;
         if (zload) {
-            bym = gld(By + c + okm);
-            bxm = gld(Bx + c + okm);
+            const unsigned wm = lo_ld<RES>(a.Blo, c + okm);
+            bym = rd<T, RES>(By, wm, 1, c + okm);
+            bxm = rd<T, RES>(Bx, wm, 0, c + okm);
         }
-        const T bxt = gld(Bx + c), byt = gld(By + c);
-        const double bx = bxt, by = byt, bz = gld(Bz + c);
+        const unsigned wb = lo_ld<RES>(a.Blo, c);
+        const V bxt = rd<T, RES>(Bx, wb, 0, c), byt = rd<T, RES>(By, wb, 1, c);
+        const double bx = bxt, by = byt, bz = rd<T, RES>(Bz, wb, 2, c);
         // E and the pole accumulators are read before any store of the cell (the CPML
         // auxiliaries), so that all its reads leave together
-        double ex = (double)load_once(&a.E[0][c]), ey = (double)load_once(&a.E[1][c]);
-        double ez = (double)load_once(&a.E[2][c]);
+        const unsigned we = lo_once<RES>(a.Elo, c);
+        double ex = (double)rd_once<T, RES>(a.E[0], we, 0, c), ey = (double)rd_once<T, RES>(a.E[1], we, 1, c);
+        double ez = (double)rd_once<T, RES>(a.E[2], we, 2, c);
         PsiRegs<NP> pr;
 #pragma unroll
         for (int n = 0; n < NP; ++n)
 #pragma unroll
             for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
-        const double dBz_y = bz - (double)gld(Bz + c + ojm);
-        const double dBx_y = bx - (double)gld(Bx + c + ojm);
-        const T bys = lane_prev(byt), bxs = lane_prev(bxt);
+        const unsigned wj = lo_ld<RES>(a.Blo, c + ojm);
+        const double dBz_y = bz - (double)rd<T, RES>(Bz, wj, 2, c + ojm);
+        const double dBx_y = bx - (double)rd<T, RES>(Bx, wj, 0, c + ojm);
+        const V bys = lane_prev(byt), bxs = lane_prev(bxt);
         const double dBy_z = by - (double)(zload ? bym : bys);
         const double dBx_z = bx - (double)(zload ? bxm : bxs);
         const double dBz_x = bz - bz_im;
@@ -920,7 +576,7 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
                     if (iny[s]) {
                         const CpmlFace &f = a.face[a.face_by_dim[1][s]];
                         double v0, v1;
-                        cpml_face<T, false>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
+                        cpml_face<T, false, RES>(f, j, layer_index(f, i, j, k), dBx_y, dBz_y, v0, v1);
                         jz += v0;
                         jx += v1;
                     }
@@ -931,7 +587,7 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
                     if (inx[s]) {
                         const CpmlFace &f = a.face[a.face_by_dim[0][s]];
                         double v0, v1;
-                        cpml_face<T, false>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
+                        cpml_face<T, false, RES>(f, i, layer_index(f, i, j, k), dBz_x, dBy_x, v0, v1);
                         jy += v0;
                         jz += v1;
                     }
@@ -940,7 +596,7 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
                     if (inz[s]) {
                         const CpmlFace &f = a.face[a.face_by_dim[2][s]];
                         double v0, v1;
-                        cpml_face<T, false>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
+                        cpml_face<T, false, RES>(f, k, layer_index(f, i, j, k), dBy_z, dBx_z, v0, v1);
                         jx += v0;
                         jy += v1;
                     }
@@ -973,17 +629,21 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
         plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i), kdy,
                                            kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom, numer,
                                            a.chi);
-        store_once(&a.E[0][c], (T)ex);
-        store_once(&a.E[1][c], (T)ey);
-        store_once(&a.E[2][c], (T)ez);
+        unsigned wn = 0;
+        const T hx = put<T, RES>(ex, wn, 0), hy = put<T, RES>(ey, wn, 1), hz = put<T, RES>(ez, wn, 2);
+        store_once(&a.E[0][c], hx);
+        store_once(&a.E[1][c], hy);
+        store_once(&a.E[2][c], hz);
+        if (RES) store_once(&a.Elo[c], wn);
     }
 }
 
 // Fused B side of one step: a thread walks L planes along x keeping E of the next plane in
 // registers as the centre of the following iteration.
-template <typename T, bool CPML, int L, bool LIN = false>
+template <typename T, bool CPML, int L, bool LIN = false, bool RES = false>
 __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int tby, int tbz)
 {
+    using V = typename Word<T, RES>::V;
     const GridDims &g = a.g;
     int j, k;
     tile_jk<LIN>(g, a.blo, tbx, tby, j, k);
@@ -999,16 +659,21 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
     // z neighbours: the last lane of a wave and the last cell of a row load them, every other
     // lane takes them from the next lane
     const bool zload = threadIdx.x == 63 || k == g.n[2];
-    T ezl[L][2];
+    V ezl[L][2];
     if (zload) {
 #pragma unroll
         for (int u = 0; u < L; ++u) {
             const int cu = c + u * sx;
-            ezl[u][0] = i0 + u <= i1 ? gld(Ey + cu + okp) : (T)0;
-            ezl[u][1] = i0 + u <= i1 ? gld(Ex + cu + okp) : (T)0;
+            ezl[u][0] = ezl[u][1] = (V)0;
+            if (i0 + u <= i1) {
+                const unsigned w = lo_ld<RES>(a.Elo, cu + okp);
+                ezl[u][0] = rd<T, RES>(Ey, w, 1, cu + okp);
+                ezl[u][1] = rd<T, RES>(Ex, w, 0, cu + okp);
+            }
         }
     }
-    const T ex0 = gld(Ex + c), ey0 = gld(Ey + c), ez0 = gld(Ez + c);
+    const unsigned w0 = lo_ld<RES>(a.Elo, c);
+    const V ex0 = rd<T, RES>(Ex, w0, 0, c), ey0 = rd<T, RES>(Ey, w0, 1, c), ez0 = rd<T, RES>(Ez, w0, 2, c);
     double ex = ex0, ey = ey0, ez = ez0;
     bool iny[2] = {false, false}, inz[2] = {false, false};
     if (CPML) {
@@ -1018,27 +683,29 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
     const double kdy = gld(a.ikdH[1] + j), kdz = gld(a.ikdH[2] + k);
     // the E and B values of all L planes are read before the first store, so that they leave
     // together (per plane: E at x+1, E_y and E_x at z+1, E_z and E_x at y+1, B)
-    T pe[L][7], pb[L][3];
+    V pe[L][7], pb[L][3];
 #pragma unroll
     for (int u = 0; u < L; ++u) {
         const int i = i0 + u;
         if (i > i1) break;
         const int cu = c + u * sx;
         const int cip = (i == g.n[0] && a.periodic[0]) ? cu - (g.n[0] - 1) * sx : cu + sx;
-        pe[u][0] = gld(Ex + cip);
-        pe[u][1] = gld(Ey + cip);
-        pe[u][2] = gld(Ez + cip);
-        pe[u][5] = gld(Ez + cu + ojp);
-        pe[u][6] = gld(Ex + cu + ojp);
-        pb[u][0] = load_once(&Bx[cu]);
-        pb[u][1] = load_once(&By[cu]);
-        pb[u][2] = load_once(&Bz[cu]);
+        const unsigned wx = lo_ld<RES>(a.Elo, cip), wy = lo_ld<RES>(a.Elo, cu + ojp);
+        const unsigned wbu = lo_once<RES>(a.Blo, cu);
+        pe[u][0] = rd<T, RES>(Ex, wx, 0, cip);
+        pe[u][1] = rd<T, RES>(Ey, wx, 1, cip);
+        pe[u][2] = rd<T, RES>(Ez, wx, 2, cip);
+        pe[u][5] = rd<T, RES>(Ez, wy, 2, cu + ojp);
+        pe[u][6] = rd<T, RES>(Ex, wy, 0, cu + ojp);
+        pb[u][0] = rd_once<T, RES>(Bx, wbu, 0, cu);
+        pb[u][1] = rd_once<T, RES>(By, wbu, 1, cu);
+        pb[u][2] = rd_once<T, RES>(Bz, wbu, 2, cu);
     }
 #pragma unroll
     for (int u = 0; u < L; ++u) {
         if (i0 + u > i1) break;
         // E at z+1: the next lane's centre value of the same plane
-        const T sy = lane_next(u == 0 ? ey0 : pe[u - 1][1]), sx_ = lane_next(u == 0 ? ex0 : pe[u - 1][0]);
+        const V sy = lane_next(u == 0 ? ey0 : pe[u - 1][1]), sx_ = lane_next(u == 0 ? ex0 : pe[u - 1][0]);
         pe[u][3] = zload ? ezl[u][0] : sy;
         pe[u][4] = zload ? ezl[u][1] : sx_;
     }
@@ -1061,7 +728,7 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
                 if (iny[s]) {
                     const CpmlFace &f = a.face[a.face_by_dim[1][s]];
                     double v0, v1;  // psi0 -> Mz from dEx_y, psi1 -> Mx from dEz_y
-                    cpml_face<T, true>(f, j, layer_index(f, i, j, k), dEx_y, dEz_y, v0, v1);
+                    cpml_face<T, true, RES>(f, j, layer_index(f, i, j, k), dEx_y, dEz_y, v0, v1);
                     mz += v0;
                     mx += v1;
                 }
@@ -1072,7 +739,7 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
                 if (inx[s]) {
                     const CpmlFace &f = a.face[a.face_by_dim[0][s]];
                     double v0, v1;  // psi0 -> My from dEz_x, psi1 -> Mz from dEy_x
-                    cpml_face<T, true>(f, i, layer_index(f, i, j, k), dEz_x, dEy_x, v0, v1);
+                    cpml_face<T, true, RES>(f, i, layer_index(f, i, j, k), dEz_x, dEy_x, v0, v1);
                     my += v0;
                     mz += v1;
                 }
@@ -1081,16 +748,21 @@ __device__ __forceinline__ void fused_B_tile(const StepArgs<T> &a, int tbx, int 
                 if (inz[s]) {
                     const CpmlFace &f = a.face[a.face_by_dim[2][s]];
                     double v0, v1;  // psi0 -> Mx from dEy_z, psi1 -> My from dEx_z
-                    cpml_face<T, true>(f, k, layer_index(f, i, j, k), dEy_z, dEx_z, v0, v1);
+                    cpml_face<T, true, RES>(f, k, layer_index(f, i, j, k), dEy_z, dEx_z, v0, v1);
                     mx += v0;
                     my += v1;
                 }
         }
         if (!a.sheets_first) add_sheets(a, i, j, k, mx, my, mz);
         const double kdx = gld(a.ikdH[0] + i);
-        store_once(&Bx[c], (T)((double)pb[u][0] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx)));
-        store_once(&By[c], (T)((double)pb[u][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my)));
-        store_once(&Bz[c], (T)((double)pb[u][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz)));
+        unsigned wn = 0;
+        const T hx = put<T, RES>((double)pb[u][0] + a.dt * ((dEy_z * kdz - dEz_y * kdy) + mx), wn, 0);
+        const T hy = put<T, RES>((double)pb[u][1] + a.dt * ((dEz_x * kdx - dEx_z * kdz) + my), wn, 1);
+        const T hz = put<T, RES>((double)pb[u][2] + a.dt * ((dEx_y * kdy - dEy_x * kdx) + mz), wn, 2);
+        store_once(&Bx[c], hx);
+        store_once(&By[c], hy);
+        store_once(&Bz[c], hz);
+        if (RES) store_once(&a.Blo[c], wn);
         ex = ex1;
         ey = ey1;
         ez = ez1;
@@ -1134,26 +806,26 @@ __device__ __forceinline__ void batch_tiles(const BatchRun *__restrict__ runs, i
 // Waves per SIMD the E kernel is compiled for: 6 (at most 80 registers) for the float32 Kerr
 // kernel with one pole and one plane per thread, measured 0.3 % faster than the 5 its registers
 // allow; no constraint otherwise (other variants would spill).
-template <typename T, int NP, bool NONLIN, int L>
+template <typename T, int NP, bool NONLIN, int L, bool RES>
 struct FusedEWaves {
-    static constexpr int v = (sizeof(T) == 4 && NP <= 1 && NONLIN && L == 1) ? 6 : 1;
+    static constexpr int v = (sizeof(T) == 4 && !RES && NP <= 1 && NONLIN && L == 1) ? 6 : 1;
 };
 
-template <typename T, int NP, bool NONLIN, bool CPML, int L>
-__global__ __attribute__((amdgpu_waves_per_eu(FusedEWaves<T, NP, NONLIN, L>::v))) __launch_bounds__(256) void k_fused_E(
+template <typename T, int NP, bool NONLIN, bool CPML, int L, bool RES = false>
+__global__ __attribute__((amdgpu_waves_per_eu(FusedEWaves<T, NP, NONLIN, L, RES>::v))) __launch_bounds__(256) void k_fused_E(
     const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
     fused_tile_of(pa->order, pa->band, pa->rev, nbx, nby, nbz, tbx, tby, tbz);
-    fused_E_tile<T, NP, NONLIN, CPML, L>(*pa, tbx, tby, tbz);
+    fused_E_tile<T, NP, NONLIN, CPML, L, false, RES>(*pa, tbx, tby, tbz);
 }
 
-template <typename T, bool CPML, int L>
+template <typename T, bool CPML, int L, bool RES = false>
 __global__ __launch_bounds__(256) void k_fused_B(const StepArgs<T> *__restrict__ pa, int nbx, int nby, int nbz)
 {
     int tbx, tby, tbz;
     fused_tile_of(pa->order, pa->band, pa->rev, nbx, nby, nbz, tbx, tby, tbz);
-    fused_B_tile<T, CPML, L>(*pa, tbx, tby, tbz);
+    fused_B_tile<T, CPML, L, false, RES>(*pa, tbx, tby, tbz);
 }
 
 // the fused kernels over the tiles of many independent runs, one persistent launch
@@ -1246,7 +918,7 @@ template void launch_fused_B_batch<double>(const StepArgs<double> *, const Batch
 template void launch_fused_B_batch<float>(const StepArgs<float> *, const BatchRun *, int, int, int, int, bool,
                                           hipStream_t);
 
-template <typename T, int L>
+template <typename T, int L, bool RES = false>
 static void launch_fused_E_L(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
     const GridDims &g = a.g;
@@ -1254,7 +926,7 @@ static void launch_fused_E_L(const StepArgs<T> &a, const StepArgs<T> *dev, int n
     dim3 blk(64, 4);
     dim3 grd(nbx, nby, nbz);
     bool cp = a.nfaces > 0;
-#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP, L>), grd, blk, 0, s, dev, nbx, nby, nbz)
+#define MPX_FE(NPV, NL, CP) hipLaunchKernelGGL((k_fused_E<T, NPV, NL, CP, L, RES>), grd, blk, 0, s, dev, nbx, nby, nbz)
 #define MPX_FE_NP(NL, CP)            \
     switch (np) {                    \
     case 0: MPX_FE(0, NL, CP); break; \
@@ -1274,6 +946,13 @@ static void launch_fused_E_L(const StepArgs<T> &a, const StepArgs<T> *dev, int n
 template <typename T>
 void launch_fused_E(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 4) {
+        // float32 + residual storage: one plane per thread (the march of the float32 default)
+        if (a.Elo) {
+            launch_fused_E_L<T, 1, true>(a, dev, np, nonlin, s);
+            return;
+        }
+    }
     switch (a.march) {
     case 2: launch_fused_E_L<T, 2>(a, dev, np, nonlin, s); break;
     case 4: launch_fused_E_L<T, 4>(a, dev, np, nonlin, s); break;
@@ -1281,7 +960,7 @@ void launch_fused_E(const StepArgs<T> &a, const StepArgs<T> *dev, int np, bool n
     }
 }
 
-template <typename T, int L>
+template <typename T, int L, bool RES = false>
 static void launch_fused_B_L(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s)
 {
     const GridDims &g = a.g;
@@ -1290,14 +969,21 @@ static void launch_fused_B_L(const StepArgs<T> &a, const StepArgs<T> *dev, hipSt
     dim3 blk(64, 4);
     dim3 grd(nbx, nby, nbz);
     if (a.nfaces > 0)
-        hipLaunchKernelGGL((k_fused_B<T, true, L>), grd, blk, 0, s, dev, nbx, nby, nbz);
+        hipLaunchKernelGGL((k_fused_B<T, true, L, RES>), grd, blk, 0, s, dev, nbx, nby, nbz);
     else
-        hipLaunchKernelGGL((k_fused_B<T, false, L>), grd, blk, 0, s, dev, nbx, nby, nbz);
+        hipLaunchKernelGGL((k_fused_B<T, false, L, RES>), grd, blk, 0, s, dev, nbx, nby, nbz);
 }
 
 template <typename T>
 void launch_fused_B(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 4) {
+        // float32 + residual storage: two planes per thread (the march of the float32 default)
+        if (a.Blo) {
+            launch_fused_B_L<T, 2, true>(a, dev, s);
+            return;
+        }
+    }
     switch (a.march) {
     case 2: launch_fused_B_L<T, 2>(a, dev, s); break;
     case 4: launch_fused_B_L<T, 4>(a, dev, s); break;
@@ -1418,7 +1104,10 @@ __global__ void k_probe(ProbeArgs<T> a, double *row)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
-    row[p] = (double)a.f[a.comp[p]][a.idx[p]];
+    const int q = a.comp[p];
+    const size_t c = a.idx[p];
+    const unsigned *lo = a.lo[q / 3];
+    row[p] = lo ? res_decode((float)a.f[q][c], lo[c], q % 3) : (double)a.f[q][c];
 }
 
 template <typename T>
@@ -1432,8 +1121,24 @@ template void launch_probe<float>(const ProbeArgs<float> &, double *, hipStream_
 // FieldEnergyDiag::calculate, src/fielddiag.cpp:255-286: per-block partial sums, then one
 // block adds the partials in a fixed order.
 template <typename T>
+__device__ __forceinline__ void cell3(const T *f0, const T *f1, const T *f2, const unsigned *lo, size_t c, double v[3])
+{
+    if (lo) {
+        const unsigned w = lo[c];
+        v[0] = res_decode((float)f0[c], w, 0);
+        v[1] = res_decode((float)f1[c], w, 1);
+        v[2] = res_decode((float)f2[c], w, 2);
+    } else {
+        v[0] = f0[c];
+        v[1] = f1[c];
+        v[2] = f2[c];
+    }
+}
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_energy_partial(GridDims g, const T *e0, const T *e1, const T *e2,
-                                                        const T *b0, const T *b1, const T *b2, double *part)
+                                                        const T *b0, const T *b1, const T *b2, const unsigned *elo,
+                                                        const unsigned *blo, double *part)
 {
     __shared__ double red[256];
     int k = blockIdx.x * 64 + threadIdx.x + 1;
@@ -1442,8 +1147,10 @@ __global__ __launch_bounds__(256) void k_energy_partial(GridDims g, const T *e0,
     double v = 0;
     if (k <= g.n[2] && j <= g.n[1]) {
         size_t c = g.at(i, j, k);
-        double ex = e0[c], ey = e1[c], ez = e2[c], bx = b0[c], by = b1[c], bz = b2[c];
-        v = ex * ex + ey * ey + ez * ez + bx * bx + by * by + bz * bz;
+        double e[3], b[3];
+        cell3(e0, e1, e2, elo, c, e);
+        cell3(b0, b1, b2, blo, c, b);
+        v = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
     }
     int t = threadIdx.y * 64 + threadIdx.x;
     red[t] = v;
@@ -1475,17 +1182,18 @@ size_t energy_partials(const GridDims &g)
 }
 
 template <typename T>
-void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s)
+void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s,
+                   const unsigned *elo, const unsigned *blo)
 {
     dim3 blk(64, 4);
     dim3 grd((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, g.n[0]);
-    hipLaunchKernelGGL(k_energy_partial<T>, grd, blk, 0, s, g, e[0], e[1], e[2], b[0], b[1], b[2], part);
+    hipLaunchKernelGGL(k_energy_partial<T>, grd, blk, 0, s, g, e[0], e[1], e[2], b[0], b[1], b[2], elo, blo, part);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, energy_partials(g), out);
 }
 template void launch_energy<double>(const GridDims &, double *const[3], double *const[3], double *, double *,
-                                    hipStream_t);
+                                    hipStream_t, const unsigned *, const unsigned *);
 template void launch_energy<float>(const GridDims &, float *const[3], float *const[3], double *, double *,
-                                   hipStream_t);
+                                   hipStream_t, const unsigned *, const unsigned *);
 
 }  // namespace mpx
 
@@ -1534,14 +1242,18 @@ void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s)
 // stream.  A block transposes a 32 (z) x 32 (x) tile of one y row through LDS, so that both
 // the reads (z contiguous) and the writes (x contiguous) are coalesced.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pack_xfast(GridDims g, const T *__restrict__ src, double *__restrict__ dst)
+__global__ __launch_bounds__(256) void k_pack_xfast(GridDims g, const T *__restrict__ src, const unsigned *lo, int q,
+                                                    double *__restrict__ dst)
 {
     __shared__ double tile[32][33];
     const int nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32, j = blockIdx.z;
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int i = i0 + r, k = k0 + threadIdx.x;
-        if (i < nx2 && k < nz2) tile[r][threadIdx.x] = (double)src[g.at(i, j, k)];
+        if (i < nx2 && k < nz2) {
+            const size_t c = g.at(i, j, k);
+            tile[r][threadIdx.x] = lo ? res_decode((float)src[c], lo[c], q) : (double)src[c];
+        }
     }
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += 8) {
@@ -1551,21 +1263,22 @@ __global__ __launch_bounds__(256) void k_pack_xfast(GridDims g, const T *__restr
 }
 
 template <typename T>
-void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s)
+void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s, const unsigned *lo, int q)
 {
     dim3 grd((g.n[2] + 2 + 31) / 32, (g.n[0] + 2 + 31) / 32, g.n[1] + 2);
-    hipLaunchKernelGGL(k_pack_xfast<T>, grd, dim3(32, 8), 0, s, g, src, dst);
+    hipLaunchKernelGGL(k_pack_xfast<T>, grd, dim3(32, 8), 0, s, g, src, lo, q, dst);
 }
-template void launch_pack_xfast<float>(const GridDims &, const float *, double *, hipStream_t);
-template void launch_pack_xfast<double>(const GridDims &, const double *, double *, hipStream_t);
+template void launch_pack_xfast<float>(const GridDims &, const float *, double *, hipStream_t, const unsigned *, int);
+template void launch_pack_xfast<double>(const GridDims &, const double *, double *, hipStream_t, const unsigned *, int);
 
 // One record of a sampler: the value at storage cell c (dim < 0) or the len = N[dim] + 2 values
 // of the storage line through c along dim, as float64.  A coordinate of a dimension in
 // wrapmask that names a ghost reads its periodic image (0 -> N, N + 1 -> 1), which is what the
 // ghost holds once the wrap has run; the caller checks 0 <= c[d] <= N[d] + 1.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sample_gather(GridDims g, const T *__restrict__ src, int dim, int c0, int c1,
-                                                      int c2, int len, int wrapmask, double *__restrict__ dst)
+__global__ __launch_bounds__(256) void k_sample_gather(GridDims g, const T *__restrict__ src, const unsigned *lo, int qc,
+                                                      int dim, int c0, int c1, int c2, int len, int wrapmask,
+                                                      double *__restrict__ dst)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= len) return;
@@ -1573,37 +1286,26 @@ __global__ __launch_bounds__(256) void k_sample_gather(GridDims g, const T *__re
     if (dim >= 0) c[dim] = q;
     for (int d = 0; d < 3; ++d)
         if ((wrapmask >> d) & 1) c[d] = c[d] == 0 ? g.n[d] : (c[d] == g.n[d] + 1 ? 1 : c[d]);
-    dst[q] = (double)src[g.at(c[0], c[1], c[2])];
+    const size_t at = g.at(c[0], c[1], c[2]);
+    dst[q] = lo ? res_decode((float)src[at], lo[at], qc) : (double)src[at];
 }
 
 template <typename T>
 void launch_sample_gather(const GridDims &g, const T *src, int dim, const int *c, int len, int wrapmask, double *dst,
-                         hipStream_t s)
+                         hipStream_t s, const unsigned *lo, int q)
 {
-    hipLaunchKernelGGL(k_sample_gather<T>, dim3((len + 255) / 256), dim3(256), 0, s, g, src, dim, c[0], c[1], c[2], len,
-                       wrapmask, dst);
+    hipLaunchKernelGGL(k_sample_gather<T>, dim3((len + 255) / 256), dim3(256), 0, s, g, src, lo, q, dim, c[0], c[1],
+                       c[2], len, wrapmask, dst);
 }
 template void launch_sample_gather<float>(const GridDims &, const float *, int, const int *, int, int, double *,
-                                         hipStream_t);
+                                         hipStream_t, const unsigned *, int);
 template void launch_sample_gather<double>(const GridDims &, const double *, int, const int *, int, int, double *,
-                                          hipStream_t);
+                                          hipStream_t, const unsigned *, int);
 
 }  // namespace mpx
 
 namespace mpx {
 
-// add the current-sheet layers of list `sh` that contain cell (i, j, k), in list order
-__device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py, int i, int j, int k, double &x,
-                                                 double &y, double &z)
-{
-    for (int s = 0; s < n; ++s)
-        if (k == sh[s].kp) {
-            int q = i * py + j;
-            x += sh[s].J[0][q];
-            y += sh[s].J[1][q];
-            z += sh[s].J[2][q];
-        }
-}
 
 // ------------------------------------------------------------------------------------
 // single-pass sweep path (periodic boundaries): E and B of a step in one kernel

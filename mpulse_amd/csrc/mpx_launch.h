@@ -36,6 +36,7 @@ template <typename T>
 struct ProbeArgs {
     int n;
     const T *f[6];
+    const unsigned *lo[2];  // residual words of E and B (float32 + residual storage), else null
     const int *comp;
     const size_t *idx;
 };
@@ -80,12 +81,15 @@ template <typename T>
 void launch_probe(const ProbeArgs<T> &a, double *row, hipStream_t s);
 size_t energy_partials(const GridDims &g);
 void launch_hbm_copy(const void *src, void *dst, size_t bytes, hipStream_t s);
+// lo / q: the residual words and slot of a float32 + residual grid (null otherwise)
 template <typename T>
-void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s);
+void launch_pack_xfast(const GridDims &g, const T *src, double *dst, hipStream_t s, const unsigned *lo = nullptr,
+                       int q = 0);
 template <typename T>
 void launch_sample_gather(const GridDims &g, const T *src, int dim, const int *c, int len, int wrapmask, double *dst,
-                         hipStream_t s);
+                         hipStream_t s, const unsigned *lo = nullptr, int q = 0);
 template <typename T>
-void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s);
+void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s,
+                   const unsigned *elo = nullptr, const unsigned *blo = nullptr);
 
 }  // namespace mpx

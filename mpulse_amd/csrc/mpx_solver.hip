@@ -58,6 +58,8 @@ struct Layer {
     int dir;
     int lo[3], hi[3];
     void *comp[3];  // device arrays (nullptr where the component is never written)
+    unsigned *rw = nullptr;  // float32 + residual storage of a CPML auxiliary: the face's residual words
+    int slot = 0;            // and this component's slot in them
     size_t count() const
     {
         return (size_t)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
@@ -106,6 +108,7 @@ struct mpx_solver {
     int solver = 0, prec = 0, path = 0;
     bool plrc = false, nonlin = false;
     size_t esz = 8;
+    bool res = false;  // float32 + residual storage (MPX_PREC_FP32_RES)
     GridDims g{};
     size_t cells = 0;  // padded element count of one grid
     hipStream_t stream = nullptr;
@@ -115,6 +118,7 @@ struct mpx_solver {
     std::vector<void *> allocs;
     void *E[3] = {nullptr, nullptr, nullptr};
     void *B[3] = {nullptr, nullptr, nullptr};
+    unsigned *Elo = nullptr, *Blo = nullptr;  // residual words of E and B (float32 + residual storage)
     double2 *psi[3][3] = {};
     int np = 0;
     int pole_ref[3] = {-1, -1, -1};  // active pole slot -> reference pole index
@@ -488,6 +492,17 @@ static void focused_plane(const mpx_factory &f, int nx, int ny, double dx, doubl
 extern "C" {
 
 const char *mpx_last_error(void) { return g_err.c_str(); }
+
+double mpx_store_round(double x, int precision)
+{
+    if (precision == MPX_PREC_FP32_BULK) return (double)(float)x;
+    if (precision == MPX_PREC_FP32_RES) {
+        unsigned w = 0;
+        const float hi = mpx::res_encode(x, w, 0);
+        return mpx::res_decode(hi, w, 0);
+    }
+    return x;
+}
 int mpx_abi_version(void) { return MPX_ABI_VERSION; }
 
 void mpx_factory_defaults(mpx_factory *f, int kind)
@@ -556,7 +571,8 @@ int mpx_create(const mpx_desc *desc, mpx_solver **out)
         return fail(MPX_ERR_INVALID, "unknown boundary");
     if (desc->plasma && desc->solver != MPX_SOLVER_PLRC_NONLIN && desc->solver != MPX_SOLVER_PLRC_LIN)
         return fail(MPX_ERR_UNSUPPORTED, "the plasma block belongs to the FDTD_PLRC solvers");
-    if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK)
+    if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK &&
+        desc->precision != MPX_PREC_FP32_RES)
         return fail(MPX_ERR_INVALID, "unknown precision");
     if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE && desc->path != MPX_PATH_SWEEP)
         return fail(MPX_ERR_INVALID, "unknown kernel path");
@@ -590,7 +606,10 @@ static int build(mpx_solver *s, const mpx_desc *desc)
     s->path = desc->path;
     s->plrc = (s->solver == MPX_SOLVER_PLRC_NONLIN || s->solver == MPX_SOLVER_PLRC_LIN);
     s->nonlin = (s->solver == MPX_SOLVER_PLRC_NONLIN);
-    s->esz = (s->prec == MPX_PREC_FP32_BULK) ? 4 : 8;
+    if (s->prec != MPX_PREC_FP64 && s->prec != MPX_PREC_FP32_BULK && s->prec != MPX_PREC_FP32_RES)
+        return fail(MPX_ERR_INVALID, "unknown precision " + std::to_string(s->prec));
+    s->res = s->prec == MPX_PREC_FP32_RES;
+    s->esz = (s->prec == MPX_PREC_FP64) ? 8 : 4;
     s->delta[0] = desc->dx;
     s->delta[1] = desc->dy;
     s->delta[2] = desc->dz;
@@ -605,6 +624,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         return fail(MPX_ERR_INVALID, "FDTD_PLRCSolver: Omega should not be zero!");  // src/fdtd_plrc.t:20-24
     if (s->plrc && s->path == MPX_PATH_REFERENCE && s->prec != MPX_PREC_FP64)
         return fail(MPX_ERR_UNSUPPORTED, "the reference-structured path runs in float64 only");
+    if (s->res && s->path == MPX_PATH_SWEEP)
+        return fail(MPX_ERR_UNSUPPORTED, "float32 + residual storage runs on the fused path");
     if (s->path == MPX_PATH_SWEEP && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
         return fail(MPX_ERR_UNSUPPORTED, "the sweep path needs a PLRC solver and the single_periodic boundary");
     for (int n = 0; n < 3; ++n)
@@ -647,6 +668,11 @@ static int build(mpx_solver *s, const mpx_desc *desc)
             s->E[c] = s->dalloc<float>(s->cells, rc);
             s->B[c] = s->dalloc<float>(s->cells, rc);
         }
+        if (rc) return rc;
+    }
+    if (s->res) {
+        s->Elo = s->dalloc<unsigned>(s->cells, rc);
+        s->Blo = s->dalloc<unsigned>(s->cells, rc);
         if (rc) return rc;
     }
 
@@ -761,6 +787,13 @@ static int build(mpx_solver *s, const mpx_desc *desc)
                         int comp = w == 0 ? F.c0 : F.c1;
                         s->layers[lc[comp]].comp[0] = p;  // each component layer owns one array
                         if (w == 0) F.psi0 = p; else F.psi1 = p;
+                    }
+                    if (s->res) {
+                        F.rw = s->dalloc<unsigned>(cnt, rc);
+                        if (rc) return rc;
+                        s->layers[lc[F.c0]].rw = s->layers[lc[F.c1]].rw = F.rw;
+                        s->layers[lc[F.c0]].slot = 0;
+                        s->layers[lc[F.c1]].slot = 1;
                     }
                     std::vector<double> bb, cc;
                     cpml_coeff(f, c.dir, isH, F.lo[dim], F.hi[dim], desc->dt, bb, cc);
@@ -1058,6 +1091,8 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
         a.B[d] = (T *)s->B[d];
         a.blo[d] = s->periodic[d] ? 1 : 0;
     }
+    a.Elo = s->Elo;
+    a.Blo = s->Blo;
     for (int n = 0; n < 3; ++n)
         for (int c = 0; c < 3; ++c) a.psi[n][c] = s->psi[n][c];
     a.dt = s->d.dt;
@@ -1211,6 +1246,8 @@ static int record_probes(mpx_solver *s)
             p.f[c] = (const float *)s->E[c];
             p.f[3 + c] = (const float *)s->B[c];
         }
+        p.lo[0] = s->Elo;
+        p.lo[1] = s->Blo;
         p.comp = s->d_probe_comp;
         p.idx = s->d_probe_idx;
         mpx::launch_probe<float>(p, row, s->stream);
@@ -1482,6 +1519,11 @@ static int materialise_ghosts(mpx_solver *s)
             if ((rc = wrap_group<float>(s, s->E, K_WRAP_E))) return rc;
             if ((rc = wrap_group<float>(s, s->B, K_WRAP_B))) return rc;
         }
+        if (s->res) {
+            void *lo[2][3] = {{s->Elo, nullptr, nullptr}, {s->Blo, nullptr, nullptr}};
+            if ((rc = wrap_group<unsigned>(s, lo[0], K_WRAP_E))) return rc;
+            if ((rc = wrap_group<unsigned>(s, lo[1], K_WRAP_B))) return rc;
+        }
     }
     return MPX_OK;
 }
@@ -1574,6 +1616,28 @@ int mpx_sync(mpx_solver *s)
 }
 
 }  // extern "C"
+
+// the residual words and slot of grid ptr when it is stored as float32 + residual (E and B)
+static const unsigned *res_of(const mpx_solver *s, const void *ptr, int *slot)
+{
+    if (!s->res || !ptr) return nullptr;
+    for (int q = 0; q < 3; ++q) {
+        *slot = q;
+        if (ptr == s->E[q]) return s->Elo;
+        if (ptr == s->B[q]) return s->Blo;
+    }
+    return nullptr;
+}
+
+// host copy of a device array of `n` residual words
+static int fetch_words(mpx_solver *s, const unsigned *d, size_t n, std::vector<unsigned> &out)
+{
+    out.resize(n);
+    if (hipMemcpyAsync(out.data(), d, n * sizeof(unsigned), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return fail(MPX_ERR_HIP, "copy of residual words failed");
+    return MPX_OK;
+}
 
 // resolve a full-storage grid name; *elem = element size; nullptr + MPX_OK means "all zero"
 static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, bool *isPsi, int *psiPart)
@@ -1709,6 +1773,10 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
     std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
+    int slot = 0;
+    const unsigned *dlo = res_of(s, ptr, &slot);
+    std::vector<unsigned> lo;
+    if (dlo && (rc = fetch_words(s, dlo, s->cells, lo))) return rc;
     for (size_t i = 0; i < nx2; ++i)
         for (size_t j = 0; j < ny2; ++j)
             for (size_t k = 0; k < nz2; ++k) {
@@ -1718,6 +1786,8 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
                     host[dst] = ((const double *)buf.data())[2 * mpx::psi_at(src) + part];
                 else if (elem == 8)
                     host[dst] = ((const double *)buf.data())[src];
+                else if (dlo)
+                    host[dst] = mpx::res_decode(((const float *)buf.data())[src], lo[src], slot);
                 else
                     host[dst] = ((const float *)buf.data())[src];
             }
@@ -1769,10 +1839,12 @@ int mpx_fetch_field(mpx_solver *s, const char *name, double *host, size_t count,
         if (!(s->d_stage = s->dalloc<double>(count, rc))) return rc;
     }
     if ((rc = materialise_ghosts(s))) return rc;
+    int slot = 0;
+    const unsigned *dlo = res_of(s, ptr, &slot);
     if (elem == 8)
         mpx::launch_pack_xfast<double>(g, (const double *)ptr, s->d_stage, s->stream);
     else
-        mpx::launch_pack_xfast<float>(g, (const float *)ptr, s->d_stage, s->stream);
+        mpx::launch_pack_xfast<float>(g, (const float *)ptr, s->d_stage, s->stream, dlo, slot);
     MPX_HIP(hipGetLastError());
     MPX_HIP(hipMemcpyAsync(host, s->d_stage, count * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     mpx_fetch *f = new mpx_fetch;
@@ -1840,6 +1912,10 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
     std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
+    int slot = 0;
+    unsigned *dlo = const_cast<unsigned *>(res_of(s, ptr, &slot));
+    std::vector<unsigned> lo;
+    if (dlo && (rc = fetch_words(s, dlo, s->cells, lo))) return rc;
     for (size_t i = 0; i < nx2; ++i)
         for (size_t j = 0; j < ny2; ++j)
             for (size_t k = 0; k < nz2; ++k) {
@@ -1849,10 +1925,15 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
                     ((double *)buf.data())[2 * mpx::psi_at(dst) + part] = host[src];
                 else if (elem == 8)
                     ((double *)buf.data())[dst] = host[src];
-                else
+                else if (dlo) {
+                    unsigned w = 0;
+                    ((float *)buf.data())[dst] = mpx::res_encode(host[src], w, slot);
+                    lo[dst] = (lo[dst] & ~(0x3ffu << (10 * slot))) | w;
+                } else
                     ((float *)buf.data())[dst] = (float)host[src];
             }
     MPX_HIP(hipMemcpyAsync(ptr, buf.data(), buf.size(), hipMemcpyHostToDevice, s->stream));
+    if (dlo) MPX_HIP(hipMemcpyAsync(dlo, lo.data(), lo.size() * sizeof(unsigned), hipMemcpyHostToDevice, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     return MPX_OK;
 }
@@ -1879,8 +1960,13 @@ int mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_t
             std::vector<unsigned char> buf(count * elem);
             MPX_HIP(hipMemcpyAsync(buf.data(), L.comp[0], buf.size(), hipMemcpyDeviceToHost, s->stream));
             MPX_HIP(hipStreamSynchronize(s->stream));
+            std::vector<unsigned> lo;
+            int rc;
+            if (!sheet && L.rw && (rc = fetch_words(s, L.rw, count, lo))) return rc;
             for (size_t q = 0; q < count; ++q)
-                host[q] = elem == 8 ? ((const double *)buf.data())[q] : ((const float *)buf.data())[q];
+                host[q] = elem == 8 ? ((const double *)buf.data())[q]
+                                    : (lo.empty() ? (double)((const float *)buf.data())[q]
+                                                  : mpx::res_decode(((const float *)buf.data())[q], lo[q], L.slot));
             return MPX_OK;
         }
     return fail(MPX_ERR_NAME, std::string("no layer ") + name + " on direction " + std::to_string(dir));
@@ -1999,9 +2085,14 @@ int mpx_sampler_record(mpx_sampler *p)
         if (elem == 8)
             mpx::launch_sample_gather<double>(s->g, (const double *)grid, p->dim, p->pos, p->len, p->wrapmask, dst,
                                               s->stream);
-        else
+        else {
+            int slot = 0;
+            const unsigned *dlo = res_of(s, grid, &slot);
+            if (dlo && !(s->plrc && s->path != MPX_PATH_REFERENCE))
+                return fail(MPX_ERR_UNSUPPORTED, "residual words without index wrap");
             mpx::launch_sample_gather<float>(s->g, (const float *)grid, p->dim, p->pos, p->len, p->wrapmask, dst,
-                                             s->stream);
+                                             s->stream, dlo, slot);
+        }
         MPX_HIP(hipGetLastError());
     } else {
         const GridDims &g = s->g;
@@ -2060,7 +2151,7 @@ int mpx_energy(mpx_solver *s, double *energy)
     } else {
         float *e[3] = {(float *)s->E[0], (float *)s->E[1], (float *)s->E[2]};
         float *b[3] = {(float *)s->B[0], (float *)s->B[1], (float *)s->B[2]};
-        mpx::launch_energy<float>(s->g, e, b, s->d_part, s->d_energy, s->stream);
+        mpx::launch_energy<float>(s->g, e, b, s->d_part, s->d_energy, s->stream, s->Elo, s->Blo);
     }
     MPX_LAUNCHED("energy");
     double v = 0;
@@ -2121,6 +2212,7 @@ int mpx_state_bytes(mpx_solver *s, size_t *bytes)
     size_t b = 0;
     for (void *p : s->allocs) (void)p;
     b += 6 * s->cells * s->esz;
+    if (s->res) b += 2 * s->cells * sizeof(unsigned);
     b += (size_t)s->np * mpx::psi_block_elems(s->cells) * 16;
     *bytes = b;
     return MPX_OK;
@@ -2462,6 +2554,8 @@ int mpx_batch_create(const mpx_desc *descs, int nruns, mpx_batch **out)
         if (d.solver != MPX_SOLVER_PLRC_NONLIN && d.solver != MPX_SOLVER_PLRC_LIN)
             return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs must use an FDTD_PLRC solver"));
         if (d.plasma) return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take the plasma block"));
+        if (d.precision == MPX_PREC_FP32_RES)
+            return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs store float64 or float32 (no residual words)"));
         for (int q = 0; q < d.n_factories; ++q)
             if (d.factories[q].kind == MPX_FACTORY_PLASMA_CURRENT)
                 return bail(fail(MPX_ERR_UNSUPPORTED, "batched runs do not take a plasma current"));

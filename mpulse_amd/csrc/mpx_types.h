@@ -39,6 +39,40 @@ __host__ __device__ __forceinline__ size_t psi_at(size_t c)
 // double2 elements of one pole's block over `cells` storage cells
 inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
 
+// Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word
+// hi = fl32(x) plus a 10-bit two's-complement residual q = rint((x - hi) 2^(160 - e)) in
+// units of 2^-10 of hi's ulp (e = biased exponent of hi, at least 1), clamped to [-512, 511];
+// the stored value hi + q 2^(e - 160) is within 2^-10 ulp(hi) of x.  The residuals of the
+// components of one cell (E or B: three, a CPML face: its two auxiliaries) share a 32-bit
+// word, slot q in bits 10q .. 10q + 9.  Both helpers run on the host (get/set of fields) and
+// on the device and give the same bits.
+__host__ __device__ __forceinline__ double res_pow2(int e)
+{
+    return __builtin_bit_cast(double, (unsigned long long)(e + 1023) << 52);  // 2^e, normal range
+}
+__host__ __device__ __forceinline__ int res_exp(float hi)
+{
+    const int e = (int)((__builtin_bit_cast(unsigned, hi) >> 23) & 0xffu);
+    return e < 1 ? 1 : e;
+}
+__host__ __device__ __forceinline__ double res_decode(float hi, unsigned w, int q)
+{
+    const int r = (int)(w << (22 - 10 * q)) >> 22;  // sign-extended slot q
+    return (double)hi + (double)r * res_pow2(res_exp(hi) - 160);
+}
+// the float32 word of x; its residual is OR-ed into slot q of w (w starts at zero)
+__host__ __device__ __forceinline__ float res_encode(double x, unsigned &w, int q)
+{
+    const float hi = (float)x;
+    const int e = res_exp(hi);
+    if (e < 255) {  // finite word (an infinite or NaN word keeps q = 0)
+        double r = rint((x - (double)hi) * res_pow2(160 - e));
+        r = r > 511.0 ? 511.0 : (r < -512.0 ? -512.0 : r);
+        w |= ((unsigned)(int)r & 0x3ffu) << (10 * q);
+    }
+    return hi;
+}
+
 // One CPML border current (a face of CPMLBorder{E,H}Current): its layer box, its two live
 // auxiliaries, and the b/c coefficient lines indexed by the normal coordinate.
 struct CpmlFace {
@@ -47,6 +81,7 @@ struct CpmlFace {
     int n1, n2;       // box extents of dims 1 and 2
     void *psi0;       // component psi0 (see the face map of CPMLBorder*Current::initStorage)
     void *psi1;
+    unsigned *rw;     // float32 + residual storage: residual words (slot 0 psi0, slot 1 psi1), else null
     int c0, c1;       // current components fed by psi0 / psi1
     int f0, f1;       // field components read for psi0 (f1 difference) / psi1 (f0 difference)
     const double *b;  // [normal - lo[dim]]
@@ -121,6 +156,7 @@ struct StepArgs {
     int periodic[3];
     T *E[3];
     T *B[3];
+    unsigned *Elo, *Blo;  // float32 + residual storage: residual words of E and B, else null
     double2 *psi[3][3];  // [active pole][component]
     const double *kdE[3];  // kappaE * delta per dimension (lines 0..n+1)
     const double *kdH[3];

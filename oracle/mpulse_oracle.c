@@ -1409,13 +1409,13 @@ static void current_step(or_sim *s, current_t *c)
     switch (c->type) {
     case CUR_CPML_E:
         cpml_step_E(s, c);
-        grid_round(c->J[c->psi0], s->store_mode);
-        grid_round(c->J[c->psi1], s->store_mode);
+        grid_round(c->J[c->psi0], s->store_mode == 3 ? 1 : s->store_mode);
+        grid_round(c->J[c->psi1], s->store_mode == 3 ? 1 : s->store_mode);
         break;
     case CUR_CPML_H:
         cpml_step_H(s, c);
-        grid_round(c->J[c->psi0], s->store_mode);
-        grid_round(c->J[c->psi1], s->store_mode);
+        grid_round(c->J[c->psi0], s->store_mode == 3 ? 1 : s->store_mode);
+        grid_round(c->J[c->psi1], s->store_mode == 3 ? 1 : s->store_mode);
         break;
     case CUR_INC_E: inc_step_E(s, c); break;
     case CUR_INC_H: inc_step_H(s, c); break;
@@ -1655,7 +1655,7 @@ static void plrc_stepD(or_sim *s, double dt)
                 }
                 plrc_step_d_cell(s, nonlin, dt, i, j, k, dx, dy, dz, jx, jy, jz);
             }
-    for (int c = 0; c < 3; ++c) grid_round(&s->E[c], s->store_mode);
+    for (int c = 0; c < 3; ++c) grid_round(&s->E[c], s->store_mode == 3 ? 2 : s->store_mode);
     apply_boundary(s, s->E);
 }
 
@@ -1690,7 +1690,7 @@ static void plrc_stepB(or_sim *s, double dt)
                                   dt * ((AT(Ex, i, j + 1, k) - AT(Ex, i, j, k)) / kappaHdy -
                                         (AT(Ey, i + 1, j, k) - AT(Ey, i, j, k)) / kappaHdx + jz);
             }
-    for (int c = 0; c < 3; ++c) grid_round(&s->B[c], s->store_mode);
+    for (int c = 0; c < 3; ++c) grid_round(&s->B[c], s->store_mode == 3 ? 2 : s->store_mode);
     apply_boundary(s, s->B);
 }
 
@@ -2198,7 +2198,7 @@ int or_get_time(or_sim *s) { return s->time; }
 
 int or_set_storage(or_sim *s, int mode)
 {
-    if (mode < 0 || mode > 2) return -1;
+    if (mode < 0 || mode > 3) return -1;
     s->store_mode = mode;
     return 0;
 }
