@@ -74,6 +74,9 @@ extern "C" {
 #define MPX_PATH_FUSED     0  /* one fused E-side and one fused B-side kernel per step */
 #define MPX_PATH_REFERENCE 1  /* one kernel per reference loop: currents, totals, update, wrap */
 #define MPX_PATH_SWEEP     2  /* one kernel per step (E and B in a single x sweep; periodic boundary) */
+#define MPX_PATH_PASS      3  /* one kernel per step: B of step n and E of step n+1 in one x sweep, so E,
+                                 B and the accumulators cross HBM once per step; the first E half and
+                                 the B half owed at a read run on the fused kernels; periodic boundary */
 
 /* CurrentFactory registry keys, src/fdtd_plrc.t:431-469 */
 #define MPX_FACTORY_CPML          1  /* "cpml_border"          src/cpml_border.cpp:14-140   */
@@ -247,6 +250,12 @@ int  mpx_sampler_destroy(mpx_sampler *p);
 /* sum over interior cells of (E^2 + B^2) * dx*dy*dz  (src/fielddiag.cpp:255-286) */
 int  mpx_energy(mpx_solver *s, double *energy);
 
+/* Waits for the queued steps.  Returns MPX_ERR_NUMERIC (and clears the condition) when, since
+   the last check, a Kerr Newton solve stopped at its iteration cap (tuning key "newton_cap",
+   default 200; the reference iterates without bound, src/fdtd_plrc.cpp:406-425) or an E update
+   met a non-finite value, in its inputs or its result (the reference goes on silently: with
+   Kerr it stores E = 0 where C is NaN, src/fdtd_plrc.cpp:405-424, and the accumulators keep
+   the NaN); mpx_last_error says which.  The state is left as computed. */
 int  mpx_sync(mpx_solver *s);
 /* Run nsteps with per-kernel HIP events on the solver's stream. */
 int  mpx_time_steps(mpx_solver *s, int nsteps, mpx_timing *out);
@@ -262,7 +271,8 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
    height from the grid); "b_reverse" 1, the default, walks that order backwards in the B-side
    kernel, so it starts on the tiles the E-side kernel wrote last (0: forwards).  No
    counterpart in the reference: the hook of
-   the tile autotuner. */
+   the tile autotuner.  "newton_cap" (>= 1, default 200) bounds the Kerr Newton iterations of a
+   cell (see mpx_sync). */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);
 
 /* Streaming copy of `bytes` between two fresh device buffers, `reps` times; mean ms per copy.

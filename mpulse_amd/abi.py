@@ -19,7 +19,7 @@ OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NAME, ERR_NUMERIC = 0,
 SOLVER_PLRC_NONLIN, SOLVER_PLRC_LIN, SOLVER_PLAIN, SOLVER_NL, SOLVER_DISP = 0, 1, 2, 3, 4
 BOUNDARY_PERIODIC, BOUNDARY_XY_PERIODIC = 0, 1
 PREC_FP64, PREC_FP32_BULK, PREC_FP32_RES = 0, 1, 2
-PATH_FUSED, PATH_REFERENCE, PATH_SWEEP = 0, 1, 2
+PATH_FUSED, PATH_REFERENCE, PATH_SWEEP, PATH_PASS = 0, 1, 2, 3
 FACTORY_CPML, FACTORY_CPML_1D, FACTORY_SHORT_PULSE, FACTORY_PLANE_WAVE, FACTORY_PLANE_GAUSS = 1, 2, 3, 4, 5
 FACTORY_FOCUSED_PULSE = 6
 FACTORY_PLASMA_CURRENT = 7

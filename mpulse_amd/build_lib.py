@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmpulse_hip.so")
-SOURCES = ["mpx_kernels.hip", "mpx_adjoint.hip", "mpx_gauss.hip", "mpx_solver.hip"]
+SOURCES = ["mpx_kernels.hip", "mpx_pass.hip", "mpx_adjoint.hip", "mpx_gauss.hip", "mpx_solver.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

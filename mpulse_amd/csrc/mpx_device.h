@@ -151,7 +151,8 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
                                                  double kdx, double kdy, double kdz, double jx, double jy,
                                                  double jz, const PIN &psi_in, const POUT &psi, size_t c,
                                                  bool store, const PoleConsts &pc, double dt, double denom,
-                                                 double numer, double chi)
+                                                 double numer, double chi, int ncap = kNewtonCap,
+                                                 unsigned *flag = nullptr)
 {
     double Psix = 0, Psiy = 0, Psiz = 0;
     const size_t pq = psi_at(c);
@@ -180,6 +181,9 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
         double E = sqrt(E2);
         double A = qt<RECIP>(chi, denom);
         double Cm = sqrt(cx * cx + cy * cy + cz * cz);
+        // a non-finite input makes Cm NaN: the reference then stores E = 0 (its C > 0 test
+        // fails) and the accumulators keep the NaN; reproduced, and reported
+        if (flag && !isfinite(Cm)) *flag |= 2u;
         if (Cm > 0) {
             // the reference's stopping rule |E - Eold| <= 1e-9; the cap only guards a cell that
             // could never meet it (non-finite data, or fields far from O(1))
@@ -189,7 +193,8 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
                 Eold = E;
                 double Q = E * E;
                 E = (2 * A * Q * E + Cm) / (3 * A * Q + 1);
-            } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
+            } while (fabs(E - Eold) > 1e-9 && ++it < ncap);
+            if (flag && it >= ncap) *flag |= 1u;  // stopped by the cap, not by the rule
             if (RECIP) {
                 const double sc = E / Cm;
                 ex = sc * cx;
@@ -256,10 +261,26 @@ __device__ __forceinline__ void plrc_e_update(double &ex, double &ey, double &ez
                                               double dBx_z, double dBz_x, double dBy_x, double dBx_y,
                                               double kdx, double kdy, double kdz, double jx, double jy, double jz,
                                               double2 *const psi[3][3], size_t c, const PoleConsts &pc,
-                                              double dt, double denom, double numer, double chi)
+                                              double dt, double denom, double numer, double chi, int ncap,
+                                              unsigned *flag)
 {
     plrc_e_update_pp<NP, NONLIN, RECIP>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kdx, kdy, kdz, jx, jy, jz, psi,
-                                 psi, c, true, pc, dt, denom, numer, chi);
+                                 psi, c, true, pc, dt, denom, numer, chi, ncap, flag);
+}
+
+// Device status of a solver (MPX_ERR_NUMERIC): bit 0 a Kerr Newton solve stopped at the
+// iteration cap, bit 1 a non-finite value in an E update (its inputs or its result).  flag holds this lane's bits; one lane of the wave ORs
+// the wave's bits into *status (a vector atomic), so a wave costs at most one atomic.
+__device__ __forceinline__ unsigned finite_flag(double x, double y, double z)
+{
+    return (isfinite(x) && isfinite(y) && isfinite(z)) ? 0u : 2u;
+}
+__device__ __forceinline__ void report(unsigned *status, unsigned flag)
+{
+    const unsigned long long b1 = __ballot((flag & 1u) != 0), b2 = __ballot((flag & 2u) != 0);
+    const unsigned long long any = b1 | b2;
+    if (any && status && (int)__lane_id() == __ffsll((unsigned long long)any) - 1)
+        atomicOr(status, (b1 ? 1u : 0u) | (b2 ? 2u : 0u));
 }
 
 

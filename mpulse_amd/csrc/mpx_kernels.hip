@@ -344,8 +344,10 @@ __global__ __launch_bounds__(256) void k_ref_stepD(StepArgs<double> a)
     double ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
     double denom, numer;
     cell_eps<NONLIN>(a, c, a.plasma ? 0.5 * a.sigma[c] * a.dt : 0.0, denom, numer);
+    unsigned fl = 0;
     plrc_e_update<NP, NONLIN, false>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, a.kdE[0][i], a.kdE[1][j],
-                                     a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, denom, numer, a.chi);
+                                     a.kdE[2][k], jx, jy, jz, a.psi, c, a.pc, a.dt, denom, numer, a.chi, a.ncap, &fl);
+    report(a.status, fl | finite_flag(ex, ey, ez));
     a.E[0][c] = ex;
     a.E[1][c] = ey;
     a.E[2][c] = ez;
@@ -626,9 +628,11 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
             cell_eps<NONLIN>(a, (size_t)c, 0.5 * sg * a.dt, denom, numer);
             idenom = 1.0 / denom;
         }
+        unsigned fl = 0;
         plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i), kdy,
                                            kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom, numer,
-                                           a.chi);
+                                           a.chi, a.ncap, &fl);
+        report(a.status, fl | finite_flag(ex, ey, ez));
         unsigned wn = 0;
         const T hx = put<T, RES>(ex, wn, 0), hy = put<T, RES>(ey, wn, 1), hz = put<T, RES>(ez, wn, 2);
         store_once(&a.E[0][c], hx);
@@ -978,9 +982,12 @@ template <typename T>
 void launch_fused_B(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
-        // float32 + residual storage: two planes per thread (the march of the float32 default)
+        // float32 + residual storage: one or two planes per thread
         if (a.Blo) {
-            launch_fused_B_L<T, 2, true>(a, dev, s);
+            if (a.march == 1)
+                launch_fused_B_L<T, 1, true>(a, dev, s);
+            else
+                launch_fused_B_L<T, 2, true>(a, dev, s);
             return;
         }
     }
@@ -1034,10 +1041,13 @@ __global__ __launch_bounds__(256) void k_simple_stepD(SimpleArgs a)
             Eold = E;
             E2 = E * E;
             E = (2 * a.chi * E2 * E + D) / (3 * a.chi * E2 + a.eps);
-        } while (fabs(E - Eold) > 1e-9 && ++it < kNewtonCap);
+        } while (fabs(E - Eold) > 1e-9 && ++it < a.ncap);
         Ex[c] = E * Dx / D;
         Ey[c] = E * Dy / D;
         Ez[c] = E * Dz / D;
+        // a zero D gives NaN, as in the reference (src/fdtd_nl.cpp:86-89): reported, not avoided
+        report(a.status, (it >= a.ncap ? 1u : 0u) | finite_flag(Ex[c], Ey[c], Ez[c]));
+        return;
     } else {
         double e[3] = {ex, ey, ez};
         double P[3] = {0, 0, 0}, Pp[3] = {0, 0, 0};
@@ -1480,9 +1490,12 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs<T> &a, int tz, int ty
             }
             if (!a.sheets_first) add_sheet_planes(a.sheet, a.nsheets, g.py, pp, j, kk, jx, jy, jz);
             double ex = (double)ce[r][0], ey = (double)ce[r][1], ez = (double)ce[r][2];
+            unsigned fl = 0;
             plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y,
                                                a.ikdE[0][pp], a.ikdE[1][j], kdz_e, jx, jy, jz, cpsi[r], a.psin, c,
-                                               store, a.pc, a.dt, a.idenom, a.numer, a.chi);
+                                               store, a.pc, a.dt, a.idenom, a.numer, a.chi, a.ncap, &fl);
+            if (store) fl |= finite_flag(ex, ey, ez);
+            report(a.status, fl);
             const T tx_ = (T)ex, ty_ = (T)ey, tz_ = (T)ez;
             if (store) {
                 a.En[0][c] = tx_;

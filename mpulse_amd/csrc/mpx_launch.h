@@ -30,6 +30,8 @@ struct SimpleArgs {
     double *Pp[3][3];
     double dx, dy, dz, dt, eps, chi;
     double eps2[3], om2[3];
+    unsigned *status;
+    int ncap;
 };
 
 template <typename T>
@@ -58,6 +60,10 @@ template <typename T>
 void launch_fused_B(const StepArgs<T> &a, const StepArgs<T> *dev, hipStream_t s);
 template <typename T>
 void launch_sweep(const SweepArgs<T> &a, int np, bool nonlin, hipStream_t s);
+// single-pass leapfrog kernel (mpx_pass.hip): blocks of 64 z cells x 4 rows of one plane
+void pass_tiles(const GridDims &g, int &nbx, int &nby, int &nbz);
+template <typename T>
+void launch_pass(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s);
 void sweep_tiles(const GridDims &g, int variant, int seg, int &ntz, int &nty, int &nseg);
 template <typename T>
 void launch_sweep_batch(const SweepArgs<T> *args, const BatchRun *runs, int nruns, int total, int variant,

@@ -95,10 +95,11 @@ struct Timer {
 };
 
 const char *kClassNames[] = {"sheet_eval", "cpml_E", "sum_J", "stepD_ref", "wrap_E", "cpml_H", "sum_M",
-                             "stepB_ref", "wrap_B", "fused_E", "fused_B", "simple_E", "simple_B", "probe", "sweep"};
+                             "stepB_ref", "wrap_B", "fused_E", "fused_B", "simple_E", "simple_B", "probe", "sweep",
+                             "pass"};
 enum {
     K_SHEET = 0, K_CPML_E, K_SUM_J, K_STEPD_REF, K_WRAP_E, K_CPML_H, K_SUM_M, K_STEPB_REF, K_WRAP_B,
-    K_FUSED_E, K_FUSED_B, K_SIMPLE_E, K_SIMPLE_B, K_PROBE, K_SWEEP, K_NCLASSES
+    K_FUSED_E, K_FUSED_B, K_SIMPLE_E, K_SIMPLE_B, K_PROBE, K_SWEEP, K_PASS, K_NCLASSES
 };
 
 }  // namespace
@@ -151,6 +152,8 @@ struct mpx_solver {
     double *d_probe_hist = nullptr;
 
     double *d_part = nullptr, *d_energy = nullptr;
+    unsigned *d_status = nullptr;  // device status (MPX_ERR_NUMERIC bits), read and cleared by mpx_sync
+    int ncap = mpx::kNewtonCap;    // Kerr Newton iteration cap
     int time = 0;
     // x planes per thread in the fused kernels (1, 2 or 4); with the pole accumulators and the
     // cell's own field streamed nontemporally, E: 1 and B: 2 are fastest in both precisions
@@ -166,12 +169,24 @@ struct mpx_solver {
     std::vector<void *> epsi2[2];      // per E face: alternate psi0 / psi1
     std::vector<int> eface_layer[2];   // per E face: layer holding psi0 / psi1
     int sweep_seg = 32, sweep_variant = 2;
+    // pass path: between calls B lags E by the B half of the last step (pending) until a read
+    // or write of the state settles it; E, B, their residual words and the CPML H
+    // auxiliaries alternate between two sets (the second: E2, B2, E2lo, B2lo, hpsi2, hrw2)
+    bool pending = false;
+    int parity = 0;
+    unsigned *E2lo = nullptr, *B2lo = nullptr;
+    std::vector<void *> hpsi2[2];      // per H face: alternate psi0 / psi1
+    std::vector<unsigned *> hrw2;      // per H face: alternate residual words
+    std::vector<int> hface_layer[2];   // per H face: layer holding psi0 / psi1
     Timer timer;
     hipStream_t owner = nullptr;  // stream of the batch that steps this run (nullptr: none)
     // device copies of the fused kernels' argument blocks (E side, B side), refreshed when the
     // host-side block changes
-    void *d_fargs[2] = {nullptr, nullptr};
-    std::vector<unsigned char> h_fargs[2];
+    // (E side, B side) x buffer parity of the pass path; the pass kernel's own per parity
+    void *d_fargs[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<unsigned char> h_fargs[4];
+    void *d_pargs[2] = {nullptr, nullptr};
+    std::vector<unsigned char> h_pargs[2];
 
     ~mpx_solver()
     {
@@ -574,7 +589,8 @@ int mpx_create(const mpx_desc *desc, mpx_solver **out)
     if (desc->precision != MPX_PREC_FP64 && desc->precision != MPX_PREC_FP32_BULK &&
         desc->precision != MPX_PREC_FP32_RES)
         return fail(MPX_ERR_INVALID, "unknown precision");
-    if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE && desc->path != MPX_PATH_SWEEP)
+    if (desc->path != MPX_PATH_FUSED && desc->path != MPX_PATH_REFERENCE && desc->path != MPX_PATH_SWEEP &&
+        desc->path != MPX_PATH_PASS)
         return fail(MPX_ERR_INVALID, "unknown kernel path");
     if (desc->n_factories < 0 || desc->n_factories > MPX_MAX_FACTORIES)
         return fail(MPX_ERR_INVALID, "too many current factories");
@@ -610,6 +626,9 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         return fail(MPX_ERR_INVALID, "unknown precision " + std::to_string(s->prec));
     s->res = s->prec == MPX_PREC_FP32_RES;
     s->esz = (s->prec == MPX_PREC_FP64) ? 8 : 4;
+    // the residual B kernel keeps its planes' decoded values in float64 registers: one plane
+    // per thread runs at 8 waves per SIMD against 5 for two (1.72 against 1.82 ms at 512^3)
+    if (s->res) s->march_b = 1;
     s->delta[0] = desc->dx;
     s->delta[1] = desc->dy;
     s->delta[2] = desc->dz;
@@ -628,6 +647,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         return fail(MPX_ERR_UNSUPPORTED, "float32 + residual storage runs on the fused path");
     if (s->path == MPX_PATH_SWEEP && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
         return fail(MPX_ERR_UNSUPPORTED, "the sweep path needs a PLRC solver and the single_periodic boundary");
+    if (s->path == MPX_PATH_PASS && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
+        return fail(MPX_ERR_UNSUPPORTED, "the pass path needs a PLRC solver and the single_periodic boundary");
     for (int n = 0; n < 3; ++n)
         if (s->plrc && !(desc->l_om2[n] > desc->l_delta[n] * desc->l_delta[n]))
             return fail(MPX_ERR_INVALID, "pole " + std::to_string(n + 1) + ": L*Om2 must exceed L*delta^2");
@@ -947,7 +968,7 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         // the plasma OptField (src/fdtd_plrc.t:431-433, PlasmaDensity) and the grids it shares
         // with the plasma current ("PlasmaDensity", "PlasmaJx".., "Sigma")
         s->plasma = desc->plasma != 0;
-        if ((s->plasma || s->pcur) && s->path == MPX_PATH_SWEEP)
+        if ((s->plasma || s->pcur) && (s->path == MPX_PATH_SWEEP || s->path == MPX_PATH_PASS))
             return fail(MPX_ERR_UNSUPPORTED, "plasma needs the fused or reference-structured path");
         if (s->plasma && desc->plasma_k < 1) return fail(MPX_ERR_INVALID, "plasma K must be >= 1");
         if (s->plasma || s->pcur) {
@@ -1050,6 +1071,41 @@ static int build(mpx_solver *s, const mpx_desc *desc)
             }
         }
     }
+    if (s->path == MPX_PATH_PASS) {
+        for (int c = 0; c < 3; ++c) {
+            if (s->esz == 8) {
+                s->E2[c] = s->dalloc<double>(s->cells, rc);
+                s->B2[c] = s->dalloc<double>(s->cells, rc);
+            } else {
+                s->E2[c] = s->dalloc<float>(s->cells, rc);
+                s->B2[c] = s->dalloc<float>(s->cells, rc);
+            }
+            if (rc) return rc;
+        }
+        if (s->res) {
+            s->E2lo = s->dalloc<unsigned>(s->cells, rc);
+            s->B2lo = s->dalloc<unsigned>(s->cells, rc);
+            if (rc) return rc;
+        }
+        for (size_t f = 0; f < s->hfaces.size(); ++f) {
+            const CpmlFace &F = s->hfaces[f];
+            const size_t cnt = (size_t)(F.hi[0] - F.lo[0] + 1) * F.n1 * F.n2;
+            for (int w = 0; w < 2; ++w) {
+                void *p = (s->esz == 8) ? (void *)s->dalloc<double>(cnt, rc) : (void *)s->dalloc<float>(cnt, rc);
+                if (rc) return rc;
+                s->hpsi2[w].push_back(p);
+                int li = -1;
+                for (size_t q = 0; q < s->layers.size(); ++q)
+                    if (s->layers[q].comp[0] == (w == 0 ? F.psi0 : F.psi1)) li = (int)q;
+                s->hface_layer[w].push_back(li);
+            }
+            unsigned *rw = nullptr;
+            if (s->res && !(rw = s->dalloc<unsigned>(cnt, rc))) return rc;
+            s->hrw2.push_back(rw);
+        }
+    }
+    s->d_status = s->dalloc<unsigned>(1, rc);
+    if (rc) return rc;
     s->d_part = s->dalloc<double>(mpx::energy_partials(g), rc);
     s->d_energy = s->dalloc<double>(1, rc);
     if (rc) return rc;
@@ -1138,6 +1194,8 @@ static StepArgs<T> step_args(mpx_solver *s, bool hside)
         if (c.type == CUR_INC_E || c.type == CUR_INC_H) a.sheet[a.nsheets++] = c.ev.sheet;
     a.sheets_first = s->sheets_first;
     a.haveJ = 0;
+    a.status = s->d_status;
+    a.ncap = s->ncap;
     a.march = hside ? s->march_b : s->march_e;
     a.order = s->tile_order;
     a.band = s->tile_band > 0 ? s->tile_band : auto_band(s->g, hside ? a.blo : nullptr, hside);
@@ -1183,24 +1241,26 @@ static mpx::SumList sum_list(mpx_solver *s, bool hside)
 
 // the step index of a sheet evaluation; a GaussInject sheet first runs its block schedule
 // (GaussInjectSourceFunc::setTime, called by IncidentSource*Current::stepScheme)
-static int sheet_time(mpx_solver *s, Current &c)
+static int sheet_time(mpx_solver *s, Current &c, int t)
 {
-    c.ev.time = s->time;
+    c.ev.time = t;
     if (c.gauss) {
-        int rc = c.gauss->setTime(s->time);
+        int rc = c.gauss->setTime(t);
         if (rc) return fail(MPX_ERR_HIP, "gauss_inject: transform of a sample block failed");
         c.gauss->params(c.ev.gauss);
-        if (c.ev.gauss.active && (s->time - c.ev.gauss.blockToff < 0 || s->time - c.ev.gauss.blockToff >= c.gauss->Nt))
+        if (c.ev.gauss.active && (t - c.ev.gauss.blockToff < 0 || t - c.ev.gauss.blockToff >= c.gauss->Nt))
             return fail(MPX_ERR_INVALID, "gauss_inject: step outside the sample block");
     }
     return MPX_OK;
 }
+static int sheet_time(mpx_solver *s, Current &c) { return sheet_time(s, c, s->time); }
 
-static int eval_sheets(mpx_solver *s, std::vector<Current> &list, int type)
+// the sheets of one side at step t (the step counter of the step they belong to)
+static int eval_sheets(mpx_solver *s, std::vector<Current> &list, int type, int t)
 {
     for (Current &c : list)
         if (c.type == type) {
-            int rc = sheet_time(s, c);
+            int rc = sheet_time(s, c, t);
             if (rc) return rc;
             s->tick(K_SHEET);
             mpx::launch_sheet_eval(c.ev, s->ft, s->stream);
@@ -1281,6 +1341,8 @@ static mpx::SimpleArgs simple_args(mpx_solver *s)
         a.eps2[n] = s->d.d_eps2[n];
         a.om2[n] = s->d.d_om2[n];
     }
+    a.status = s->d_status;
+    a.ncap = s->ncap;
     return a;
 }
 
@@ -1332,6 +1394,8 @@ static mpx::SweepArgs<T> sweep_args(mpx_solver *s)
     a.sheets_first = s->sheets_first;
     a.seg = s->sweep_seg;
     a.variant = s->sweep_variant;
+    a.status = s->d_status;
+    a.ncap = s->ncap;
     return a;
 }
 
@@ -1355,6 +1419,7 @@ static void swap_sweep_buffers(mpx_solver *s)
 template <typename T>
 static const StepArgs<T> *dev_args(mpx_solver *s, const StepArgs<T> &a, int side)
 {
+    side += 2 * s->parity;
     const unsigned char *p = (const unsigned char *)&a;
     std::vector<unsigned char> &h = s->h_fargs[side];
     if (h.size() == sizeof(a) && std::memcmp(h.data(), p, sizeof(a)) == 0) return (const StepArgs<T> *)s->d_fargs[side];
@@ -1370,6 +1435,172 @@ static const StepArgs<T> *dev_args(mpx_solver *s, const StepArgs<T> &a, int side
         hipStreamSynchronize(s->stream) != hipSuccess)
         return nullptr;
     return (const StepArgs<T> *)s->d_fargs[side];
+}
+
+// The E half (E-side sheets at step t, fused E kernel) or the B half (H-side sheets at step t,
+// fused B kernel) of step t on the fused path.
+template <typename T>
+static int fused_half_t(mpx_solver *s, bool hside)
+{
+    StepArgs<T> a = step_args<T>(s, hside);
+    const StepArgs<T> *d = dev_args(s, a, hside ? 1 : 0);
+    if (!d) return fail(MPX_ERR_HIP, "argument upload failed");
+    s->tick(hside ? K_FUSED_B : K_FUSED_E);
+    if (hside)
+        mpx::launch_fused_B<T>(a, d, s->stream);
+    else
+        mpx::launch_fused_E<T>(a, d, s->np, s->nonlin, s->stream);
+    s->tock();
+    MPX_LAUNCHED(hside ? "fused B" : "fused E");
+    return MPX_OK;
+}
+
+static int fused_half(mpx_solver *s, bool hside, int t)
+{
+    int rc = eval_sheets(s, hside ? s->hcur : s->ecur, hside ? CUR_INC_H : CUR_INC_E, t);
+    if (rc) return rc;
+    return s->esz == 8 ? fused_half_t<double>(s, hside) : fused_half_t<float>(s, hside);
+}
+
+template <typename T>
+static mpx::PassArgs<T> pass_args(mpx_solver *s)
+{
+    mpx::PassArgs<T> a{};
+    a.g = s->g;
+    for (int c = 0; c < 3; ++c) {
+        a.Eo[c] = (const T *)s->E[c];
+        a.En[c] = (T *)s->E2[c];
+        a.Bo[c] = (const T *)s->B[c];
+        a.Bn[c] = (T *)s->B2[c];
+        a.ikdE[c] = s->d_ikdE[c];
+        a.ikdH[c] = s->d_ikdH[c];
+        for (int n = 0; n < 3; ++n) a.psi[n][c] = s->psi[n][c];
+    }
+    a.Eolo = s->Elo;
+    a.Bolo = s->Blo;
+    a.Enlo = s->E2lo;
+    a.Bnlo = s->B2lo;
+    a.dt = s->d.dt;
+    a.eps = s->d.eps;
+    a.chi = s->d.chi;
+    a.denom = s->denom;
+    a.numer = s->numer;
+    a.idenom = 1.0 / s->denom;
+    a.sumChi0 = s->sumChi0;
+    a.sumXi0 = s->sumXi0;
+    a.epsf = s->d_eps;
+    a.plasma = 0;
+    a.pc = s->pc;
+    a.nefaces = (int)s->efaces.size();
+    a.nhfaces = (int)s->hfaces.size();
+    for (int d = 0; d < 3; ++d)
+        for (int q = 0; q < 2; ++q) a.efbd[d][q] = a.hfbd[d][q] = -1;
+    for (size_t f = 0; f < s->efaces.size(); ++f) a.eface[f] = s->efaces[f];
+    for (size_t f = 0; f < s->hfaces.size(); ++f) {
+        a.hface[f] = s->hfaces[f];
+        a.hface[f].psi0 = s->hpsi2[0][f];
+        a.hface[f].psi1 = s->hpsi2[1][f];
+        a.hface[f].rw = s->hrw2[f];
+        a.hold[f][0] = s->hfaces[f].psi0;
+        a.hold[f][1] = s->hfaces[f].psi1;
+        a.hold_rw[f] = s->hfaces[f].rw;
+    }
+    for (const Current &c : s->ecur)
+        if (c.face >= 0) a.efbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    for (const Current &c : s->hcur)
+        if (c.face >= 0) a.hfbd[dim_of(c.dir)][is_high(c.dir) ? 0 : 1] = c.face;
+    for (const Current &c : s->ecur)
+        if (c.type == CUR_INC_E) a.sheet[a.nsheets++] = c.ev.sheet;
+    for (const Current &c : s->hcur)
+        if (c.type == CUR_INC_H) a.hsheet[a.nhsheets++] = c.ev.sheet;
+    a.sheets_first = s->sheets_first;
+    a.band = s->tile_band > 0 ? s->tile_band : auto_band(s->g, nullptr, false);
+    a.status = s->d_status;
+    a.ncap = s->ncap;
+    return a;
+}
+
+// after a pass the new set holds the state: exchange the two sets
+static void swap_pass_buffers(mpx_solver *s)
+{
+    for (int c = 0; c < 3; ++c) {
+        std::swap(s->E[c], s->E2[c]);
+        std::swap(s->B[c], s->B2[c]);
+    }
+    std::swap(s->Elo, s->E2lo);
+    std::swap(s->Blo, s->B2lo);
+    for (size_t f = 0; f < s->hfaces.size(); ++f) {
+        std::swap(s->hfaces[f].psi0, s->hpsi2[0][f]);
+        std::swap(s->hfaces[f].psi1, s->hpsi2[1][f]);
+        std::swap(s->hfaces[f].rw, s->hrw2[f]);
+        for (int w = 0; w < 2; ++w)
+            if (s->hface_layer[w][f] >= 0) {
+                Layer &L = s->layers[s->hface_layer[w][f]];
+                L.comp[0] = w == 0 ? s->hfaces[f].psi0 : s->hfaces[f].psi1;
+                L.rw = s->hfaces[f].rw;
+            }
+    }
+    s->parity ^= 1;
+}
+
+template <typename T>
+static int pass_launch_t(mpx_solver *s)
+{
+    const mpx::PassArgs<T> a = pass_args<T>(s);
+    // device copy per buffer parity, uploaded again only when it changed
+    const int side = s->parity;
+    const unsigned char *p = (const unsigned char *)&a;
+    std::vector<unsigned char> &h = s->h_pargs[side];
+    if (!(h.size() == sizeof(a) && std::memcmp(h.data(), p, sizeof(a)) == 0)) {
+        if (!s->d_pargs[side]) {
+            if (hipMalloc(&s->d_pargs[side], sizeof(mpx::PassArgs<double>)) != hipSuccess) {
+                s->d_pargs[side] = nullptr;
+                return fail(MPX_ERR_NOMEM, "argument block allocation failed");
+            }
+            s->allocs.push_back(s->d_pargs[side]);
+        }
+        h.assign(p, p + sizeof(a));
+        if (hipMemcpyAsync(s->d_pargs[side], h.data(), sizeof(a), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess)
+            return fail(MPX_ERR_HIP, "argument upload failed");
+    }
+    s->tick(K_PASS);
+    mpx::launch_pass<T>(a, (const mpx::PassArgs<T> *)s->d_pargs[side], s->np, s->nonlin, s->stream);
+    s->tock();
+    MPX_LAUNCHED("pass");
+    return MPX_OK;
+}
+
+// One step on the pass path.  Settled state at time t (E^t, B^t): the E half of step t.
+// Pending state at time t (E^t, B^(t-1)): one pass, the B half of step t - 1 (H-side sheets
+// at t - 1) and the E half of step t (E-side sheets at t).  Either way the state is pending at
+// time t + 1 afterwards.
+static int pass_step(mpx_solver *s)
+{
+    int rc;
+    if (!s->pending) {
+        if ((rc = fused_half(s, false, s->time))) return rc;
+    } else {
+        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H, s->time - 1))) return rc;
+        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E, s->time))) return rc;
+        rc = s->esz == 8 ? pass_launch_t<double>(s) : pass_launch_t<float>(s);
+        if (rc) return rc;
+        swap_pass_buffers(s);
+    }
+    s->pending = true;
+    s->time++;
+    return MPX_OK;
+}
+
+// Completes a pending B half (pass path) so that E and B belong to the same step.  Every
+// entry point that reads or writes the state calls it first.
+static int settle(mpx_solver *s)
+{
+    if (!s->pending) return MPX_OK;
+    int rc = fused_half(s, true, s->time - 1);
+    if (rc) return rc;
+    s->pending = false;
+    return MPX_OK;
 }
 
 // one full step: FDTD_PLRCSolver::stepScheme (src/fdtd_plrc.t:127-168) or the plain/nl/disp
@@ -1458,8 +1689,8 @@ static int step_once(mpx_solver *s)
         MPX_LAUNCHED("stepB");
         if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
     } else if (s->path == MPX_PATH_SWEEP) {
-        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
-        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
+        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E, s->time))) return rc;
+        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H, s->time))) return rc;
         s->tick(K_SWEEP);
         if (s->esz == 8)
             mpx::launch_sweep<double>(sweep_args<double>(s), s->np, s->nonlin, s->stream);
@@ -1469,39 +1700,8 @@ static int step_once(mpx_solver *s)
         MPX_LAUNCHED("sweep");
         swap_sweep_buffers(s);
     } else {
-        if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
-        if ((rc = eval_sheets(s, s->hcur, CUR_INC_H))) return rc;
-        if (s->esz == 8) {
-            StepArgs<double> ae = step_args<double>(s, false);
-            const StepArgs<double> *de = dev_args(s, ae, 0);
-            if (!de) return fail(MPX_ERR_HIP, "argument upload failed");
-            s->tick(K_FUSED_E);
-            mpx::launch_fused_E<double>(ae, de, s->np, s->nonlin, s->stream);
-            s->tock();
-            MPX_LAUNCHED("fused E");
-            StepArgs<double> ah = step_args<double>(s, true);
-            const StepArgs<double> *dh = dev_args(s, ah, 1);
-            if (!dh) return fail(MPX_ERR_HIP, "argument upload failed");
-            s->tick(K_FUSED_B);
-            mpx::launch_fused_B<double>(ah, dh, s->stream);
-            s->tock();
-            MPX_LAUNCHED("fused B");
-        } else {
-            StepArgs<float> ae = step_args<float>(s, false);
-            const StepArgs<float> *de = dev_args(s, ae, 0);
-            if (!de) return fail(MPX_ERR_HIP, "argument upload failed");
-            s->tick(K_FUSED_E);
-            mpx::launch_fused_E<float>(ae, de, s->np, s->nonlin, s->stream);
-            s->tock();
-            MPX_LAUNCHED("fused E");
-            StepArgs<float> ah = step_args<float>(s, true);
-            const StepArgs<float> *dh = dev_args(s, ah, 1);
-            if (!dh) return fail(MPX_ERR_HIP, "argument upload failed");
-            s->tick(K_FUSED_B);
-            mpx::launch_fused_B<float>(ah, dh, s->stream);
-            s->tock();
-            MPX_LAUNCHED("fused B");
-        }
+        if ((rc = fused_half(s, false, s->time))) return rc;
+        if ((rc = fused_half(s, true, s->time))) return rc;
     }
     s->time++;
     return record_probes(s);
@@ -1534,8 +1734,14 @@ int mpx_step(mpx_solver *s, int nsteps)
 {
     if (!s || nsteps < 0) return fail(MPX_ERR_INVALID, "invalid solver or step count");
     MPX_NOT_IN_BATCH(s);
+    // probes sample E and B after every step: on the pass path they take the fused halves
+    const bool pass = s->path == MPX_PATH_PASS && s->nprobe == 0;
+    if (!pass) {
+        int rc = settle(s);
+        if (rc) return rc;
+    }
     for (int n = 0; n < nsteps; ++n) {
-        int rc = step_once(s);
+        int rc = pass ? pass_step(s) : step_once(s);
         if (rc) return rc;
     }
     return MPX_OK;
@@ -1548,6 +1754,7 @@ int mpx_step_init(mpx_solver *s)
     MPX_NOT_IN_BATCH(s);
     if (s->esz != 8) return fail(MPX_ERR_UNSUPPORTED, "stepSchemeInit is provided in float64 only");
     int rc;
+    if ((rc = settle(s))) return rc;
     if (!s->plrc) {
         mpx::SimpleArgs a = simple_args(s);
         a.dt = 0.5 * s->d.dt;
@@ -1578,7 +1785,7 @@ int mpx_step_init(mpx_solver *s)
     MPX_LAUNCHED("stepB");
     if ((rc = wrap_group<double>(s, s->B, K_WRAP_B))) return rc;
     // currents' stepSchemeInit: CPML E is empty, incident E/H and CPML H run one stepScheme
-    if ((rc = eval_sheets(s, s->ecur, CUR_INC_E))) return rc;
+    if ((rc = eval_sheets(s, s->ecur, CUR_INC_E, s->time))) return rc;
     for (Current &c : s->hcur) {
         if (c.type == CUR_CPML_H) {
             mpx::launch_ref_cpml(ah, c.face, true, s->stream);
@@ -1603,8 +1810,25 @@ int mpx_set_time(mpx_solver *s, int t)
 {
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
     MPX_NOT_IN_BATCH(s);
+    if (int rs = settle(s)) return rs;
     s->time = t;
     return MPX_OK;
+}
+
+// the device status since the last check (cleared): MPX_ERR_NUMERIC when a bit is set
+static int check_status(mpx_solver *s)
+{
+    unsigned st = 0;
+    MPX_HIP(hipMemcpyAsync(&st, s->d_status, sizeof st, hipMemcpyDeviceToHost, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    if (!st) return MPX_OK;
+    const unsigned zero = 0;
+    MPX_HIP(hipMemcpyAsync(s->d_status, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
+    MPX_HIP(hipStreamSynchronize(s->stream));
+    std::string what;
+    if (st & 1u) what = "a Kerr Newton solve stopped at the iteration cap (" + std::to_string(s->ncap) + ")";
+    if (st & 2u) what += std::string(what.empty() ? "" : "; ") + "a non-finite value in an E update";
+    return fail(MPX_ERR_NUMERIC, what + " on the device before step " + std::to_string(s->time));
 }
 
 int mpx_sync(mpx_solver *s)
@@ -1612,7 +1836,7 @@ int mpx_sync(mpx_solver *s)
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
     MPX_OWNER_SYNC(s);
     MPX_HIP(hipStreamSynchronize(s->stream));
-    return MPX_OK;
+    return check_status(s);
 }
 
 }  // extern "C"
@@ -1752,6 +1976,7 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
 {
     if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
     MPX_OWNER_SYNC(s);
+    if (int rs = settle(s)) return rs;
     const GridDims &g = s->g;
     size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
@@ -1814,6 +2039,7 @@ int mpx_fetch_field(mpx_solver *s, const char *name, double *host, size_t count,
 {
     if (!s || !name || !host || !handle) return fail(MPX_ERR_INVALID, "null argument");
     MPX_OWNER_SYNC(s);
+    if (int rs = settle(s)) return rs;
     const GridDims &g = s->g;
     const size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
@@ -1890,6 +2116,7 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
 {
     if (!s || !name || !host) return fail(MPX_ERR_INVALID, "null argument");
     MPX_OWNER_SYNC(s);
+    if (int rs = settle(s)) return rs;
     const GridDims &g = s->g;
     size_t nx2 = g.n[0] + 2, ny2 = g.n[1] + 2, nz2 = g.n[2] + 2;
     if (count != nx2 * ny2 * nz2) return fail(MPX_ERR_INVALID, "buffer size must be (nx+2)(ny+2)(nz+2)");
@@ -1942,6 +2169,7 @@ int mpx_get_layer(mpx_solver *s, const char *name, int dir, double *host, size_t
 {
     if (!s || !name) return fail(MPX_ERR_INVALID, "null argument");
     MPX_OWNER_SYNC(s);
+    if (int rs = settle(s)) return rs;
     for (const Layer &L : s->layers)
         if (L.dir == dir && L.name == name) {
             if (lo && hi)
@@ -1977,6 +2205,7 @@ int mpx_set_probes(mpx_solver *s, int nprobes, const int *ijk, const int *comp, 
     if (!s || nprobes < 0 || capacity < 0 || (nprobes > 0 && (!ijk || !comp)))
         return fail(MPX_ERR_INVALID, "invalid probe arguments");
     MPX_NOT_IN_BATCH(s);
+    if (int rs = settle(s)) return rs;
     std::vector<size_t> idx(nprobes);
     for (int p = 0; p < nprobes; ++p) {
         for (int d = 0; d < 3; ++d)
@@ -2073,6 +2302,7 @@ int mpx_sampler_record(mpx_sampler *p)
     mpx_solver *s = p->s;
     MPX_NOT_IN_BATCH(s);
     if (p->pending >= p->capacity) return fail(MPX_ERR_INVALID, "sampler full: drain it first");
+    if (int rs = settle(s)) return rs;
     if (p->device) {
         // looked up at every record: the sweep path swaps its two sets of E and B after each step
         void *grid;
@@ -2144,6 +2374,7 @@ int mpx_energy(mpx_solver *s, double *energy)
 {
     if (!s || !energy) return fail(MPX_ERR_INVALID, "null argument");
     MPX_OWNER_SYNC(s);
+    if (int rs = settle(s)) return rs;
     if (s->esz == 8) {
         double *e[3] = {(double *)s->E[0], (double *)s->E[1], (double *)s->E[2]};
         double *b[3] = {(double *)s->B[0], (double *)s->B[1], (double *)s->B[2]};
@@ -2275,6 +2506,9 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
     } else if (!std::strcmp(key, "sweep_seg")) {
         if (value < 1) return fail(MPX_ERR_INVALID, "sweep_seg must be >= 1");
         s->sweep_seg = value < s->g.n[0] ? value : s->g.n[0];
+    } else if (!std::strcmp(key, "newton_cap")) {
+        if (value < 1) return fail(MPX_ERR_INVALID, "newton_cap must be >= 1");
+        s->ncap = value;
     } else if (!std::strcmp(key, "sweep_variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");
         s->sweep_variant = value;
@@ -2645,6 +2879,8 @@ int mpx_batch_sync(mpx_batch *b)
 {
     if (!b) return fail(MPX_ERR_INVALID, "null batch");
     MPX_HIP(hipStreamSynchronize(b->stream));
+    for (size_t r = 0; r < b->runs.size(); ++r)
+        if (int rc = check_status(b->runs[r])) return fail(rc, "run " + std::to_string(r) + ": " + g_err);
     return MPX_OK;
 }
 

@@ -185,6 +185,8 @@ struct StepArgs {
     const T *Jt[3];
     int haveJ;
     int blo[3];                 // first updated B index per dimension (fused path)
+    unsigned *status;           // device status word (report) and Newton iteration cap
+    int ncap;
     int march;                  // fused path: x planes walked by one thread
     int order;                  // fused path: 0 = z tiles fastest in the block order, 1 = x chunks fastest,
                                 // 2 = bands of `band` tile rows, walked plane by plane
@@ -220,6 +222,44 @@ struct SweepArgs {
     int sheets_first;
     int seg;       // x planes per block
     int variant;   // thread-block shape
+    unsigned *status;
+    int ncap;
+};
+
+// Single-pass leapfrog path: one kernel advances B of step n (B^n -> B^(n+1), from E^(n+1))
+// and then E of step n + 1 (E^(n+1) -> E^(n+2)).  "o" buffers are read, "n" buffers written
+// (E, B, their residual words, the CPML H auxiliaries); the pole accumulators and the CPML E
+// auxiliaries, which no halo reads, are updated in place.
+template <typename T>
+struct PassArgs {
+    GridDims g;
+    const T *Eo[3];
+    T *En[3];
+    const T *Bo[3];
+    T *Bn[3];
+    const unsigned *Eolo, *Bolo;  // residual words (float32 + residual storage), else null
+    unsigned *Enlo, *Bnlo;
+    double2 *psi[3][3];
+    const double *ikdE[3];
+    const double *ikdH[3];
+    double dt, eps, chi, denom, numer, idenom, sumChi0, sumXi0;
+    const double *epsf;           // per-cell permittivity or null
+    int plasma;                   // always 0 on this path (read by cell_eps)
+    PoleConsts pc;
+    int nefaces, nhfaces;
+    CpmlFace eface[kMaxFaces];    // E auxiliaries, in place
+    int efbd[3][2];
+    CpmlFace hface[kMaxFaces];    // psi0 / psi1 / rw: the new H auxiliaries
+    const void *hold[kMaxFaces][2];   // the old H auxiliaries
+    const unsigned *hold_rw[kMaxFaces];
+    int hfbd[3][2];
+    int nsheets, nhsheets;
+    Sheet sheet[kMaxSheets];      // E-side sheets (J of step n + 1)
+    Sheet hsheet[kMaxSheets];     // H-side sheets (M of step n)
+    int sheets_first;
+    int band;                     // rows of a band in the banded tile order
+    unsigned *status;             // device status word and Newton iteration cap
+    int ncap;
 };
 
 // Adjoint of one step of the fused float64 PLRC update without Lorentz poles.  The adjoints

@@ -1,0 +1,154 @@
+"""The single-pass leapfrog path (MPX_PATH_PASS) against the fused path and the oracle (GPU).
+
+The pass kernel computes B of step n and E of step n + 1 with the fused kernels' arithmetic
+term for term, so after any number of steps -- with the half step it owes settled whenever
+the state is read -- every field, accumulator and CPML auxiliary equals the fused path's bit
+for bit, in float64, FP32-bulk and FP32-res.  Grids have a partial last z wave and a partial
+last row group, and the banded tile order runs with several band heights (tile_band), so the
+recomputed neighbours of the kernel (x - 1, y - 1, lane 0's z - 1, the periodic wraps) lie on
+tile and band boundaries.
+"""
+import numpy as np
+import pytest
+
+from mpulse_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def pair(make, prec, band=0):
+    da = make(prec, abi.PATH_FUSED)
+    db = make(prec, abi.PATH_PASS)
+    a, b = abi.Solver(da), abi.Solver(db)
+    if band:
+        b.tune("tile_band", band)
+    return a, b
+
+
+def same_state(a, b, names=FIELDS + ("PsiR1x", "PsiI1z")):
+    for n in names:
+        x, y = a.get(n), b.get(n)
+        assert np.array_equal(x, y), (n, float(np.max(np.abs(x - y))))
+
+
+def pulse(n=37, nz=70, poles=((1.1, 4.0, 0.005),), d_cpml=6):
+    def make(prec, path):
+        d, _ = configs.plrc_kerr_pulse(n, d_cpml, 5.0, 6, 2e9, 40, d_src=10, precision=prec, path=path, nz=nz,
+                                       poles=poles)
+        d.ny = n + 6  # a partial last row group
+        d.factories[1].tshift = 8.0
+        return d
+    return make
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK, abi.PREC_FP32_RES])
+@pytest.mark.parametrize("band", [0, 3])
+def test_pass_equals_fused(prec, band):
+    print(f"case: pass vs fused, precision {prec}, tile_band {band}", flush=True)
+    a, b = pair(pulse(), prec, band)
+    with a, b:
+        for n in (1, 3, 17, 9):  # settled between the calls by the reads
+            a.step(n)
+            b.step(n)
+            same_state(a, b)
+        for name, d in (("CPMLPsiBx", abi.NORTH), ("CPMLPsiBy", abi.UP), ("CPMLPsiEz", abi.EAST)):
+            x, y = a.get_layer(name, d)[0], b.get_layer(name, d)[0]
+            assert np.array_equal(x, y), name
+        assert a.energy() == b.energy()
+        assert np.array_equal(a.fetch("Ez"), b.fetch("Ez"))
+
+
+def test_pass_two_poles_lin_core_and_plane_wave():
+    """Two active poles with the linear core, and a tilted plane wave with background fields."""
+    def make2(prec, path):
+        d = pulse(n=30, nz=66, poles=((0.69, 25.0, 0.0), (0.87, 0.2, 0.01)))(prec, path)
+        d.solver = abi.SOLVER_PLRC_LIN
+        return d
+
+    def make_plane(prec, path):
+        d, _ = configs.plrc_kerr_pulse(28, 5, 5.0, 6, 1e9, 30, d_src=9, precision=prec, path=path, nz=40)
+        pw = abi.factory_defaults(abi.FACTORY_PLANE_WAVE)
+        pw.d = 9
+        pw.k[0], pw.k[1], pw.k[2] = 0.3, 0.1, 1.0
+        pw.h[0], pw.h[1] = 0.2, 0.5
+        pw.h_bg[0], pw.e_bg[1] = 0.01, -0.02
+        d.factories[1] = pw
+        return d
+    for make in (make2, make_plane):
+        for prec in (abi.PREC_FP64, abi.PREC_FP32_RES):
+            print(f"case: {make.__name__} precision {prec}", flush=True)
+            a, b = pair(make, prec, band=2)
+            with a, b:
+                a.step(13)
+                b.step(13)
+                same_state(a, b, FIELDS)
+
+
+def test_pass_cpml_1d_and_per_cell_eps():
+    def make(prec, path):
+        d, _ = configs.plrc_kerr_pulse(26, 6, 5.0, 6, 2e9, 30, d_src=10, precision=prec, path=path, nz=50)
+        d.factories[0].kind = abi.FACTORY_CPML_1D
+        d.factories[1].tshift = 8.0
+        return d
+    rng = np.random.default_rng(4)
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_BULK):
+        print(f"case: cpml_border_1d + Eps, precision {prec}", flush=True)
+        a, b = pair(make, prec, band=5)
+        with a, b:
+            eps = 1.0 + rng.random(a.shape)
+            a.set("Eps", eps)
+            b.set("Eps", eps)
+            a.step(11)
+            b.step(11)
+            same_state(a, b, FIELDS)
+
+
+def test_pass_set_field_mid_run():
+    """A write between steps settles the owed half step first, then replaces the grid."""
+    a, b = pair(pulse(n=24, nz=40), abi.PREC_FP64, band=1)
+    rng = np.random.default_rng(8)
+    with a, b:
+        a.step(5)
+        b.step(5)
+        x = rng.standard_normal(a.shape)
+        a.set("By", x)
+        b.set("By", x)
+        a.step(6)
+        b.step(6)
+        same_state(a, b)
+        assert a.time == b.time == 11
+
+
+def test_pass_against_oracle():
+    """64^3, one pole + Kerr, 10-cell CPML, short pulse, 400 steps (float64): every field
+    within 1e-12 of the oracle."""
+    from oracle_lib import Oracle
+    d, meta = configs.config_small(precision=abi.PREC_FP64, path=abi.PATH_PASS)
+    steps = meta["steps"]
+    with abi.Solver(d) as s:
+        s.step(steps)
+        got = {f: s.get(f) for f in FIELDS}
+    o = Oracle(d, omp=True)
+    o.step(steps)
+    for f in FIELDS:
+        r = o.get(f)
+        rel = np.linalg.norm(got[f] - r) / np.linalg.norm(r)
+        print(f, f"{rel:.2e}")
+        assert rel < 1e-12, f
+    o.close()
+
+
+def test_pass_probes_fall_back_to_fused_halves():
+    d, meta = configs.config_small(precision=abi.PREC_FP64, path=abi.PATH_PASS)
+    d2, _ = configs.config_small(precision=abi.PREC_FP64, path=abi.PATH_FUSED)
+    with abi.Solver(d) as a, abi.Solver(d2) as b:
+        a.step(4)
+        b.step(4)
+        for s in (a, b):
+            s.set_probes(meta["probes"], [0] * len(meta["probes"]), 32)
+        a.step(20)
+        b.step(20)
+        assert np.array_equal(a.get_probes(), b.get_probes())
+        same_state(a, b, FIELDS)
