@@ -249,6 +249,18 @@ int  mpx_sampler_destroy(mpx_sampler *p);
 
 /* sum over interior cells of (E^2 + B^2) * dx*dy*dz  (src/fielddiag.cpp:255-286) */
 int  mpx_energy(mpx_solver *s, double *energy);
+/* mpx_energy without allocating and without waiting: the same value, bit for bit, written to
+   the caller's device memory d_energy (one double) by kernels queued on the solver's stream;
+   the caller waits (mpx_sync) before reading it.  No counterpart in the reference: for callers
+   that cannot allocate device memory -- code under HIP graph capture, tight serving loops.  The
+   partial sums live in a file-scope device array of MPX_NOALLOC_PARTIALS doubles (4 MiB) that
+   each GPU holds until the process ends; a grid needing more partials, ceil(nz/64) x
+   ceil(ny/4) x nx > MPX_NOALLOC_PARTIALS (512^3 fits), is refused with MPX_ERR_UNSUPPORTED.
+   Every call on one GPU shares that array: calls that overlap on one GPU (other streams, other
+   solvers, other threads) race.  A pass-path solver owing a half step is refused (settle it
+   with any read first). */
+#define MPX_NOALLOC_PARTIALS (1 << 19)
+int  mpx_energy_to(mpx_solver *s, double *d_energy);
 
 /* Waits for the queued steps.  Returns MPX_ERR_NUMERIC (and clears the condition) when, since
    the last check, a Kerr Newton solve stopped at its iteration cap (tuning key "newton_cap",

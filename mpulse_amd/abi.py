@@ -158,6 +158,7 @@ def load():
         "mpx_set_probes": (C.c_int, [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]),
         "mpx_get_probes": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "mpx_energy": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "mpx_energy_to": (C.c_int, [P, C.c_void_p]),
         "mpx_sync": (C.c_int, [P]),
         "mpx_time_steps": (C.c_int, [P, C.c_int, C.POINTER(Timing)]),
         "mpx_state_bytes": (C.c_int, [P, C.POINTER(C.c_size_t)]),
@@ -415,6 +416,11 @@ class Solver:
         e = C.c_double()
         self._check(self.lib.mpx_energy(self.h, C.byref(e)))
         return e.value
+
+    def energy_to(self, device_ptr):
+        """mpx_energy_to: the energy written to device memory at device_ptr (one float64) on the
+        solver's stream, without allocating or waiting; sync() before reading it."""
+        self._check(self.lib.mpx_energy_to(self.h, C.c_void_p(int(device_ptr))))
 
     def time_steps(self, nsteps):
         t = Timing()

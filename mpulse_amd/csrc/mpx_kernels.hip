@@ -1172,7 +1172,7 @@ __global__ __launch_bounds__(256) void k_energy_partial(GridDims g, const T *e0,
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_sum_partials(const double *part, size_t n, double *out)
+__global__ __launch_bounds__(256) void k_sum_partials(const double *part, size_t n, double *out, double scale)
 {
     __shared__ double red[256];
     double v = 0;
@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *part, size_t
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = red[0];
+    if (threadIdx.x == 0) *out = red[0] * scale;
 }
 
 size_t energy_partials(const GridDims &g)
@@ -1193,17 +1193,17 @@ size_t energy_partials(const GridDims &g)
 
 template <typename T>
 void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s,
-                   const unsigned *elo, const unsigned *blo)
+                   const unsigned *elo, const unsigned *blo, double scale)
 {
     dim3 blk(64, 4);
     dim3 grd((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, g.n[0]);
     hipLaunchKernelGGL(k_energy_partial<T>, grd, blk, 0, s, g, e[0], e[1], e[2], b[0], b[1], b[2], elo, blo, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, energy_partials(g), out);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, part, energy_partials(g), out, scale);
 }
 template void launch_energy<double>(const GridDims &, double *const[3], double *const[3], double *, double *,
-                                    hipStream_t, const unsigned *, const unsigned *);
+                                    hipStream_t, const unsigned *, const unsigned *, double);
 template void launch_energy<float>(const GridDims &, float *const[3], float *const[3], double *, double *,
-                                   hipStream_t, const unsigned *, const unsigned *);
+                                   hipStream_t, const unsigned *, const unsigned *, double);
 
 }  // namespace mpx
 

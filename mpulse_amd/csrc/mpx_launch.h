@@ -96,6 +96,6 @@ void launch_sample_gather(const GridDims &g, const T *src, int dim, const int *c
                          hipStream_t s, const unsigned *lo = nullptr, int q = 0);
 template <typename T>
 void launch_energy(const GridDims &g, T *const e[3], T *const b[3], double *part, double *out, hipStream_t s,
-                   const unsigned *elo = nullptr, const unsigned *blo = nullptr);
+                   const unsigned *elo = nullptr, const unsigned *blo = nullptr, double scale = 1.0);
 
 }  // namespace mpx

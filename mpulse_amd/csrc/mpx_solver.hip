@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <array>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mpulse_hip.h"
@@ -113,6 +114,7 @@ struct mpx_solver {
     GridDims g{};
     size_t cells = 0;  // padded element count of one grid
     hipStream_t stream = nullptr;
+    int device = 0;  // the GPU current when the solver was created
     int periodic[3] = {1, 1, 1};
     double delta[3] = {0, 0, 0};
 
@@ -668,6 +670,7 @@ static int build(mpx_solver *s, const mpx_desc *desc)
     if (s->plrc && s->path != MPX_PATH_REFERENCE && s->cells >= (size_t)0x7fffffff)
         return fail(MPX_ERR_UNSUPPORTED, "grid too large for the fused kernels' 32-bit indexing");
 
+    if (hipGetDevice(&s->device) != hipSuccess) return fail(MPX_ERR_HIP, "hipGetDevice failed");
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(MPX_ERR_HIP, "hipStreamCreate failed");
 
@@ -2389,6 +2392,64 @@ int mpx_energy(mpx_solver *s, double *energy)
     MPX_HIP(hipMemcpyAsync(&v, s->d_energy, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     *energy = v * (s->d.dx * s->d.dy * s->d.dz);
+    return MPX_OK;
+}
+
+}  // extern "C"
+
+// Scratch of mpx_energy_to: the partial sums of the energy reduction.  Each GPU the library
+// runs on holds its own copy of the array until the process ends.  In host code the array's
+// name is no device address: the address on each GPU comes from hipGetSymbolAddress, taken
+// once per device (with that device current) at the entry point's first call there.
+static __device__ __align__(256) double scratch[MPX_NOALLOC_PARTIALS];
+static std::mutex g_scratch_mu;
+static std::vector<double *> g_scratch;  // per device id; null until looked up
+
+static int scratch_on(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch.size() <= device) g_scratch.resize(device + 1, nullptr);
+    if (!g_scratch[device]) {
+        int cur = 0;
+        MPX_HIP(hipGetDevice(&cur));
+        if (cur != device) MPX_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) MPX_HIP(hipSetDevice(cur));
+        if (e != hipSuccess || !p) return fail(MPX_ERR_HIP, std::string("hipGetSymbolAddress of the scratch failed: ") +
+                                                                 hipGetErrorString(e));
+        g_scratch[device] = (double *)p;
+    }
+    *out = g_scratch[device];
+    return MPX_OK;
+}
+
+extern "C" {
+
+int mpx_energy_to(mpx_solver *s, double *d_energy)
+{
+    if (!s || !d_energy) return fail(MPX_ERR_INVALID, "null argument");
+    MPX_NOT_IN_BATCH(s);
+    if (s->pending)
+        return fail(MPX_ERR_INVALID, "the pass path owes the B half of the last step: read the state or call "
+                                     "mpx_energy once before mpx_energy_to");
+    const size_t np = mpx::energy_partials(s->g);
+    if (np > (size_t)MPX_NOALLOC_PARTIALS)
+        return fail(MPX_ERR_UNSUPPORTED, "the grid needs " + std::to_string(np) + " partial sums (ceil(nz/64) ceil(ny/4) nx); "
+                                         "the scratch holds " + std::to_string(MPX_NOALLOC_PARTIALS) + ": use mpx_energy");
+    double *part = nullptr;
+    if (int rc = scratch_on(s->device, &part)) return rc;
+    const double dv = s->d.dx * s->d.dy * s->d.dz;
+    if (s->esz == 8) {
+        double *e[3] = {(double *)s->E[0], (double *)s->E[1], (double *)s->E[2]};
+        double *b[3] = {(double *)s->B[0], (double *)s->B[1], (double *)s->B[2]};
+        mpx::launch_energy<double>(s->g, e, b, part, d_energy, s->stream, nullptr, nullptr, dv);
+    } else {
+        float *e[3] = {(float *)s->E[0], (float *)s->E[1], (float *)s->E[2]};
+        float *b[3] = {(float *)s->B[0], (float *)s->B[1], (float *)s->B[2]};
+        mpx::launch_energy<float>(s->g, e, b, part, d_energy, s->stream, s->Elo, s->Blo, dv);
+    }
+    MPX_LAUNCHED("energy");
     return MPX_OK;
 }
 
