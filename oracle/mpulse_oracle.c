@@ -15,6 +15,18 @@
  */
 #include "mpulse_oracle.h"
 
+/* ORACLE_MUTANT (test builds only, `make mutants`): a planted mistake, so that the tests can
+   show that each of their physics checks fails when the part of the path it pins is wrong.
+   0 = the oracle as it is.  1: the H sheet of a low face at `distance` (no decrement);
+   2: no half-cell offset of the H-side CPML profile; 3: the outermost H-side CPML cell clamped
+   to x = 1; 4: the CPML b/c tables one cell toward the interior; 5: sign of the E-side CPML
+   psi0 update flipped; 6: the low-side kappaE line one cell toward the interior; 7: fdtd-nl
+   without the chi E^2 term of D; 8: fdtd-nl with the curl's sign flipped; 9: fdtd-disp with
+   b = dt^2 Om^2 eps instead of eps2; 10: fdtd-disp without the previous-P term of E. */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 #include <complex.h>
 #include <math.h>
 #include <stdio.h>
@@ -1039,11 +1051,16 @@ static int cpml_make_coeff(or_sim *s, current_t *c, int isH)
         break;
     }
     if (!line_alloc(&c->b, lowk, highk, 0.0) || !line_alloc(&c->c, lowk, highk, 0.0)) return 0;
-    double offset = isH ? 0.5 : 0.0;
+    double offset = (isH && ORACLE_MUTANT != 2) ? 0.5 : 0.0;
     for (int k = 0; k < c->thickness; ++k) {
         double x = 1 - ((double)k - offset) / (double)c->thickness;
+        if (ORACLE_MUTANT == 3 && x > 1) x = 1;
         double x3 = x * x * x;
         int pos = c->reverse ? (lowk + k) : (highk - k);
+        if (ORACLE_MUTANT == 4) {
+            pos += c->reverse ? 1 : -1;
+            if (pos < lowk || pos > highk) continue;
+        }
         double sigma = x3 * c->sigmaMax;
         double kappa = 1 + (c->kappaMax - 1) * x3;
         double a = c->aMax * (1 - x);
@@ -1124,7 +1141,7 @@ static void cpml_step_E(or_sim *s, current_t *c)
                 --m[dim];
                 double *p0 = &AT(Psi0, i0, i1, i2);
                 double *p1 = &AT(Psi1, i0, i1, i2);
-                *p0 = LN(&c->b, j) * (*p0) -
+                *p0 = LN(&c->b, j) * (*p0) - (ORACLE_MUTANT == 5 ? -1.0 : 1.0) *
                       LN(&c->c, j) * (AT(B1, i0, i1, i2) - AT(B1, m[0], m[1], m[2])) / dx;
                 *p1 = LN(&c->b, j) * (*p1) +
                       LN(&c->c, j) * (AT(B0, i0, i1, i2) - AT(B0, m[0], m[1], m[2])) / dx;
@@ -1171,7 +1188,7 @@ static void cpml_init_kappa(or_sim *s, int thickness, double kappaMax)
             for (int i = 0; i <= thickness; ++i) {
                 double x = 1 - (double)i / (double)thickness;
                 double x3 = x * x * x;
-                LN(KE, s->low[dim] + i) = 1 + (kappaMax - 1) * x3;
+                LN(KE, s->low[dim] + i + (ORACLE_MUTANT == 6)) = 1 + (kappaMax - 1) * x3;
             }
             for (int i = 0; i < thickness; ++i) {
                 double x = 1 - ((double)i + 0.5) / (double)thickness;
@@ -1268,7 +1285,7 @@ static void inc_dims(current_t *c, double dx, double dy, double dz)
    IncidentSourceHCurrent::initStorage src/incsource.t:87-103 */
 static void inc_init_storage(or_sim *s, current_t *c, int isH)
 {
-    if (isH && !c->reverse) c->distance = c->distance - 1;
+    if (isH && !c->reverse && ORACLE_MUTANT != 1) c->distance = c->distance - 1;
     const char *nm[3];
     if (!isH) {
         nm[0] = "IncidentEJx";
@@ -1802,12 +1819,14 @@ static void nl_stepD(or_sim *s, double dt)
             for (int k = s->low[2] + 1; k <= s->high[2]; ++k) {
                 double ex = AT(Ex, i, j, k), ey = AT(Ey, i, j, k), ez = AT(Ez, i, j, k);
                 double E2 = ex * ex + ey * ey + ez * ez;
-                double Dx = (eps + chi * E2) * ex + dt * ((AT(Bz, i, j, k) - AT(Bz, i, j - 1, k)) / dy -
-                                                          (AT(By, i, j, k) - AT(By, i, j, k - 1)) / dz);
-                double Dy = (eps + chi * E2) * ey + dt * ((AT(Bx, i, j, k) - AT(Bx, i, j, k - 1)) / dz -
-                                                          (AT(Bz, i, j, k) - AT(Bz, i - 1, j, k)) / dx);
-                double Dz = (eps + chi * E2) * ez + dt * ((AT(By, i, j, k) - AT(By, i - 1, j, k)) / dx -
-                                                          (AT(Bx, i, j, k) - AT(Bx, i, j - 1, k)) / dy);
+                const double f = eps + (ORACLE_MUTANT == 7 ? 0.0 : chi * E2);
+                const double sg = ORACLE_MUTANT == 8 ? -1.0 : 1.0;
+                double Dx = f * ex + sg * dt * ((AT(Bz, i, j, k) - AT(Bz, i, j - 1, k)) / dy -
+                                                (AT(By, i, j, k) - AT(By, i, j, k - 1)) / dz);
+                double Dy = f * ey + sg * dt * ((AT(Bx, i, j, k) - AT(Bx, i, j, k - 1)) / dz -
+                                                (AT(Bz, i, j, k) - AT(Bz, i - 1, j, k)) / dx);
+                double Dz = f * ez + sg * dt * ((AT(By, i, j, k) - AT(By, i - 1, j, k)) / dx -
+                                                (AT(Bx, i, j, k) - AT(Bx, i, j - 1, k)) / dy);
                 double E = sqrt(E2);
                 double D = sqrt(Dx * Dx + Dy * Dy + Dz * Dz);
                 double Eold;
@@ -1842,7 +1861,7 @@ static void disp_stepD(or_sim *s, double dt)
                     double *pxp = &AT(&s->Pp[n][0], i, j, k), *pyp = &AT(&s->Pp[n][1], i, j, k),
                            *pzp = &AT(&s->Pp[n][2], i, j, k);
                     double a = dt2 * s->d.d_om2[n];
-                    double b = a * s->d.d_eps2[n];
+                    double b = a * (ORACLE_MUTANT == 9 ? s->d.eps : s->d.d_eps2[n]);
                     double pxn = (2 - a) * (*px) - (*pxp) + b * ex;
                     double pyn = (2 - a) * (*py) - (*pyp) + b * ey;
                     double pzn = (2 - a) * (*pz) - (*pzp) + b * ez;
@@ -1859,6 +1878,7 @@ static void disp_stepD(or_sim *s, double dt)
                     Pyp += *pyp;
                     Pzp += *pzp;
                 }
+                if (ORACLE_MUTANT == 10) Pxp = Pyp = Pzp = 0;
                 double Dx = ex + Pxp - Px + dt * ((AT(Bz, i, j, k) - AT(Bz, i, j - 1, k)) / dy -
                                                   (AT(By, i, j, k) - AT(By, i, j, k - 1)) / dz);
                 double Dy = ey + Pyp - Py + dt * ((AT(Bx, i, j, k) - AT(Bx, i, j, k - 1)) / dz -

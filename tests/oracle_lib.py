@@ -19,8 +19,13 @@ REF_PULSE = os.path.join(ROOT, "oracle", "_ref", "refpulse")
 _libs = {}
 
 
-def load(omp=False):
-    path = LIB_OMP if omp else LIB
+def mutant_path(m):
+    """A build of the oracle with planted mistake `m` (`make -C oracle mutants`)."""
+    return os.path.join(ROOT, "oracle", "_build", f"liboracle_mut{m}.so")
+
+
+def load(omp=False, path=None):
+    path = path or (LIB_OMP if omp else LIB)
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):
@@ -70,8 +75,8 @@ def _dp(a):
 class Oracle:
     """Same surface as mpulse_amd.abi.Solver, computed on the CPU by the oracle."""
 
-    def __init__(self, desc, omp=False):
-        self.lib = load(omp)
+    def __init__(self, desc, omp=False, lib=None):
+        self.lib = load(omp, lib)
         self.desc = desc
         self.shape = (desc.nx + 2, desc.ny + 2, desc.nz + 2)
         err = C.create_string_buffer(256)

@@ -250,11 +250,14 @@ def test_zero_dimensional_kerr_lorentz_converges():
     assert 3.7 < errs[0] / errs[1] < 4.3, errs
 
 
-@pytest.mark.parametrize("solver", [abi.SOLVER_PLRC_LIN, abi.SOLVER_PLRC_NONLIN, abi.SOLVER_PLAIN])
+@pytest.mark.parametrize("solver", [abi.SOLVER_PLRC_LIN, abi.SOLVER_PLRC_NONLIN, abi.SOLVER_PLAIN, abi.SOLVER_NL,
+                                    abi.SOLVER_DISP])
 def test_vacuum_yee_energy_invariant(solver):
     """For the leapfrog Yee scheme in a periodic vacuum box, sum E^n.E^n + sum B^{n-1}.B^n
     (B before and after the step) is exactly conserved; it needs the E and B curls to be
-    negative adjoints of each other, so any wrong index, sign or spacing breaks it."""
+    negative adjoints of each other, so any wrong index, sign or spacing breaks it.  fdtd-nl
+    runs with chi = 0 (its Newton solve then returns D / eps) and fdtd-disp with no
+    oscillator strength (P stays 0)."""
     n = (6, 7, 8)
     d = small_desc(n, solver, dx=0.3)
     d.dy = d.dz = 0.3  # the plain solver's Bz update uses dx for dy (src/fdtd_plain.cpp:115-116)
@@ -307,6 +310,85 @@ def test_discrete_plane_wave_eigenmode(axis):
     err = np.abs(o.get(ecomp) - mode(0.0, t))[1:-1, 1:-1, 1:-1].max()
     errb = np.abs(o.get(bcomp) - mode(0.5, t + 0.5 * d.dt))[1:-1, 1:-1, 1:-1].max()
     assert err < 1e-12 and errb < 1e-12, (err, errb)
+
+
+# ---------------------------------------------------------------------------------------
+# fdtd-nl (src/fdtd_nl.cpp:38-100) and fdtd-disp (src/fdtd_disp.cpp:94-206)
+# ---------------------------------------------------------------------------------------
+
+def cardano(eps, chi, D):
+    """the real root of chi E^3 + eps E = D (eps, chi > 0)"""
+    p, q = eps / chi, -D / chi
+    disc = math.sqrt(q * q / 4 + p ** 3 / 27)
+    return float(np.cbrt(-q / 2 + disc) + np.cbrt(-q / 2 - disc))
+
+
+def test_nl_vector_cubic_with_uniform_curl():
+    """fdtd-nl with a uniform E and a B growing linearly in z: the curl is the same in every
+    cell, so one step must give D = (eps + chi |E|^2) E + dt curl B and then the E along D
+    whose magnitude is the Cardano root of chi |E|^3 + eps |E| = |D|.  Dropping chi |E|^2
+    from D or flipping the curl's sign moves the result by > 1e-2."""
+    eps, chi, h = 1.5, 0.2, 0.3
+    d = small_desc((4, 4, 6), abi.SOLVER_NL, dx=h, dt=0.1)
+    d.dy = d.dz = h
+    d.eps, d.chi = eps, chi
+    o = ol.Oracle(d)
+    E0 = np.array([0.9, -0.6, 0.0])
+    b1, b2 = 0.7, -0.4  # By = b1 k, Bx = b2 k
+    kk = np.broadcast_to(np.arange(d.nz + 2, dtype=float), o.shape).copy()
+    for c, v in zip(("Ex", "Ey", "Ez"), E0):
+        o.set(c, np.full(o.shape, v))
+    o.set("By", b1 * kk)
+    o.set("Bx", b2 * kk)
+    o.step(1)
+    f = eps + chi * float(E0 @ E0)
+    D = np.array([f * E0[0] - d.dt * b1 / h, f * E0[1] + d.dt * b2 / h, 0.0])
+    Dn = float(np.linalg.norm(D))
+    want = cardano(eps, chi, Dn) * D / Dn
+    inner = (slice(1, -1),) * 3
+    for c, w in zip(("Ex", "Ey", "Ez"), want):
+        got = o.get(c)[inner]
+        assert np.all(np.abs(got - w) <= 1e-12 * max(1.0, abs(w))), (c, got.ravel()[0], w)
+
+
+def disp_exact(t, D0, eps2, om2):
+    """Undamped Lorentz oscillators driven by E = D0 - sum P: P_n'' = om2_n (eps2_n E - P_n),
+    P(0) = P'(0) = 0, solved by scipy (DOP853)."""
+    from scipy.integrate import solve_ivp
+    m = len(eps2)
+
+    def rhs(_, y):
+        E = D0 - sum(y[:m])
+        return np.concatenate([y[m:], [om2[n] * (eps2[n] * E - y[n]) for n in range(m)]])
+
+    sol = solve_ivp(rhs, (0, t[-1]), np.zeros(2 * m), method="DOP853", rtol=1e-12, atol=1e-14, dense_output=True)
+    return D0 - sol.sol(t)[:m].sum(axis=0)
+
+
+def test_disp_zero_dimensional_oscillators_converge():
+    """fdtd-disp in a uniform field (no curl), two oscillators: E^n against the analytic
+    solution at n dt + dt/2 (the zero history P^-1 = P^0 = 0 centres the start at -dt/2).
+    The error falls 4x as dt halves and is small.  Using the background eps for a pole's
+    strength, or dropping the previous polarisation from E, fails both bounds."""
+    eps2, om2, D0 = (0.8, 0.3), (3.0, 0.5), 0.7
+    T = 8.0
+    errs = []
+    for dt in (0.02, 0.01):
+        d = small_desc((2, 2, 2), abi.SOLVER_DISP, dt=dt)
+        d.eps = 1.7  # unused by fdtd-disp's E update
+        for n in range(2):
+            d.d_eps2[n], d.d_om2[n] = eps2[n], om2[n]
+        o = ol.Oracle(d)
+        o.set("Ex", np.full(o.shape, D0))
+        nsteps = int(round(T / dt))
+        es = []
+        for _ in range(nsteps):
+            o.step(1)
+            es.append(o.get("Ex")[1, 1, 1])
+        t = (np.arange(nsteps) + 1.5) * dt
+        errs.append(float(np.max(np.abs(np.array(es) - disp_exact(t, D0, eps2, om2)))))
+    assert errs[1] < 2e-4 * D0, errs
+    assert 3.7 < errs[0] / errs[1] < 4.3, errs
 
 
 def test_periodic_wrap_matches_brute_force():
@@ -372,6 +454,140 @@ def test_cpml_absorbs():
     b = longer.get("Ex")[2, 2, 11:80]
     refl = np.abs(a - b).max() / np.abs(longer.get("Ex")).max()
     assert refl < 0.06, refl
+
+
+CPML_T = 10
+
+
+def cpml_box(axis, length, kappa_max=3.0, a_max=0.25):
+    """vacuum, 3-D CPML (cpml_border) of CPML_T cells on every face; `length` cells along
+    `axis`, 2 CPML_T + 2 across"""
+    n = [2 * CPML_T + 2] * 3
+    n[axis] = length
+    d = small_desc(tuple(n), abi.SOLVER_PLRC_LIN)
+    cp = abi.factory_defaults(abi.FACTORY_CPML)
+    cp.d = CPML_T
+    cp.sigma_max = configs.sigma_max(CPML_T)
+    cp.kappa_max, cp.a_max = kappa_max, a_max
+    abi.add_factory(d, cp)
+    return d
+
+
+PACKET_COMP = {0: ("Ey", "Bz"), 1: ("Ez", "Bx"), 2: ("Ex", "By")}
+
+
+def set_packet(o, axis, length, x0, sgn, lam=10.0, w=8.0):
+    """a one-way Gaussian packet (carrier 10 cells, 1/e half width 8 cells) uniform across
+    `axis`, centred on cell x0 and moving toward +axis (sgn = 1) or -axis: E at t = 0, B a
+    half cell and a half step on"""
+    h, dt = configs.DX, configs.DT
+    idx = np.arange(length + 2) - 1.0
+    k0 = 2 * math.pi / (lam * h)
+
+    def f(shift, t):
+        s = (idx + shift - x0) * h - sgn * t
+        v = np.exp(-(s / (w * h)) ** 2) * np.cos(k0 * s)
+        shape = [1, 1, 1]
+        shape[axis] = length + 2
+        return np.broadcast_to(v.reshape(shape), o.shape).copy()
+
+    e, b = PACKET_COMP[axis]
+    o.set(e, f(0.0, 0.0))
+    o.set(b, sgn * f(0.5, 0.5 * dt))
+
+
+def cpml_reflection(axis, sgn, short=60, extra=110):
+    """Largest difference, over the interior and the 140 steps in which the echo crosses it,
+    between a packet hitting the face 20 cells ahead and the same packet in a box `extra`
+    cells longer on that side (whose echo comes later), relative to the unit packet."""
+    runs = []
+    for length in (short, short + extra):
+        off = length - short if sgn < 0 else 0
+        o = ol.Oracle(cpml_box(axis, length), omp=True)
+        set_packet(o, axis, length, off + (short - CPML_T - 20 if sgn > 0 else CPML_T + 20), sgn)
+        runs.append((o, off))
+    e = PACKET_COMP[axis][0]
+
+    def line(o, off):
+        a = o.get(e)
+        m = [a.shape[0] // 2, a.shape[1] // 2, a.shape[2] // 2]
+        m[axis] = slice(off + 1 + CPML_T, off + short - CPML_T + 1)
+        return a[tuple(m)]
+
+    for o, _ in runs:
+        o.step(140)
+    worst = 0.0
+    for _ in range(14):
+        for o, _ in runs:
+            o.step(10)
+        worst = max(worst, float(np.abs(line(*runs[0]) - line(*runs[1])).max()))
+    for o, _ in runs:
+        o.close()
+    return worst
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_cpml_3d_reflects_little_on_every_face(axis):
+    """A one-way packet at normal incidence on each of the six faces of the 3-D CPML
+    (10 cells, R0 = 1e-6 cubic grading, kappa_max 3, a_max 0.25): the echo is 2.4e-4 (high
+    faces) and 2.5e-4 (low faces) of the packet.  Planted mistakes measured on the same
+    runs: the low-side kappa_E line one cell inward 2.4e-3 on the low faces; the b/c tables
+    one cell inward 4.2e-3; no half-cell offset of the H-side grading 1.4e-2; the E
+    auxiliary's sign flipped 0.14."""
+    for sgn in (1, -1):
+        r = cpml_reflection(axis, sgn)
+        assert r < 4e-4, (axis, sgn, r)
+
+
+def test_cpml_coefficient_profiles():
+    """Every entry of the b/c tables of the six faces is the recursive-convolution pair of
+    Roden and Gedney (2000) for the cubic grading, b = exp(-(sigma/kappa + a) dt / eps0),
+    c = sigma (b - 1) / (kappa (sigma + kappa a)), with sigma = sigma_max x^3,
+    kappa = 1 + (kappa_max - 1) x^3, a = a_max (1 - x), at the table node's depth x: the
+    E-side entry j cells in from the outer edge at x = 1 - j / T, the H-side entry half a
+    cell further out, x = 1 - (j - 1/2) / T -- the same layout mirrored on the low faces
+    (src/cpml_border.cpp:287-339).  The outermost H entry sits at x = 1 + 1/(2T): the grading
+    is extrapolated there, not clamped.  (The clamp is invisible to the reflection test --
+    the echo moves from 2.36e-4 to 2.29e-4 -- so this table is its only pin.)"""
+    n = (2 * CPML_T + 4, 2 * CPML_T + 6, 2 * CPML_T + 8)
+    d = small_desc(n, abi.SOLVER_PLRC_LIN)
+    cp = abi.factory_defaults(abi.FACTORY_CPML)
+    cp.d = CPML_T
+    cp.sigma_max = configs.sigma_max(CPML_T)
+    cp.kappa_max, cp.a_max = 7.0, 0.4
+    abi.add_factory(d, cp)
+    o = ol.Oracle(d)
+    T, dt = CPML_T, d.dt
+    for direction in (abi.NORTH, abi.SOUTH, abi.WEST, abi.EAST, abi.UP, abi.DOWN):
+        high = direction in (abi.NORTH, abi.EAST, abi.UP)
+        for is_h in (0, 1):
+            b, c, lo, hi = o.cpml_coeff(is_h, direction)
+            assert hi - lo + 1 == T, (direction, lo, hi)
+            for q, pos in enumerate(range(lo, hi + 1)):
+                j = hi - pos if high else pos - lo
+                x = 1 - (j - 0.5 * is_h) / T
+                sig = cp.sigma_max * x ** 3
+                kap = 1 + (cp.kappa_max - 1) * x ** 3
+                a = cp.a_max * (1 - x)
+                bw = math.exp(-(sig / kap + a) * dt)
+                cw = sig * (bw - 1) / (kap * (sig + kap * a))
+                assert abs(b[q] - bw) <= 1e-14 and abs(c[q] - cw) <= 1e-14 * abs(cw), (direction, is_h, pos)
+
+
+def test_incident_sheet_planes():
+    """The source sheets of the down face (every source factory injects there): the E sheet
+    (J from the sampled H) on plane d and the H sheet (M from the sampled E) on plane d - 1,
+    where IncidentSourceHCurrent::initStorage lowers the distance on the low faces
+    (src/incsource.t:87-103).  The plane and short-pulse sources sample at their own sheet's
+    index, so a sheet on another plane only moves their pulse by a cell; the focused source
+    samples at fixed heights, so there the physics test below pins the plane too (an H sheet
+    on plane d leaks 0.091 of the peak below the sheets, against 0.028 with the sheet on
+    plane d - 1; the bound is 0.04)."""
+    d = vacuum_column(60, d_src=17)
+    o = ol.Oracle(d)
+    for name, plane in (("IncidentEJx", 17), ("IncidentEJy", 17), ("IncidentHJx", 16), ("IncidentHJy", 16)):
+        _, lo, hi = o.get_layer(name, abi.DOWN)
+        assert lo[2] == hi[2] == plane, (name, lo, hi)
 
 
 # ---------------------------------------------------------------------------------------
