@@ -131,13 +131,23 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     10-cell CPML, 500 steps each) stepped together by the batched persistent kernels replayed
     from captured graphs with autotuned tiles, in float64 and in FP32-bulk, against the same
     runs launched one after another on the float64 plain (reference-structured) path and on
-    the fused path of the same precision.  Times cover the steps only (set-up excluded), wall
-    clock around each set."""
+    the path of the batch's tuned structure (fused or sweep) in the same precision.  Every
+    batched run's E and B after the last step are compared bit for bit with that run on its
+    own (SHA-256 of the six fields).  Times cover the steps only (set-up excluded), wall clock
+    around each set."""
+    import hashlib
+
     import torch
 
     from mpulse_amd import abi, configs
 
-    def sequential(descs, path):
+    def digest(view):
+        h = hashlib.sha256()
+        for name in ("Ex", "Ey", "Ez", "Bx", "By", "Bz"):
+            h.update(view.get(name).tobytes())
+        return h.hexdigest()
+
+    def sequential(descs, path, digests=None):
         total = 0.0
         for d in descs:
             d.path = path
@@ -147,6 +157,8 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
             s.step(steps)
             s.sync()
             total += time.perf_counter() - t0
+            if digests is not None:
+                digests.append(digest(s))
             s.close()
         torch.cuda.empty_cache()
         return total
@@ -159,9 +171,10 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
         b.step(steps)
         b.sync()
         t = time.perf_counter() - t0
+        digests = [digest(b.run(r)) for r in range(len(descs))]
         b.close()
         torch.cuda.empty_cache()
-        return t, tune
+        return t, tune, digests
 
     descs64 = [d for d, _ in configs.config_sweep(nruns=nruns, steps=steps, precision=abi.PREC_FP64)]
     cells = sum(d.nx * d.ny * d.nz for d in descs64)
@@ -171,15 +184,26 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     print(f"# sequential_plain (float64): {t_plain:.3f} s", file=sys.stderr, flush=True)
     for name, prec in (("fp64", abi.PREC_FP64), ("fp32_bulk", abi.PREC_FP32_BULK)):
         descs = [d for d, _ in configs.config_sweep(nruns=nruns, steps=steps, precision=prec)]
-        t_b, tune = batched(descs)
-        t_f = sequential(descs, abi.PATH_FUSED)
+        t_b, tune, got = batched(descs)
+        same_path = abi.PATH_FUSED if tune["structure"] == "fused" else abi.PATH_SWEEP
+        want = []
+        t_f = sequential(descs, same_path, want)
+        identical = [g == w for g, w in zip(got, want)]
+        t_fused = t_f if same_path == abi.PATH_FUSED else sequential(descs, abi.PATH_FUSED)
         out[name] = {"batched": {"s": t_b, "cell_updates_per_s": cells * steps / t_b, "tuning": tune},
-                     "sequential_fused": {"s": t_f, "cell_updates_per_s": cells * steps / t_f},
+                     "sequential_same_path": {"path": tune["structure"], "s": t_f,
+                                              "cell_updates_per_s": cells * steps / t_f},
                      "speedup_vs_sequential_plain": t_plain / t_b,
-                     "speedup_vs_sequential_fused": t_f / t_b}
+                     "speedup_vs_sequential_same_path": t_f / t_b,
+                     "sequential_fused": {"s": t_fused, "cell_updates_per_s": cells * steps / t_fused},
+                     "speedup_vs_sequential_fused": t_fused / t_b,
+                     "runs_bit_identical_to_sequential": int(sum(identical)),
+                     "all_runs_bit_identical": all(identical)}
         print(f"# batch {name}: {t_b:.3f} s for {nruns} runs x {steps} steps, tuning {tune}; "
-              f"{t_plain / t_b:.2f}x sequential float64 plain, {t_f / t_b:.2f}x sequential {name} fused "
-              f"({t_f:.3f} s)", file=sys.stderr, flush=True)
+              f"{t_plain / t_b:.2f}x sequential float64 plain, {t_f / t_b:.2f}x sequential {name} "
+              f"{tune['structure']} ({t_f:.3f} s), {t_fused / t_b:.2f}x sequential {name} fused ({t_fused:.3f} s); "
+              f"{sum(identical)}/{nruns} runs bit-identical to their "
+              f"sequential runs", file=sys.stderr, flush=True)
     return out
 
 
@@ -301,10 +325,12 @@ def adjoint_gradient(fd=True):
         hc = 1e-4 * d.chi
         dp.chi, dm.chi = d.chi + hc, d.chi - hc
         fdc = (run(dp, full, False)[0][0] - run(dm, full, False)[0][0]) / (2 * hc)
-        out["fd_check"] = {"h": h, "cells": errs, "max_rel_eps": max(e["rel"] for e in errs),
-                           "max_abs_grad": gmax, "chi_fd": fdc, "chi_rel": abs(gchi - fdc) / abs(fdc)}
-        print(f"# adjoint vs finite differences: max rel (eps) {out['fd_check']['max_rel_eps']:.2e}, "
-              f"chi rel {out['fd_check']['chi_rel']:.2e}", file=sys.stderr, flush=True)
+        fc = {"h": h, "cells": errs, "max_rel_eps": max(e["rel"] for e in errs), "max_abs_grad": gmax,
+              "chi_fd": fdc, "chi_rel": abs(gchi - fdc) / abs(fdc), "bar": 1e-4}
+        fc["pass"] = fc["max_rel_eps"] <= 1e-4 and fc["chi_rel"] <= 1e-4
+        out["fd_check"] = fc
+        print(f"# adjoint vs finite differences: max rel (eps) {fc['max_rel_eps']:.2e}, chi rel "
+              f"{fc['chi_rel']:.2e}: {'pass' if fc['pass'] else 'FAIL'} (bar 1e-4)", file=sys.stderr, flush=True)
     return out
 
 

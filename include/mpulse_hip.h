@@ -330,10 +330,11 @@ int  mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step);
    current state; the adjoint of every step is then run backwards through the same discrete
    update (fused float64 kernels), recomputing the forward E of each segment of
    checkpoint_every steps from a stored checkpoint.  Gradients: dJ/dEps(x) for every interior
-   cell (full-storage layout, zero in the ghosts) and dJ/dchi.  The solver ends at the final
-   forward state.  Scope: FDTD_PLRC solvers without active Lorentz poles, float64, fused
-   path, periodic boundary (CPML and current sheets allowed).  With dJ_deps and dJ_dchi both
-   NULL only the forward objective is computed. */
+   cell (full-storage layout, zero in the ghosts) and dJ/dchi (0 for the linear core).  The
+   solver ends at the final forward state.  Scope: FDTD_PLRC solvers (Kerr or linear core,
+   with or without Lorentz poles), float64, fused path, periodic boundary (CPML and current
+   sheets allowed; no plasma, no gauss_inject).  With dJ_deps and dJ_dchi both NULL only the
+   forward objective is computed. */
 typedef struct mpx_objective {
     int    plane_k;   /* z index of the observation plane, 1..nz */
     double weight;

@@ -1,19 +1,26 @@
-// Discrete adjoint of one time step of the fused float64 PLRC update without Lorentz poles
-// (the forward kernels are fused_E_tile / fused_B_tile in mpx_kernels.hip; the reference
-// update is FDTD_PLRCNonlinCore::plrcStepD, src/fdtd_plrc.cpp:304-427, plrcStepB,
-// src/fdtd_plrc.cpp:429-459, and the CPML auxiliaries of src/cpml_border.cpp:398-425,
-// 499-527).  Periodic boundary in every dimension: E and B cells are 1..N, neighbours wrap.
+// Discrete adjoint of one time step of the fused float64 PLRC update (the forward kernels are
+// fused_E_tile / fused_B_tile in mpx_kernels.hip; the reference update is
+// FDTD_PLRCNonlinCore::plrcStepD, src/fdtd_plrc.cpp:304-427, with the pole recursion of
+// src/fdtd_plrc.cpp:321-353, plrcStepB, src/fdtd_plrc.cpp:429-459, and the CPML auxiliaries
+// of src/cpml_border.cpp:398-425, 499-527).  Periodic boundary in every dimension: E and B
+// cells are 1..N, neighbours wrap.
 //
 // Forward step n (per cell; D_F[d][q] = difference along d of component q of field F):
 //   E kernel: psiE' = b psiE -/+ c D_B / delta (face layers), J = sheets + psiE',
-//             R = curl_kappa(D_B) + J, C = ((eps + chi |E|^2) E + dt R) / eps,
-//             E' = (e / s) C with s = |C| and (chi / eps) e^3 + e = s;
+//             R = curl_kappa(D_B) + J, poles p: P_p = Dd_p E + Psi_p (complex, Dd = dchi0 -
+//             dxi0), Psi_p' = dxi0_p E + Crec_p P_p,
+//             C = ((numer + chi |E|^2) E + dt R + sum_p Re P_p) / denom with
+//             denom = eps + sumChi0 - sumXi0 and numer = eps - sumXi0,
+//             E' = (e / s) C with s = |C| and (chi / denom) e^3 + e = s (the linear solver:
+//             chi = 0, E' = C);
 //   B kernel: psiH' = b psiH +/- c D_E' / delta, B' = B + dt (curl_kappa(D_E') + M),
 //             M = sheets + psiH'.
 // The reverse of step n runs three kernels: k_adj_bside (adjoint of the B kernel: the
 // adjoints of its E' differences and of psiH), k_adj_eside (gathers the adjoint of E', adds
-// the objective, takes the Newton solve, eps and chi back, then the adjoints of the E
-// kernel's B differences and of psiE) and k_adj_bgather (adds those to the adjoint of B).
+// the objective, takes the Newton solve, eps, chi and the pole recursion back, then the
+// adjoints of the E kernel's B differences and of psiE) and k_adj_bgather (adds those to the
+// adjoint of B).  A complex accumulator's adjoint is held as (dL/dRe, dL/dIm); through
+// w = K z it goes back as conj(K) lw, and a real input x of w = G x receives Re(conj(G) lw).
 #include <hip/hip_runtime.h>
 
 #include "mpx_launch.h"
@@ -105,7 +112,11 @@ __global__ __launch_bounds__(256) void k_adj_eside(AdjArgs a)
     if (k == a.kobj)
         for (int q = 0; q < 3; ++q) le[q] += 2.0 * a.wobj * e1[q];
     const double eps = a.epsf ? a.epsf[c] : a.eps;
-    const double A = a.chi / eps;
+    double S = 0.0;  // sum_p Re Dd_p: the part of sum_p Re P_p proportional to E
+    for (int p = 0; p < a.np; ++p) S += a.pc.dd[p].x;
+    const double denom = eps + a.sumChi0 - a.sumXi0;
+    const double numer = eps - a.sumXi0;
+    const double A = a.chi / denom;
     // the Kerr solve E' = (e / s) C: C is recovered from E' (e = |E'|, s = A e^3 + e)
     const double e = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
     double cv[3] = {0, 0, 0}, cb[3], Ab = 0.0;
@@ -121,17 +132,30 @@ __global__ __launch_bounds__(256) void k_adj_eside(AdjArgs a)
     } else {
         for (int q = 0; q < 3; ++q) cb[q] = le[q];  // the limit of dE'/dC at C = 0 is the identity
     }
-    // C = ((eps + chi |E|^2) E + dt R) / eps
+    // C denom = (numer + S + chi |E|^2) E + dt R + sum_p Re Psi_p; denom and numer both
+    // grow by d eps, so dC/deps = (E - C) / denom
     const double E2 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
     const double Ecb = e0[0] * cb[0] + e0[1] * cb[1] + e0[2] * cb[2];
-    const double f = eps + a.chi * E2;
-    for (int q = 0; q < 3; ++q) a.lE[q][c] = (f * cb[q] + 2.0 * a.chi * e0[q] * Ecb) / eps;
+    const double f = numer + S + a.chi * E2;
+    double lnew[3];
+    for (int q = 0; q < 3; ++q) lnew[q] = (f * cb[q] + 2.0 * a.chi * e0[q] * Ecb) / denom;
+    // poles: Psi_p' = G_p E + Crec_p Psi_p with G_p = dxi0_p + Crec_p Dd_p, and C reads Re Psi_p
+    for (int p = 0; p < a.np; ++p) {
+        const double2 K = a.pc.crec[p], X = a.pc.dxi0[p], Dd = a.pc.dd[p];
+        const double Gr = X.x + (K.x * Dd.x - K.y * Dd.y), Gi = X.y + (K.x * Dd.y + K.y * Dd.x);
+        for (int q = 0; q < 3; ++q) {
+            const double2 l = a.lpsi[p][q][c];
+            lnew[q] += Gr * l.x + Gi * l.y;
+            a.lpsi[p][q][c] = make_double2(K.x * l.x + K.y * l.y + cb[q] / denom, K.x * l.y - K.y * l.x);
+        }
+    }
+    for (int q = 0; q < 3; ++q) a.lE[q][c] = lnew[q];
     double ecb = 0.0;
     for (int q = 0; q < 3; ++q) ecb += (e0[q] - cv[q]) * cb[q];
-    a.epsb[c] += -Ab * a.chi / (eps * eps) + ecb / eps;
-    a.chib[c] += Ab / eps + E2 * Ecb / eps;
+    a.epsb[c] += -Ab * a.chi / (denom * denom) + ecb / denom;
+    a.chib[c] += Ab / denom + E2 * Ecb / denom;
     double rb[3];
-    for (int q = 0; q < 3; ++q) rb[q] = a.dt * cb[q] / eps;
+    for (int q = 0; q < 3; ++q) rb[q] = a.dt * cb[q] / denom;
     // R_x = D[y][z] iky - D[z][y] ikz + J_x, R_y = D[z][x] ikz - D[x][z] ikx + J_y,
     // R_z = D[x][y] ikx - D[y][x] iky + J_z
     const double ikx = a.ikdE[0][i], iky = a.ikdE[1][j], ikz = a.ikdE[2][k];

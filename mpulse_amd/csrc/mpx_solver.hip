@@ -3103,7 +3103,8 @@ struct DevBufs {
     }
 };
 
-// the forward state of the fused float64 path without poles: E, B and the CPML auxiliaries
+// the forward state of the fused float64 path: E, B, the pole accumulators and the CPML
+// auxiliaries
 struct StateRef {
     std::vector<std::pair<void *, size_t>> arr;  // (device pointer, doubles)
 };
@@ -3115,6 +3116,7 @@ StateRef state_of(mpx_solver *s)
         r.arr.push_back({s->E[c], s->cells});
         r.arr.push_back({s->B[c], s->cells});
     }
+    for (int p = 0; p < s->np; ++p) r.arr.push_back({s->psi[p][0], mpx::psi_block_elems(s->cells) * 2});
     for (const CpmlFace &F : s->efaces) {
         r.arr.push_back({F.psi0, face_count(F)});
         r.arr.push_back({F.psi1, face_count(F)});
@@ -3135,9 +3137,8 @@ int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K,
 {
     if (!s || !obj || !J || nsteps < 1 || K < 1) return fail(MPX_ERR_INVALID, "invalid adjoint arguments");
     MPX_NOT_IN_BATCH(s);
-    if (!s->plrc || s->np != 0 || s->esz != 8 || s->path != MPX_PATH_FUSED || s->plasma || s->pcur)
-        return fail(MPX_ERR_UNSUPPORTED, "the adjoint covers the float64 fused FDTD_PLRC update without active "
-                                         "Lorentz poles or plasma");
+    if (!s->plrc || s->esz != 8 || s->path != MPX_PATH_FUSED || s->plasma || s->pcur)
+        return fail(MPX_ERR_UNSUPPORTED, "the adjoint covers the float64 fused FDTD_PLRC update without plasma");
     if (!s->periodic[0] || !s->periodic[1] || !s->periodic[2])
         return fail(MPX_ERR_UNSUPPORTED, "the adjoint needs the periodic boundary");
     for (const std::vector<Current> *list : {&s->ecur, &s->hcur})
@@ -3190,9 +3191,18 @@ int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K,
     mpx::AdjArgs a{};
     a.g = s->g;
     a.dt = s->d.dt;
-    a.chi = s->d.chi;
+    a.chi = s->nonlin ? s->d.chi : 0.0;  // the linear core has no Kerr term
     a.eps = s->d.eps;
     a.epsf = s->d_eps;
+    a.np = s->np;
+    a.pc = s->pc;
+    a.sumChi0 = s->sumChi0;
+    a.sumXi0 = s->sumXi0;
+    for (int p = 0; p < s->np; ++p)
+        for (int q = 0; q < 3; ++q) {
+            a.lpsi[p][q] = (double2 *)mem.get(2 * s->cells, st);
+            if (!a.lpsi[p][q]) return fail(MPX_ERR_NOMEM, "adjoint allocation failed");
+        }
     for (int d = 0; d < 3; ++d) {
         a.ikdE[d] = s->d_ikdE[d];
         a.ikdH[d] = s->d_ikdH[d];
@@ -3280,7 +3290,7 @@ int mpx_adjoint_grad(mpx_solver *s, const mpx_objective *obj, int nsteps, int K,
         for (int i = 1; i <= g.n[0]; ++i)
             for (int j = 1; j <= g.n[1]; ++j)
                 for (int k = 1; k <= g.n[2]; ++k) sum += hb[g.at(i, j, k)];
-        *dJ_dchi = sum;
+        *dJ_dchi = s->nonlin ? sum : 0.0;
     }
     MPX_HIP(hipStreamSynchronize(st));
     return MPX_OK;

@@ -283,6 +283,12 @@ struct AdjArgs {
     CpmlFace hface[kMaxFaces];
     int efbd[3][2], hfbd[3][2];
     double *lpsiE[kMaxFaces][2], *lpsiH[kMaxFaces][2];  // adjoints of the CPML auxiliaries
+    // Lorentz poles: constants of the active poles and the adjoints of their accumulators,
+    // [pole][component] at cell c (storage index)
+    int np;
+    PoleConsts pc;
+    double sumChi0, sumXi0;
+    double2 *lpsi[3][3];
 };
 
 // one run of a batch: its tile grid and the number of its first tile in the batch's list
