@@ -445,9 +445,9 @@ __device__ __forceinline__ void add_sheet_planes(const Sheet *sh, int n, int py,
     for (int s = 0; s < n; ++s)
         if (k == sh[s].kp) {
             int q = i * py + j;
-            x += sh[s].J[0][q];
-            y += sh[s].J[1][q];
-            z += sh[s].J[2][q];
+            x += gld(sh[s].J[0] + q);
+            y += gld(sh[s].J[1] + q);
+            z += gld(sh[s].J[2] + q);
         }
 }
 

@@ -55,13 +55,15 @@ __device__ __forceinline__ void cpml_hface_pp(const CpmlFace &f, const T *o0, co
 template <typename T, bool CPML, bool RES>
 __device__ __forceinline__ void pass_m(const PassArgs<T> &a, int i, int j, int k, bool store, double dEy_z,
                                        double dEx_z, double dEz_y, double dEx_y, double dEz_x, double dEy_x,
-                                       double &mx, double &my, double &mz, bool cp, bool sh)
+                                       double &mx, double &my, double &mz, bool cp, bool sh, bool live = true)
 {
     mx = my = mz = 0.0;
     if (sh && a.sheets_first) add_sheet_planes(a.hsheet, a.nhsheets, a.g.py, i, j, k, mx, my, mz);
     if (CPML && cp) {
         bool in[2];
         in_faces(a.hfbd, a.hface, 1, j, in);
+        in[0] = in[0] && live;
+        in[1] = in[1] && live;
 #pragma unroll
         for (int s = 0; s < 2; ++s)
             if (in[s]) {
@@ -74,6 +76,8 @@ __device__ __forceinline__ void pass_m(const PassArgs<T> &a, int i, int j, int k
                 mx += v1;
             }
         in_faces(a.hfbd, a.hface, 0, i, in);
+        in[0] = in[0] && live;
+        in[1] = in[1] && live;
 #pragma unroll
         for (int s = 0; s < 2; ++s)
             if (in[s]) {
@@ -86,6 +90,8 @@ __device__ __forceinline__ void pass_m(const PassArgs<T> &a, int i, int j, int k
                 mz += v1;
             }
         in_faces(a.hfbd, a.hface, 2, k, in);
+        in[0] = in[0] && live;
+        in[1] = in[1] && live;
 #pragma unroll
         for (int s = 0; s < 2; ++s)
             if (in[s]) {
@@ -105,13 +111,15 @@ __device__ __forceinline__ void pass_m(const PassArgs<T> &a, int i, int j, int k
 template <typename T, bool CPML, bool RES>
 __device__ __forceinline__ void pass_j(const PassArgs<T> &a, int i, int j, int k, double dBz_y, double dBy_z,
                                        double dBx_z, double dBz_x, double dBy_x, double dBx_y, double &jx,
-                                       double &jy, double &jz, bool cp, bool sh)
+                                       double &jy, double &jz, bool cp, bool sh, bool live = true)
 {
     jx = jy = jz = 0.0;
     auto add_cpml = [&]() {
         if (CPML && cp) {
             bool in[2];
             in_faces(a.efbd, a.eface, 1, j, in);
+            in[0] = in[0] && live;
+            in[1] = in[1] && live;
 #pragma unroll
             for (int s = 0; s < 2; ++s)
                 if (in[s]) {
@@ -122,6 +130,8 @@ __device__ __forceinline__ void pass_j(const PassArgs<T> &a, int i, int j, int k
                     jx += v1;
                 }
             in_faces(a.efbd, a.eface, 0, i, in);
+            in[0] = in[0] && live;
+            in[1] = in[1] && live;
 #pragma unroll
             for (int s = 0; s < 2; ++s)
                 if (in[s]) {
@@ -132,6 +142,8 @@ __device__ __forceinline__ void pass_j(const PassArgs<T> &a, int i, int j, int k
                     jz += v1;
                 }
             in_faces(a.efbd, a.eface, 2, k, in);
+            in[0] = in[0] && live;
+            in[1] = in[1] && live;
 #pragma unroll
             for (int s = 0; s < 2; ++s)
                 if (in[s]) {
@@ -182,15 +194,24 @@ __device__ __forceinline__ Cell3<T, RES> ld3(const T *const f[3], const unsigned
 
 // B^(n+1) at a cell from its B^n, the six E differences (dEy_z, dEx_z, dEz_y, dEx_y, dEz_x,
 // dEy_x) and its position; the CPML H auxiliaries are written only with store
+// (kdx, kdy, kdz: the reciprocal kappa * delta lines of the H side at i, j, k)
+template <typename T, bool CPML, bool RES>
+__device__ __forceinline__ void b_cell_k(const PassArgs<T> &a, int i, int j, int k, const Cell3<T, RES> &b0,
+                                         const double d[6], bool store, T h[3], unsigned &w, double b[3], bool cp,
+                                         bool sh, bool live, double kdx, double kdy, double kdz)
+{
+    double mx, my, mz;
+    pass_m<T, CPML, RES>(a, i, j, k, store, d[0], d[1], d[2], d[3], d[4], d[5], mx, my, mz, cp, sh, live);
+    b_update<T, RES>((double)b0.x, (double)b0.y, (double)b0.z, d[0], d[1], d[2], d[3], d[4], d[5], kdx, kdy, kdz,
+                     a.dt, mx, my, mz, h, w, b);
+}
 template <typename T, bool CPML, bool RES>
 __device__ __forceinline__ void b_cell(const PassArgs<T> &a, int i, int j, int k, const Cell3<T, RES> &b0,
                                        const double d[6], bool store, T h[3], unsigned &w, double b[3], bool cp,
-                                       bool sh)
+                                       bool sh, bool live = true)
 {
-    double mx, my, mz;
-    pass_m<T, CPML, RES>(a, i, j, k, store, d[0], d[1], d[2], d[3], d[4], d[5], mx, my, mz, cp, sh);
-    b_update<T, RES>((double)b0.x, (double)b0.y, (double)b0.z, d[0], d[1], d[2], d[3], d[4], d[5], gld(a.ikdH[0] + i),
-                     gld(a.ikdH[1] + j), gld(a.ikdH[2] + k), a.dt, mx, my, mz, h, w, b);
+    b_cell_k<T, CPML, RES>(a, i, j, k, b0, d, store, h, w, b, cp, sh, live, gld(a.ikdH[0] + i), gld(a.ikdH[1] + j),
+                           gld(a.ikdH[2] + k));
 }
 
 // One cell per thread, in the fused kernels' banded tile order (64 z cells x 4 rows per block).
@@ -342,6 +363,391 @@ __global__ __launch_bounds__(256) void k_pass(const PassArgs<T> *__restrict__ pa
     if (RES) store_once(&a.Enlo[c], ew);
 }
 
+// ---------------------------------------------------------------------------------------
+// The same pass with each block marching over a segment of x planes (k_march).
+//
+// A block owns 64 z cells x R rows and walks `seg` x planes.  It runs R + 1 waves: waves 1..R
+// hold the rows j0 .. j0 + R - 1, wave 0 the row j0 - 1 below the tile, whose B^(n+1) the first
+// row's E update needs.  Per plane every wave computes B^(n+1) of its row; the E update then
+// takes B of the x - 1 plane from the wave's own result of the previous plane (registers), B
+// of the y - 1 row from the wave below through LDS, B of the z - 1 cell from the previous lane
+// (DPP) and, for lane 0, from the column k0 - 1 that wave 0's first R lanes compute beside the
+// halo row.  So every word of E, B and the accumulators crosses HBM once per step, apart from
+// the halo: one row per tile, one column cell per row, one x plane per segment (recomputed,
+// never stored); the y + 1 and z + 1 E words of the B curl are the same lines other waves
+// read, from L2.  One barrier per plane; the LDS rows are double-buffered by plane parity, so a
+// row written for plane p + 1 never overwrites one still being read for plane p.
+//
+// Each wave requests the words of plane p + 1 before it computes plane p.  Every such load and
+// every field store is issued unconditionally, so that the number of memory operations in
+// flight is the same on every path and a wave waits only for the plane it computes, not for
+// the one it prefetched or for its own stores: lanes outside the grid (a partial last row of
+// tiles or z tile) read and write the front padding of their row (storage positions 0 .. 30
+// of a row, before the ghost cell k = 0, hold no cell and are read by no kernel), the last
+// plane of a segment prefetches position 0 of the plane (padding), and the CPML auxiliaries,
+// which only some cells have, are updated under a per-lane mask.
+// ---------------------------------------------------------------------------------------
+
+// a cell of E or B as stored: three words and the residual word (zero without residuals)
+template <typename T, bool RES>
+struct Raw3 {
+    T x, y, z;
+    unsigned w;
+};
+template <typename T, bool RES>
+__device__ __forceinline__ Cell3<T, RES> cell_of(const Raw3<T, RES> &r)
+{
+    if constexpr (RES) return {res_decode(r.x, r.w, 0), res_decode(r.y, r.w, 1), res_decode(r.z, r.w, 2)};
+    else return {r.x, r.y, r.z};
+}
+// two components of a row of cells and their residual words, staged in LDS
+template <typename T, bool RES>
+struct Row2 {
+    T a[64], b[64];
+    unsigned w[RES ? 64 : 1];
+};
+template <typename T, bool RES>
+__device__ __forceinline__ void row_put(Row2<T, RES> &r, int l, T a, T b, unsigned w)
+{
+    r.a[l] = a;
+    r.b[l] = b;
+    if (RES) r.w[l] = w;
+}
+template <typename T, bool RES>
+__device__ __forceinline__ unsigned row_w(const Row2<T, RES> &r, int l)
+{
+    if constexpr (RES) return r.w[l];
+    else return 0u;
+}
+
+// Word at byte offset bo (per lane, 32 bits: a plane holds (ny + 2) pz words < 2^32 bytes)
+// from the plane base `base` (wave-uniform): the load and store take the scalar-base form
+template <typename T>
+__device__ __forceinline__ T ldp(const T *base, unsigned bo)
+{
+    return gld((const T *)((const char *)base + bo));
+}
+template <typename T>
+__device__ __forceinline__ void stp_once(T *base, unsigned bo, T v)
+{
+    store_once((T *)((char *)base + bo), v);
+}
+// A read-only table entry at a wave-uniform address, through the constant address space: a
+// scalar load into SGPRs, counted apart from the vector loads in flight
+__device__ __forceinline__ double kload(const double *p)
+{
+    return *(const __attribute__((address_space(4))) double *)p;
+}
+// three components and the residual word of the cell at bo in plane `pl` (storage offset)
+template <typename T, bool RES>
+__device__ __forceinline__ Raw3<T, RES> ldp3(const T *const f[3], const unsigned *lo, size_t pl, unsigned bo)
+{
+    return {ldp(f[0] + pl, bo), ldp(f[1] + pl, bo), ldp(f[2] + pl, bo), RES ? ldp(lo + pl, bo) : 0u};
+}
+
+// Whether a CPML face of the E side or of the H side holds cell (i, j, k)
+template <typename T>
+__device__ __forceinline__ bool in_cpml(const PassArgs<T> &a, bool eside, int i, int j, int k)
+{
+    const int x[3] = {i, j, k};
+    bool in = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int f = eside ? a.efbd[d][q] : a.hfbd[d][q];
+            if (f >= 0) {
+                const CpmlFace &F = eside ? a.eface[f] : a.hface[f];
+                in |= x[d] >= F.lo[d] && x[d] <= F.hi[d];
+            }
+        }
+    return in;
+}
+
+// accumulators of the active poles through one base pointer per pole: component q of a pole
+// sits 64 elements after component q - 1 (psi_at), so a lane needs one address per pole
+template <int NP>
+struct PsiBase {
+    double2 *b[NP > 0 ? NP : 1];
+    struct Pole {
+        double2 *p;
+        __device__ double2 *operator[](int q) const { return p + 64 * q; }
+    };
+    __device__ Pole operator[](int n) const { return Pole{b[n]}; }
+};
+
+// The words a row wave reads for plane p ahead of its B update: E^(n+1) of the row at p + 1, B^n
+// at p, the y + 1 and z + 1 E words of the B curl, the cell's permittivity.  The halo wave also
+// reads the column cell's E (p and p + 1) and B and its y + 1 and z + 1 E words.  The
+// accumulators are requested separately, after the previous plane's E update.
+template <typename T, bool RES, int NP>
+struct MarchStage {
+    Raw3<T, RES> en, b0, eu, ez1;
+    double eps;  // the cell's permittivity (per-cell Eps)
+};
+template <typename T, bool RES>
+struct ColStage {
+    Raw3<T, RES> e0, b0, ez1, eu, en;
+};
+
+// A block barrier that orders the waves' LDS accesses only.  __syncthreads() also waits for
+// every global load and store the wave has in flight; the march kernel exchanges data between
+// waves through LDS alone, so the loads of the next plane and the stores of this one stay in
+// flight across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R>
+__global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__restrict__ pa, int nbz, int nby,
+                                                        int nseg)
+{
+    using RW = Raw3<T, RES>;
+    using C3 = Cell3<T, RES>;
+    const PassArgs<T> &a = *pa;
+    const GridDims &g = a.g;
+    const int Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
+    const size_t sx = (size_t)g.sx;
+    const int pz = g.pz, z0 = g.z0;
+    constexpr unsigned SZ = sizeof(T);
+    __shared__ Row2<T, RES> brow[2][R + 1], bcol[2];
+
+    // tile of this block: each XCD (blocks are dealt round-robin over the 8 XCDs) takes a
+    // contiguous share of the order z tiles fastest, then tile rows, then x segments, so the
+    // tiles whose rows border each other run at the same time on the same L2
+    int tz, ty, ts;
+    {
+        const unsigned lin = blockIdx.x, total = (unsigned)nbz * nby * nseg;
+        const unsigned per = total / 8u, xcd = lin & 7u, idx = lin >> 3;
+        unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
+        tz = (int)(t % (unsigned)nbz);
+        t /= (unsigned)nbz;
+        ty = (int)(t % (unsigned)nby);
+        ts = (int)(t / (unsigned)nby);
+    }
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // wave-uniform role
+    const int k0 = tz * 64 + 1, j0 = ty * R + 1;
+    const int i0 = ts * a.seg + 1, i1 = min(i0 + a.seg - 1, Nx);
+    const bool kok = k0 + lane <= Nz;
+    const int k = kok ? k0 + lane : Nz;  // lanes past the row end compute a copy of cell Nz
+    const int kp1 = k == Nz ? 1 : k + 1;
+    bool sh = false;  // a source sheet lies on a z plane this wave touches (wave-uniform)
+    unsigned fl = 0;
+
+    if (w == 0) {
+        // ---- halo wave: B^(n+1) of the row j0 - 1 and of the column k0 - 1 of each tile row
+        const int j = j0 == 1 ? Ny : j0 - 1;
+        const bool ok = kok;
+        const unsigned pad = (unsigned)(j * pz + lane % 31);
+        const unsigned o = ok ? (unsigned)(j * pz + k + z0) : pad;
+        const unsigned oyp = ok ? (unsigned)(j0 * pz + k + z0) : pad;  // j + 1 is j0, also across the wrap
+        const unsigned ozp = ok ? (unsigned)(j * pz + kp1 + z0) : pad;
+        const int kc = k0 == 1 ? Nz : k0 - 1;
+        const int jc = j0 + lane;
+        const bool col = lane < R && jc <= Ny;
+        const unsigned oc = col ? (unsigned)(jc * pz + kc + z0) : 0u;
+        const unsigned ocz = col ? (unsigned)(jc * pz + k0 + z0) : 0u;  // kc + 1 is k0, also across the wrap
+        const unsigned ocy = col ? (unsigned)((jc == Ny ? 1 : jc + 1) * pz + kc + z0) : 0u;
+        for (int q = 0; q < a.nhsheets; ++q) sh |= a.hsheet[q].kp == k || (col && a.hsheet[q].kp == kc);
+        sh = __ballot(sh) != 0;
+        auto load = [&](int p, bool real, MarchStage<T, RES, NP> &st, ColStage<T, RES> &cs) {
+            const int pn = p == Nx ? 1 : p + 1;
+            const size_t bp = (size_t)p * sx, bn = (size_t)pn * sx;
+            const unsigned m = real ? 0xffffffffu : 0u;
+            st.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (o & m) * SZ);
+            st.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (o & m) * SZ);
+            st.eu = ldp3<T, RES>(a.Eo, a.Eolo, bp, (oyp & m) * SZ);
+            st.ez1 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ozp & m) * SZ);
+            cs.e0 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (oc & m) * SZ);
+            cs.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (oc & m) * SZ);
+            cs.ez1 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ocz & m) * SZ);
+            cs.eu = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ocy & m) * SZ);
+            cs.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (oc & m) * SZ);
+        };
+        const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
+        const double khyc = gld(a.ikdH[1] + (col ? jc : j)), khzc = kload(a.ikdH[2] + kc);
+        MarchStage<T, RES, NP> cur, nxt;
+        ColStage<T, RES> ccur, cnxt;
+        RW ec = ldp3<T, RES>(a.Eo, a.Eolo, (size_t)i0 * sx, o * SZ);
+        load(i0, true, cur, ccur);
+        for (int p = i0; p <= i1; ++p) {
+            const bool last = p == i1;
+            load(last ? p : p + 1, !last, nxt, cnxt);
+            const int buf = (p - i0) & 1;
+            const double kxh = kload(a.ikdH[0] + p);
+            bool cp = false;
+            if (CPML) cp = __ballot((ok && in_cpml(a, false, p, j, k)) || (col && in_cpml(a, false, p, jc, kc))) != 0;
+            // the halo row
+            const C3 e = cell_of(ec), en = cell_of(cur.en), eu = cell_of(cur.eu), ez = cell_of(cur.ez1);
+            {
+                const double ex = e.x, ey = e.y, ezz = e.z;
+                const double d[6] = {(double)ez.y - ey, (double)ez.x - ex, (double)eu.z - ezz, (double)eu.x - ex,
+                                     (double)en.z - ezz, (double)en.y - ey};
+                T h[3];
+                unsigned hw;
+                double b[3];
+                b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(cur.b0), d, false, h, hw, b, cp, sh, ok, kxh, khy, khz);
+                row_put(brow[buf][0], lane, h[0], h[2], hw);
+            }
+            // the column
+            {
+                const C3 ce = cell_of(ccur.e0), cz = cell_of(ccur.ez1), cu = cell_of(ccur.eu), cn = cell_of(ccur.en);
+                const double cx = ce.x, cy = ce.y, czz = ce.z;
+                const double d[6] = {(double)cz.y - cy, (double)cz.x - cx, (double)cu.z - czz, (double)cu.x - cx,
+                                     (double)cn.z - czz, (double)cn.y - cy};
+                T h[3];
+                unsigned hw;
+                double b[3];
+                b_cell_k<T, CPML, RES>(a, p, col ? jc : j, col ? kc : k, cell_of(ccur.b0), d, false, h, hw, b, cp,
+                                       sh, col, kxh, khyc, khzc);
+                row_put(bcol[buf], lane, h[0], h[1], hw);
+            }
+            lds_barrier();
+            ec = cur.en;
+            cur = nxt;
+            ccur = cnxt;
+        }
+    } else {
+        // ---- row wave: B^(n+1) and E^(n+2) of row j0 + w - 1
+        const int jr = j0 + w - 1;
+        const bool ok = kok && jr <= Ny;
+        const int j = jr <= Ny ? jr : Ny;
+        const unsigned pad = (unsigned)(j * pz + lane % 31);
+        const unsigned o = ok ? (unsigned)(j * pz + k + z0) : pad;
+        const unsigned oyp = ok ? (unsigned)((j == Ny ? 1 : j + 1) * pz + k + z0) : pad;
+        const unsigned ozp = ok ? (unsigned)(j * pz + kp1 + z0) : pad;
+        for (int q = 0; q < a.nhsheets; ++q) sh |= a.hsheet[q].kp == k;
+        for (int q = 0; q < a.nsheets; ++q) sh |= a.sheet[q].kp == k;
+        sh = __ballot(sh) != 0;
+        PsiBase<NP> psb;
+#pragma unroll
+        for (int n = 0; n < NP; ++n) psb.b[n] = a.psi[n][0];
+        // fields: the plane's words at the lane's offsets (real) or at the padding position 0;
+        // accumulators: the cell's (psi) or element 0 of the block
+        auto load = [&](int p, bool real, MarchStage<T, RES, NP> &st) {
+            const int pn = p == Nx ? 1 : p + 1;
+            const size_t bp = (size_t)p * sx, bn = (size_t)pn * sx;
+            const unsigned m = real ? 0xffffffffu : 0u;
+            st.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (o & m) * SZ);
+            st.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (o & m) * SZ);
+            const unsigned by = (oyp & m) * SZ, bz = (ozp & m) * SZ;
+            st.eu = {ldp(a.Eo[0] + bp, by), T(0), ldp(a.Eo[2] + bp, by), RES ? ldp(a.Eolo + bp, by) : 0u};
+            st.ez1 = {ldp(a.Eo[0] + bp, bz), ldp(a.Eo[1] + bp, bz), T(0), RES ? ldp(a.Eolo + bp, bz) : 0u};
+            if (a.epsf) st.eps = ldp(a.epsf + bp, (o & m) * 8u);
+        };
+        // the accumulators of the cell at plane p, or (real false) element 0 of each block
+        auto load_psi = [&](int p, bool real, PsiRegs<NP> &pr) {
+            const size_t pq = real ? psi_at((size_t)p * sx + o) : 0;
+#pragma unroll
+            for (int n = 0; n < NP; ++n)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) pr.v[n][r] = load_once(a.psi[n][0] + pq + 64 * r);
+        };
+        const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
+        const double key = kload(a.ikdE[1] + j), kez = gld(a.ikdE[2] + k);
+        // B^(n+1) of the row at plane p from E^(n+1) at p (ec) and the stage of p; CPML H
+        // auxiliaries stored with `store`
+        auto bstep = [&](int p, const RW &ecw, const MarchStage<T, RES, NP> &st, bool store, bool cp, double kxh,
+                         T h[3], unsigned &hw, double b[3]) {
+            const C3 e = cell_of(ecw), en = cell_of(st.en);
+            const double ex = e.x, ey = e.y, ez = e.z;
+            const double d[6] = {got<T, RES>(st.ez1.y, st.ez1.w, 1) - ey, got<T, RES>(st.ez1.x, st.ez1.w, 0) - ex,
+                                 got<T, RES>(st.eu.z, st.eu.w, 2) - ez, got<T, RES>(st.eu.x, st.eu.w, 0) - ex,
+                                 (double)en.z - ez, (double)en.y - ey};
+            b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(st.b0), d, store, h, hw, b, cp, sh, ok, kxh, khy, khz);
+        };
+        MarchStage<T, RES, NP> cur, nxt;
+        // the segment's x - 1 plane: B^(n+1) recomputed there (not stored), for the first plane's E
+        const int pm = i0 == 1 ? Nx : i0 - 1;
+        RW ec = ldp3<T, RES>(a.Eo, a.Eolo, (size_t)pm * sx, o * SZ);
+        load(pm, true, cur);
+        load(i0, true, nxt);
+        PsiRegs<NP> pr;
+        load_psi(i0, true, pr);
+        double bp1, bp2;  // By, Bz of B^(n+1) at plane p - 1
+        {
+            const bool cp = CPML && __ballot(ok && in_cpml(a, false, pm, j, k)) != 0;
+            T h[3];
+            unsigned hw;
+            double b[3];
+            bstep(pm, ec, cur, false, cp, kload(a.ikdH[0] + pm), h, hw, b);
+            bp1 = b[1];
+            bp2 = b[2];
+        }
+        ec = cur.en;
+        cur = nxt;
+        for (int p = i0; p <= i1; ++p) {
+            const bool last = p == i1;
+            const int buf = (p - i0) & 1;
+            const size_t bp = (size_t)p * sx;
+            const unsigned ob = o * SZ;
+            const double kxh = kload(a.ikdH[0] + p), kxe = kload(a.ikdE[0] + p);
+            const bool cp = CPML && __ballot(ok && (in_cpml(a, false, p, j, k) || in_cpml(a, true, p, j, k))) != 0;
+            T bh[3];
+            unsigned bw;
+            double b[3];
+            bstep(p, ec, cur, true, cp, kxh, bh, bw, b);
+            const RW en = cur.en;  // E^(n+1) at p + 1: E at the next plane
+            const double ee = cur.eps;
+            // the next plane's words, once this plane's are consumed
+            load(last ? p : p + 1, !last, cur);
+            stp_once(a.Bn[0] + bp, ob, bh[0]);
+            stp_once(a.Bn[1] + bp, ob, bh[1]);
+            stp_once(a.Bn[2] + bp, ob, bh[2]);
+            if (RES) stp_once(a.Bnlo + bp, ob, bw);
+            row_put(brow[buf][w], lane, bh[0], bh[2], bw);
+            lds_barrier();
+            // B^(n+1) of the z - 1 cell: previous lane, or the column for lane 0
+            double bzm0 = lane_prev(b[0]), bzm1 = lane_prev(b[1]);
+            if (lane == 0) {
+                const Row2<T, RES> &cr = bcol[buf];
+                const unsigned cw = row_w(cr, w - 1);
+                bzm0 = got<T, RES>(cr.a[w - 1], cw, 0);
+                bzm1 = got<T, RES>(cr.b[w - 1], cw, 1);
+            }
+            const Row2<T, RES> &dn = brow[buf][w - 1];
+            const unsigned dw = row_w(dn, lane);
+            const double bym0 = got<T, RES>(dn.a[lane], dw, 0), bym2 = got<T, RES>(dn.b[lane], dw, 2);
+            // E^(n+2) at the cell
+            const C3 e = cell_of(ec);
+            const double dBz_y = b[2] - bym2, dBx_y = b[0] - bym0;
+            const double dBy_z = b[1] - bzm1, dBx_z = b[0] - bzm0;
+            const double dBz_x = b[2] - bp2, dBy_x = b[1] - bp1;
+            double jx, jy, jz;
+            pass_j<T, CPML, RES>(a, p, j, k, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, jx, jy, jz, cp, sh, ok);
+            double idenom = a.idenom, numer = a.numer;
+            if (a.epsf) {  // cell_eps with the prefetched permittivity
+                const double sg = 0.0;  // no plasma resistivity on this path
+                const double denom = NONLIN ? ee + a.sumChi0 - a.sumXi0 + sg : ee + (a.sumChi0 - a.sumXi0) + sg;
+                numer = ee - a.sumXi0 - sg;
+                idenom = 1.0 / denom;
+            }
+            double nex = e.x, ney = e.y, nez = e.z;
+            unsigned lf = 0;
+            plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kxe, key,
+                                               kez, jx, jy, jz, pr, psb, bp + o, true, a.pc, a.dt, idenom, numer,
+                                               a.chi, a.ncap, &lf);
+            lf |= finite_flag(nex, ney, nez);
+            fl |= ok ? lf : 0u;
+            unsigned ew = 0;
+            const T hx = put<T, RES>(nex, ew, 0), hy = put<T, RES>(ney, ew, 1), hz = put<T, RES>(nez, ew, 2);
+            stp_once(a.En[0] + bp, ob, hx);
+            stp_once(a.En[1] + bp, ob, hy);
+            stp_once(a.En[2] + bp, ob, hz);
+            if (RES) stp_once(a.Enlo + bp, ob, ew);
+            load_psi(last ? p : p + 1, !last, pr);
+            bp1 = b[1];
+            bp2 = b[2];
+            ec = en;
+        }
+    }
+    report(a.status, fl);
+}
+
 // tile grid: 64 z cells x 4 rows x 1 plane per block
 void pass_tiles(const GridDims &g, int &nbx, int &nby, int &nbz)
 {
@@ -374,16 +780,58 @@ static void launch_pass_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np, 
 #undef MPX_PS
 }
 
+// x march: 64 z cells x R rows x `seg` planes per block, R + 1 waves
+template <typename T, bool RES, int R>
+static void launch_march_r(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
+{
+    const int nbz = (a.g.n[2] + 63) / 64, nby = (a.g.n[1] + R - 1) / R, nseg = (a.g.n[0] + a.seg - 1) / a.seg;
+    dim3 grd(nbz * nby * nseg), blk(64, R + 1);
+    const bool cp = a.nefaces > 0 || a.nhfaces > 0;
+#define MPX_MS(NPV, NL, CP) \
+    hipLaunchKernelGGL((k_march<T, NPV, NL, CP, RES, R>), grd, blk, 0, s, dev, nbz, nby, nseg)
+#define MPX_MS_NP(NL, CP)             \
+    switch (np) {                     \
+    case 0: MPX_MS(0, NL, CP); break; \
+    case 1: MPX_MS(1, NL, CP); break; \
+    case 2: MPX_MS(2, NL, CP); break; \
+    default: MPX_MS(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cp) { MPX_MS_NP(true, true) } else { MPX_MS_NP(true, false) }
+    } else {
+        if (cp) { MPX_MS_NP(false, true) } else { MPX_MS_NP(false, false) }
+    }
+#undef MPX_MS_NP
+#undef MPX_MS
+}
+
+template <typename T, bool RES>
+static void launch_march_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
+{
+    if (a.rows == 3)
+        launch_march_r<T, RES, 3>(a, dev, np, nonlin, s);
+    else if (a.rows == 5)
+        launch_march_r<T, RES, 5>(a, dev, np, nonlin, s);
+    else
+        launch_march_r<T, RES, 7>(a, dev, np, nonlin, s);
+}
+
 template <typename T>
 void launch_pass(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
         if (a.Eolo) {
-            launch_pass_t<T, true>(a, dev, np, nonlin, s);
+            if (a.kernel == 1)
+                launch_march_t<T, true>(a, dev, np, nonlin, s);
+            else
+                launch_pass_t<T, true>(a, dev, np, nonlin, s);
             return;
         }
     }
-    launch_pass_t<T, false>(a, dev, np, nonlin, s);
+    if (a.kernel == 1)
+        launch_march_t<T, false>(a, dev, np, nonlin, s);
+    else
+        launch_pass_t<T, false>(a, dev, np, nonlin, s);
 }
 template void launch_pass<double>(const PassArgs<double> &, const PassArgs<double> *, int, bool, hipStream_t);
 template void launch_pass<float>(const PassArgs<float> &, const PassArgs<float> *, int, bool, hipStream_t);

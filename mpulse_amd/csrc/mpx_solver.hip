@@ -164,6 +164,9 @@ struct mpx_solver {
                                    // 2 = bands of tile_band rows walked plane by plane (fastest)
     int tile_band = 0;             // 0: chosen from the grid by auto_band
     int b_reverse = 1;             // B kernel walks the banded order backwards
+    // pass path: 1 = x march (k_march, `march_seg` planes x `march_rows` rows per block),
+    // 0 = one cell per thread (k_pass)
+    int pass_kernel = 1, march_seg = 16, march_rows = 7;
     double *d_stage = nullptr;     // x-fastest float64 copy of one grid for mpx_fetch_field
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
@@ -1520,6 +1523,9 @@ static mpx::PassArgs<T> pass_args(mpx_solver *s)
     a.band = s->tile_band > 0 ? s->tile_band : auto_band(s->g, nullptr, false);
     a.status = s->d_status;
     a.ncap = s->ncap;
+    a.kernel = s->pass_kernel;
+    a.seg = s->march_seg < s->g.n[0] ? s->march_seg : s->g.n[0];
+    a.rows = s->march_rows;
     return a;
 }
 
@@ -2570,6 +2576,15 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
     } else if (!std::strcmp(key, "newton_cap")) {
         if (value < 1) return fail(MPX_ERR_INVALID, "newton_cap must be >= 1");
         s->ncap = value;
+    } else if (!std::strcmp(key, "pass_kernel")) {
+        if (value != 0 && value != 1) return fail(MPX_ERR_INVALID, "pass_kernel must be 0 or 1");
+        s->pass_kernel = value;
+    } else if (!std::strcmp(key, "march_seg")) {
+        if (value < 1) return fail(MPX_ERR_INVALID, "march_seg must be >= 1");
+        s->march_seg = value;
+    } else if (!std::strcmp(key, "march_rows")) {
+        if (value != 3 && value != 5 && value != 7) return fail(MPX_ERR_INVALID, "march_rows must be 3, 5 or 7");
+        s->march_rows = value;
     } else if (!std::strcmp(key, "sweep_variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");
         s->sweep_variant = value;

@@ -260,6 +260,9 @@ struct PassArgs {
     int band;                     // rows of a band in the banded tile order
     unsigned *status;             // device status word and Newton iteration cap
     int ncap;
+    int kernel;                   // 0: one cell per thread (k_pass), 1: x march (k_march)
+    int seg;                      // k_march: x planes per block
+    int rows;                     // k_march: tile rows per block (3, 5 or 7)
 };
 
 // Adjoint of one step of the fused float64 PLRC update without Lorentz poles.  The adjoints

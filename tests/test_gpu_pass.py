@@ -6,7 +6,9 @@ the state is read -- every field, accumulator and CPML auxiliary equals the fuse
 for bit, in float64, FP32-bulk and FP32-res.  Grids have a partial last z wave and a partial
 last row group, and the banded tile order runs with several band heights (tile_band), so the
 recomputed neighbours of the kernel (x - 1, y - 1, lane 0's z - 1, the periodic wraps) lie on
-tile and band boundaries.
+tile and band boundaries.  The x-march kernel runs with several tile heights and segment
+lengths, so its halo row, its z - 1 column and its recomputed segment-start plane lie on
+partial tiles and segments.
 """
 import numpy as np
 import pytest
@@ -18,13 +20,27 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 
 
-def pair(make, prec, band=0):
+def pair(make, prec, band=0, tuning=None):
     da = make(prec, abi.PATH_FUSED)
     db = make(prec, abi.PATH_PASS)
     a, b = abi.Solver(da), abi.Solver(db)
     if band:
         b.tune("tile_band", band)
+    for key, value in (tuning or {}).items():
+        b.tune(key, value)
     return a, b
+
+
+# the pass kernels and launch shapes: one cell per thread (two band heights), the x march with
+# its default 7-row tile and 32-plane segments (one partial segment), 3-row tiles over 5-plane
+# segments (a partial last segment), and one-plane segments (every plane a segment start)
+KERNELS = {
+    "cell": (0, {"pass_kernel": 0}),
+    "cell_band3": (3, {"pass_kernel": 0}),
+    "march": (0, {}),
+    "march_r3_s5": (0, {"march_rows": 3, "march_seg": 5}),
+    "march_r7_s1": (0, {"march_rows": 7, "march_seg": 1}),
+}
 
 
 def same_state(a, b, names=FIELDS + ("PsiR1x", "PsiI1z")):
@@ -44,10 +60,11 @@ def pulse(n=37, nz=70, poles=((1.1, 4.0, 0.005),), d_cpml=6):
 
 
 @pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK, abi.PREC_FP32_RES])
-@pytest.mark.parametrize("band", [0, 3])
-def test_pass_equals_fused(prec, band):
-    print(f"case: pass vs fused, precision {prec}, tile_band {band}", flush=True)
-    a, b = pair(pulse(), prec, band)
+@pytest.mark.parametrize("kernel", list(KERNELS))
+def test_pass_equals_fused(prec, kernel):
+    band, tuning = KERNELS[kernel]
+    print(f"case: pass vs fused, precision {prec}, kernel {kernel}", flush=True)
+    a, b = pair(pulse(), prec, band, tuning)
     with a, b:
         for n in (1, 3, 17, 9):  # settled between the calls by the reads
             a.step(n)

@@ -18,10 +18,13 @@ pytestmark = pytest.mark.gpu
 
 class DeviceDouble:
     """One float64 of device memory owned by the test (the caller), through the HIP runtime
-    the library links."""
+    the library links.  The library is loaded first and the runtime is opened by its soname,
+    so that the process's one runtime copy is used: torch ships its own libamdhip64.so, and
+    opening the other file by its plain name would start a second runtime in the process."""
 
     def __init__(self):
-        self.hip = C.CDLL("libamdhip64.so")
+        abi.load()
+        self.hip = C.CDLL("libamdhip64.so.7")
         self.p = C.c_void_p()
         assert self.hip.hipMalloc(C.byref(self.p), C.c_size_t(8)) == 0
         self.write(-1.0)
