@@ -367,25 +367,28 @@ __global__ __launch_bounds__(256) void k_pass(const PassArgs<T> *__restrict__ pa
 // The same pass with each block marching over a segment of x planes (k_march).
 //
 // A block owns 64 z cells x R rows and walks `seg` x planes.  It runs R + 1 waves: waves 1..R
-// hold the rows j0 .. j0 + R - 1, wave 0 the row j0 - 1 below the tile, whose B^(n+1) the first
-// row's E update needs.  Per plane every wave computes B^(n+1) of its row; the E update then
-// takes B of the x - 1 plane from the wave's own result of the previous plane (registers), B
-// of the y - 1 row from the wave below through LDS, B of the z - 1 cell from the previous lane
-// (DPP) and, for lane 0, from the column k0 - 1 that wave 0's first R lanes compute beside the
-// halo row.  So every word of E, B and the accumulators crosses HBM once per step, apart from
-// the halo: one row per tile, one column cell per row, one x plane per segment (recomputed,
-// never stored); the y + 1 and z + 1 E words of the B curl are the same lines other waves
-// read, from L2.  One barrier per plane; the LDS rows are double-buffered by plane parity, so a
-// row written for plane p + 1 never overwrites one still being read for plane p.
+// hold the rows j0 .. j0 + R - 1, wave 0 (the halo wave) the row j0 - 1 below the tile, whose
+// B^(n+1) the first row's E update needs.  Per plane every wave computes B^(n+1) of its row;
+// the E update then takes B of the x - 1 plane from the wave's own result of the previous plane
+// (registers), B of the y - 1 row from the wave below and B of the z - 1 cell from the previous
+// lane (DPP) or, for lane 0, from the column k0 - 1, which the halo wave's first R lanes
+// compute beside its row.  The B curl's E of the y + 1 row comes from the wave above, and the
+// z + 1 cell's E from the next lane or, at the end of the row segment, from the right column
+// the halo wave reads.  Everything exchanged goes through LDS as decoded float64 values, so
+// every word of E, B and the accumulators crosses HBM once per step apart from the halo (one
+// row and one column cell per tile row, one x plane per segment, recomputed and never stored)
+// and is decoded once.  One barrier per plane; the LDS rows are double-buffered by plane
+// parity, so a row written for plane p + 1 never overwrites one still being read for plane p.
 //
-// Each wave requests the words of plane p + 1 before it computes plane p.  Every such load and
-// every field store is issued unconditionally, so that the number of memory operations in
-// flight is the same on every path and a wave waits only for the plane it computes, not for
-// the one it prefetched or for its own stores: lanes outside the grid (a partial last row of
-// tiles or z tile) read and write the front padding of their row (storage positions 0 .. 30
-// of a row, before the ghost cell k = 0, hold no cell and are read by no kernel), the last
-// plane of a segment prefetches position 0 of the plane (padding), and the CPML auxiliaries,
-// which only some cells have, are updated under a per-lane mask.
+// A row wave requests the words of plane p + 1 as soon as it has used those of plane p, and
+// the accumulators of plane p + 1 after the E update of plane p.  Every such load and every
+// field store is issued unconditionally, so that the number of memory operations in flight
+// is the same on every path and a wave waits only for the plane it computes, not for the one
+// it prefetched or for its own stores: lanes outside the grid (a partial last row of tiles or
+// z tile) read and write the front padding of their row (storage positions 0 .. 30 of a row,
+// before the ghost cell k = 0, hold no cell and are read by no kernel), the last plane of a
+// segment prefetches position 0 of the plane (padding), and the CPML auxiliaries, which only
+// some cells have, are updated under a per-lane mask.
 // ---------------------------------------------------------------------------------------
 
 // a cell of E or B as stored: three words and the residual word (zero without residuals)
@@ -400,28 +403,13 @@ __device__ __forceinline__ Cell3<T, RES> cell_of(const Raw3<T, RES> &r)
     if constexpr (RES) return {res_decode(r.x, r.w, 0), res_decode(r.y, r.w, 1), res_decode(r.z, r.w, 2)};
     else return {r.x, r.y, r.z};
 }
-// two components of a row of cells and their residual words, staged in LDS
-template <typename T, bool RES>
-struct Row2 {
-    T a[64], b[64];
-    unsigned w[RES ? 64 : 1];
+// two decoded components of the 64 cells of a row segment, staged in LDS
+struct RowD {
+    double a[64], b[64];
 };
-template <typename T, bool RES>
-__device__ __forceinline__ void row_put(Row2<T, RES> &r, int l, T a, T b, unsigned w)
-{
-    r.a[l] = a;
-    r.b[l] = b;
-    if (RES) r.w[l] = w;
-}
-template <typename T, bool RES>
-__device__ __forceinline__ unsigned row_w(const Row2<T, RES> &r, int l)
-{
-    if constexpr (RES) return r.w[l];
-    else return 0u;
-}
 
 // Word at byte offset bo (per lane, 32 bits: a plane holds (ny + 2) pz words < 2^32 bytes)
-// from the plane base `base` (wave-uniform): the load and store take the scalar-base form
+// from the plane base `base` (wave-uniform)
 template <typename T>
 __device__ __forceinline__ T ldp(const T *base, unsigned bo)
 {
@@ -476,20 +464,6 @@ struct PsiBase {
     __device__ Pole operator[](int n) const { return Pole{b[n]}; }
 };
 
-// The words a row wave reads for plane p ahead of its B update: E^(n+1) of the row at p + 1, B^n
-// at p, the y + 1 and z + 1 E words of the B curl, the cell's permittivity.  The halo wave also
-// reads the column cell's E (p and p + 1) and B and its y + 1 and z + 1 E words.  The
-// accumulators are requested separately, after the previous plane's E update.
-template <typename T, bool RES, int NP>
-struct MarchStage {
-    Raw3<T, RES> en, b0, eu, ez1;
-    double eps;  // the cell's permittivity (per-cell Eps)
-};
-template <typename T, bool RES>
-struct ColStage {
-    Raw3<T, RES> e0, b0, ez1, eu, en;
-};
-
 // A block barrier that orders the waves' LDS accesses only.  __syncthreads() also waits for
 // every global load and store the wave has in flight; the march kernel exchanges data between
 // waves through LDS alone, so the loads of the next plane and the stores of this one stay in
@@ -513,7 +487,10 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
     const size_t sx = (size_t)g.sx;
     const int pz = g.pz, z0 = g.z0;
     constexpr unsigned SZ = sizeof(T);
-    __shared__ Row2<T, RES> brow[2][R + 1], bcol[2];
+    // erow[b][s]: E (x, z) of tile slot s (row j0 - 1 + s, wrapped) at the plane of parity b;
+    // brow[b][s]: B^(n+1) (x, z) of slot s; bcol[b][l]: B^(n+1) (x, y) of the column k0 - 1 of
+    // tile row l; ecol[b][l]: E (x, y) of the right-end neighbour of tile row l
+    __shared__ RowD erow[2][R + 2], brow[2][R + 1], bcol[2], ecol[2];
 
     // tile of this block: each XCD (blocks are dealt round-robin over the 8 XCDs) takes a
     // contiguous share of the order z tiles fastest, then tile rows, then x segments, so the
@@ -532,112 +509,121 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // wave-uniform role
     const int k0 = tz * 64 + 1, j0 = ty * R + 1;
     const int i0 = ts * a.seg + 1, i1 = min(i0 + a.seg - 1, Nx);
+    const int pm = i0 == 1 ? Nx : i0 - 1;  // the x - 1 plane of the segment, recomputed
     const bool kok = k0 + lane <= Nz;
     const int k = kok ? k0 + lane : Nz;  // lanes past the row end compute a copy of cell Nz
-    const int kp1 = k == Nz ? 1 : k + 1;
+    // the last lane of the segment and the cell after it (periodic image at the row end)
+    const int lr = k0 + 63 <= Nz ? 63 : Nz - k0;
+    const int kr = k0 + lr == Nz ? 1 : k0 + 64;
+    const int kc = k0 == 1 ? Nz : k0 - 1;  // the column before the segment
+    auto wrap_row = [&](int r) { return (r - 1) % Ny + 1; };
     bool sh = false;  // a source sheet lies on a z plane this wave touches (wave-uniform)
     unsigned fl = 0;
 
     if (w == 0) {
-        // ---- halo wave: B^(n+1) of the row j0 - 1 and of the column k0 - 1 of each tile row
+        // ---- halo wave: B^(n+1) of the row j0 - 1 and of the column k0 - 1 of each tile row;
+        // E of the row above the tile and of the right-end cells, for the row waves
         const int j = j0 == 1 ? Ny : j0 - 1;
         const bool ok = kok;
         const unsigned pad = (unsigned)(j * pz + lane % 31);
         const unsigned o = ok ? (unsigned)(j * pz + k + z0) : pad;
-        const unsigned oyp = ok ? (unsigned)(j0 * pz + k + z0) : pad;  // j + 1 is j0, also across the wrap
-        const unsigned ozp = ok ? (unsigned)(j * pz + kp1 + z0) : pad;
-        const int kc = k0 == 1 ? Nz : k0 - 1;
+        const unsigned ozp = ok ? (unsigned)(j * pz + (k == Nz ? 1 : k + 1) + z0) : pad;
+        const int ju = wrap_row(j0 + R);  // the row above the tile
+        const unsigned ou = ok ? (unsigned)(ju * pz + k + z0) : pad;
         const int jc = j0 + lane;
         const bool col = lane < R && jc <= Ny;
         const unsigned oc = col ? (unsigned)(jc * pz + kc + z0) : 0u;
         const unsigned ocz = col ? (unsigned)(jc * pz + k0 + z0) : 0u;  // kc + 1 is k0, also across the wrap
         const unsigned ocy = col ? (unsigned)((jc == Ny ? 1 : jc + 1) * pz + kc + z0) : 0u;
+        const unsigned ocr = col ? (unsigned)(jc * pz + kr + z0) : 0u;
         for (int q = 0; q < a.nhsheets; ++q) sh |= a.hsheet[q].kp == k || (col && a.hsheet[q].kp == kc);
         sh = __ballot(sh) != 0;
-        auto load = [&](int p, bool real, MarchStage<T, RES, NP> &st, ColStage<T, RES> &cs) {
-            const int pn = p == Nx ? 1 : p + 1;
-            const size_t bp = (size_t)p * sx, bn = (size_t)pn * sx;
-            const unsigned m = real ? 0xffffffffu : 0u;
-            st.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (o & m) * SZ);
-            st.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (o & m) * SZ);
-            st.eu = ldp3<T, RES>(a.Eo, a.Eolo, bp, (oyp & m) * SZ);
-            st.ez1 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ozp & m) * SZ);
-            cs.e0 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (oc & m) * SZ);
-            cs.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (oc & m) * SZ);
-            cs.ez1 = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ocz & m) * SZ);
-            cs.eu = ldp3<T, RES>(a.Eo, a.Eolo, bp, (ocy & m) * SZ);
-            cs.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (oc & m) * SZ);
-        };
         const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
         const double khyc = gld(a.ikdH[1] + (col ? jc : j)), khzc = kload(a.ikdH[2] + kc);
-        MarchStage<T, RES, NP> cur, nxt;
-        ColStage<T, RES> ccur, cnxt;
-        RW ec = ldp3<T, RES>(a.Eo, a.Eolo, (size_t)i0 * sx, o * SZ);
-        load(i0, true, cur, ccur);
+        // E of the row above the tile and of the right-end cells at plane p, into buffer b
+        auto provide = [&](int p, int b) {
+            const size_t bp = (size_t)p * sx;
+            const RW u = ldp3<T, RES>(a.Eo, a.Eolo, bp, ou * SZ);
+            const RW rc = ldp3<T, RES>(a.Eo, a.Eolo, bp, ocr * SZ);
+            const C3 ud = cell_of(u), rd_ = cell_of(rc);
+            erow[b][R + 1].a[lane] = ud.x;
+            erow[b][R + 1].b[lane] = ud.z;
+            ecol[b].a[lane] = rd_.x;
+            ecol[b].b[lane] = rd_.y;
+        };
+        provide(pm, 0);
+        lds_barrier();
+        provide(i0, 1);
+        lds_barrier();
         for (int p = i0; p <= i1; ++p) {
-            const bool last = p == i1;
-            load(last ? p : p + 1, !last, nxt, cnxt);
-            const int buf = (p - i0) & 1;
+            const int pn = p == Nx ? 1 : p + 1;
+            const size_t bp = (size_t)p * sx, bn = (size_t)pn * sx;
+            const int be = (p - i0 + 1) & 1, bb = (p - i0) & 1;  // parities of the E and B buffers
             const double kxh = kload(a.ikdH[0] + p);
+            // this plane's words: the halo row, its z + 1 cell, the column cells
+            const RW e0 = ldp3<T, RES>(a.Eo, a.Eolo, bp, o * SZ), en = ldp3<T, RES>(a.Eo, a.Eolo, bn, o * SZ);
+            const RW b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, o * SZ), ez = ldp3<T, RES>(a.Eo, a.Eolo, bp, ozp * SZ);
+            const RW ce0 = ldp3<T, RES>(a.Eo, a.Eolo, bp, oc * SZ), cb0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, oc * SZ);
+            const RW cez = ldp3<T, RES>(a.Eo, a.Eolo, bp, ocz * SZ), ceu = ldp3<T, RES>(a.Eo, a.Eolo, bp, ocy * SZ);
+            const RW cen = ldp3<T, RES>(a.Eo, a.Eolo, bn, oc * SZ);
             bool cp = false;
             if (CPML) cp = __ballot((ok && in_cpml(a, false, p, j, k)) || (col && in_cpml(a, false, p, jc, kc))) != 0;
-            // the halo row
-            const C3 e = cell_of(ec), en = cell_of(cur.en), eu = cell_of(cur.eu), ez = cell_of(cur.ez1);
-            {
-                const double ex = e.x, ey = e.y, ezz = e.z;
-                const double d[6] = {(double)ez.y - ey, (double)ez.x - ex, (double)eu.z - ezz, (double)eu.x - ex,
-                                     (double)en.z - ezz, (double)en.y - ey};
+            {  // the halo row; E of its y + 1 row is the tile's first row (slot 1)
+                const C3 e = cell_of(e0), n = cell_of(en), z = cell_of(ez);
+                const double d[6] = {(double)z.y - e.y, (double)z.x - e.x, erow[be][1].b[lane] - e.z,
+                                     erow[be][1].a[lane] - e.x, (double)n.z - e.z, (double)n.y - e.y};
                 T h[3];
                 unsigned hw;
                 double b[3];
-                b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(cur.b0), d, false, h, hw, b, cp, sh, ok, kxh, khy, khz);
-                row_put(brow[buf][0], lane, h[0], h[2], hw);
+                b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(b0), d, false, h, hw, b, cp, sh, ok, kxh, khy, khz);
+                brow[bb][0].a[lane] = b[0];
+                brow[bb][0].b[lane] = b[2];
             }
-            // the column
-            {
-                const C3 ce = cell_of(ccur.e0), cz = cell_of(ccur.ez1), cu = cell_of(ccur.eu), cn = cell_of(ccur.en);
-                const double cx = ce.x, cy = ce.y, czz = ce.z;
-                const double d[6] = {(double)cz.y - cy, (double)cz.x - cx, (double)cu.z - czz, (double)cu.x - cx,
-                                     (double)cn.z - czz, (double)cn.y - cy};
+            {  // the column
+                const C3 e = cell_of(ce0), z = cell_of(cez), u = cell_of(ceu), n = cell_of(cen);
+                const double d[6] = {(double)z.y - e.y, (double)z.x - e.x, (double)u.z - e.z, (double)u.x - e.x,
+                                     (double)n.z - e.z, (double)n.y - e.y};
                 T h[3];
                 unsigned hw;
                 double b[3];
-                b_cell_k<T, CPML, RES>(a, p, col ? jc : j, col ? kc : k, cell_of(ccur.b0), d, false, h, hw, b, cp,
-                                       sh, col, kxh, khyc, khzc);
-                row_put(bcol[buf], lane, h[0], h[1], hw);
+                b_cell_k<T, CPML, RES>(a, p, col ? jc : j, col ? kc : k, cell_of(cb0), d, false, h, hw, b, cp, sh,
+                                       col, kxh, khyc, khzc);
+                bcol[bb].a[lane] = b[0];
+                bcol[bb].b[lane] = b[1];
             }
+            if (p < i1) provide(pn, be ^ 1);
             lds_barrier();
-            ec = cur.en;
-            cur = nxt;
-            ccur = cnxt;
         }
     } else {
-        // ---- row wave: B^(n+1) and E^(n+2) of row j0 + w - 1
+        // ---- row wave: B^(n+1) and E^(n+2) of row j0 + w - 1 (slot w)
         const int jr = j0 + w - 1;
-        const bool ok = kok && jr <= Ny;
-        const int j = jr <= Ny ? jr : Ny;
+        const bool rok = jr <= Ny;
+        const bool ok = kok && rok;
+        // a wave past the last row only provides E of its (wrapped) row to the wave below
+        const int j = rok ? jr : wrap_row(jr);
         const unsigned pad = (unsigned)(j * pz + lane % 31);
-        const unsigned o = ok ? (unsigned)(j * pz + k + z0) : pad;
-        const unsigned oyp = ok ? (unsigned)((j == Ny ? 1 : j + 1) * pz + k + z0) : pad;
-        const unsigned ozp = ok ? (unsigned)(j * pz + kp1 + z0) : pad;
+        const unsigned o = kok ? (unsigned)(j * pz + k + z0) : pad;  // loads
+        const unsigned os = ok ? o : pad;  // stores: only the tile's own cells are written
         for (int q = 0; q < a.nhsheets; ++q) sh |= a.hsheet[q].kp == k;
         for (int q = 0; q < a.nsheets; ++q) sh |= a.sheet[q].kp == k;
         sh = __ballot(sh) != 0;
         PsiBase<NP> psb;
 #pragma unroll
         for (int n = 0; n < NP; ++n) psb.b[n] = a.psi[n][0];
-        // fields: the plane's words at the lane's offsets (real) or at the padding position 0;
-        // accumulators: the cell's (psi) or element 0 of the block
-        auto load = [&](int p, bool real, MarchStage<T, RES, NP> &st) {
+        const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
+        const double key = kload(a.ikdE[1] + j), kez = gld(a.ikdE[2] + k);
+        // E^(n+1) of the row at p + 1 and B^n at p (real: at the lane's cell, else at the
+        // padding position 0), the cell's permittivity
+        struct Stage {
+            RW en, b0;
+            double eps;
+        };
+        auto load = [&](int p, bool real, Stage &st) {
             const int pn = p == Nx ? 1 : p + 1;
-            const size_t bp = (size_t)p * sx, bn = (size_t)pn * sx;
             const unsigned m = real ? 0xffffffffu : 0u;
-            st.en = ldp3<T, RES>(a.Eo, a.Eolo, bn, (o & m) * SZ);
-            st.b0 = ldp3<T, RES>(a.Bo, a.Bolo, bp, (o & m) * SZ);
-            const unsigned by = (oyp & m) * SZ, bz = (ozp & m) * SZ;
-            st.eu = {ldp(a.Eo[0] + bp, by), T(0), ldp(a.Eo[2] + bp, by), RES ? ldp(a.Eolo + bp, by) : 0u};
-            st.ez1 = {ldp(a.Eo[0] + bp, bz), ldp(a.Eo[1] + bp, bz), T(0), RES ? ldp(a.Eolo + bp, bz) : 0u};
-            if (a.epsf) st.eps = ldp(a.epsf + bp, (o & m) * 8u);
+            st.en = ldp3<T, RES>(a.Eo, a.Eolo, (size_t)pn * sx, (o & m) * SZ);
+            st.b0 = ldp3<T, RES>(a.Bo, a.Bolo, (size_t)p * sx, (o & m) * SZ);
+            if (a.epsf) st.eps = ldp(a.epsf + (size_t)p * sx, (o & m) * 8u);
         };
         // the accumulators of the cell at plane p, or (real false) element 0 of each block
         auto load_psi = [&](int p, bool real, PsiRegs<NP> &pr) {
@@ -647,73 +633,77 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
 #pragma unroll
                 for (int r = 0; r < 3; ++r) pr.v[n][r] = load_once(a.psi[n][0] + pq + 64 * r);
         };
-        const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
-        const double key = kload(a.ikdE[1] + j), kez = gld(a.ikdE[2] + k);
-        // B^(n+1) of the row at plane p from E^(n+1) at p (ec) and the stage of p; CPML H
-        // auxiliaries stored with `store`
-        auto bstep = [&](int p, const RW &ecw, const MarchStage<T, RES, NP> &st, bool store, bool cp, double kxh,
+        // B^(n+1) of the row at plane p from E^(n+1) at p (e, decoded), at p + 1 (n) and the
+        // E rows above and to the right (LDS buffer be); CPML H auxiliaries stored with `store`
+        auto bstep = [&](int p, int be, const C3 &e, const C3 &n, const RW &b0w, bool store, bool cp, double kxh,
                          T h[3], unsigned &hw, double b[3]) {
-            const C3 e = cell_of(ecw), en = cell_of(st.en);
-            const double ex = e.x, ey = e.y, ez = e.z;
-            const double d[6] = {got<T, RES>(st.ez1.y, st.ez1.w, 1) - ey, got<T, RES>(st.ez1.x, st.ez1.w, 0) - ex,
-                                 got<T, RES>(st.eu.z, st.eu.w, 2) - ez, got<T, RES>(st.eu.x, st.eu.w, 0) - ex,
-                                 (double)en.z - ez, (double)en.y - ey};
-            b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(st.b0), d, store, h, hw, b, cp, sh, ok, kxh, khy, khz);
+            double zx = lane_next((double)e.x), zy = lane_next((double)e.y);
+            if (lane == lr) {
+                zx = ecol[be].a[w - 1];
+                zy = ecol[be].b[w - 1];
+            }
+            const double d[6] = {zy - e.y, zx - e.x, erow[be][w + 1].b[lane] - e.z, erow[be][w + 1].a[lane] - e.x,
+                                 (double)n.z - e.z, (double)n.y - e.y};
+            b_cell_k<T, CPML, RES>(a, p, j, k, cell_of(b0w), d, store, h, hw, b, cp, sh, ok, kxh, khy, khz);
         };
-        MarchStage<T, RES, NP> cur, nxt;
-        // the segment's x - 1 plane: B^(n+1) recomputed there (not stored), for the first plane's E
-        const int pm = i0 == 1 ? Nx : i0 - 1;
-        RW ec = ldp3<T, RES>(a.Eo, a.Eolo, (size_t)pm * sx, o * SZ);
+        Stage cur;
+        C3 e = cell_of(ldp3<T, RES>(a.Eo, a.Eolo, (size_t)pm * sx, o * SZ));  // E^(n+1) at plane pm
         load(pm, true, cur);
-        load(i0, true, nxt);
-        PsiRegs<NP> pr;
-        load_psi(i0, true, pr);
+        erow[0][w].a[lane] = e.x;
+        erow[0][w].b[lane] = e.z;
+        lds_barrier();
+        // the segment's x - 1 plane: B^(n+1) recomputed there (not stored), for the first plane's E
         double bp1, bp2;  // By, Bz of B^(n+1) at plane p - 1
         {
             const bool cp = CPML && __ballot(ok && in_cpml(a, false, pm, j, k)) != 0;
+            const C3 n = cell_of(cur.en);
             T h[3];
             unsigned hw;
             double b[3];
-            bstep(pm, ec, cur, false, cp, kload(a.ikdH[0] + pm), h, hw, b);
+            bstep(pm, 0, e, n, cur.b0, false, cp, kload(a.ikdH[0] + pm), h, hw, b);
             bp1 = b[1];
             bp2 = b[2];
+            e = n;
+            erow[1][w].a[lane] = e.x;
+            erow[1][w].b[lane] = e.z;
         }
-        ec = cur.en;
-        cur = nxt;
+        load(i0, true, cur);
+        PsiRegs<NP> pr;
+        load_psi(i0, true, pr);
+        lds_barrier();
         for (int p = i0; p <= i1; ++p) {
             const bool last = p == i1;
-            const int buf = (p - i0) & 1;
+            const int be = (p - i0 + 1) & 1, bb = (p - i0) & 1;  // parities of the E and B buffers
             const size_t bp = (size_t)p * sx;
-            const unsigned ob = o * SZ;
+            const unsigned ob = os * SZ;
             const double kxh = kload(a.ikdH[0] + p), kxe = kload(a.ikdE[0] + p);
             const bool cp = CPML && __ballot(ok && (in_cpml(a, false, p, j, k) || in_cpml(a, true, p, j, k))) != 0;
+            const C3 n = cell_of(cur.en);  // E^(n+1) at p + 1
+            const double ee = cur.eps;
             T bh[3];
             unsigned bw;
             double b[3];
-            bstep(p, ec, cur, true, cp, kxh, bh, bw, b);
-            const RW en = cur.en;  // E^(n+1) at p + 1: E at the next plane
-            const double ee = cur.eps;
+            bstep(p, be, e, n, cur.b0, true, cp, kxh, bh, bw, b);
             // the next plane's words, once this plane's are consumed
             load(last ? p : p + 1, !last, cur);
             stp_once(a.Bn[0] + bp, ob, bh[0]);
             stp_once(a.Bn[1] + bp, ob, bh[1]);
             stp_once(a.Bn[2] + bp, ob, bh[2]);
             if (RES) stp_once(a.Bnlo + bp, ob, bw);
-            row_put(brow[buf][w], lane, bh[0], bh[2], bw);
+            brow[bb][w].a[lane] = b[0];
+            brow[bb][w].b[lane] = b[2];
+            erow[be ^ 1][w].a[lane] = n.x;
+            erow[be ^ 1][w].b[lane] = n.z;
             lds_barrier();
-            // B^(n+1) of the z - 1 cell: previous lane, or the column for lane 0
+            // B^(n+1) of the y - 1 row (the wave below) and of the z - 1 cell (previous lane,
+            // or the column for lane 0)
             double bzm0 = lane_prev(b[0]), bzm1 = lane_prev(b[1]);
             if (lane == 0) {
-                const Row2<T, RES> &cr = bcol[buf];
-                const unsigned cw = row_w(cr, w - 1);
-                bzm0 = got<T, RES>(cr.a[w - 1], cw, 0);
-                bzm1 = got<T, RES>(cr.b[w - 1], cw, 1);
+                bzm0 = bcol[bb].a[w - 1];
+                bzm1 = bcol[bb].b[w - 1];
             }
-            const Row2<T, RES> &dn = brow[buf][w - 1];
-            const unsigned dw = row_w(dn, lane);
-            const double bym0 = got<T, RES>(dn.a[lane], dw, 0), bym2 = got<T, RES>(dn.b[lane], dw, 2);
+            const double bym0 = brow[bb][w - 1].a[lane], bym2 = brow[bb][w - 1].b[lane];
             // E^(n+2) at the cell
-            const C3 e = cell_of(ec);
             const double dBz_y = b[2] - bym2, dBx_y = b[0] - bym0;
             const double dBy_z = b[1] - bzm1, dBx_z = b[0] - bzm0;
             const double dBz_x = b[2] - bp2, dBy_x = b[1] - bp1;
@@ -729,7 +719,7 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
             double nex = e.x, ney = e.y, nez = e.z;
             unsigned lf = 0;
             plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kxe, key,
-                                               kez, jx, jy, jz, pr, psb, bp + o, true, a.pc, a.dt, idenom, numer,
+                                               kez, jx, jy, jz, pr, psb, bp + os, true, a.pc, a.dt, idenom, numer,
                                                a.chi, a.ncap, &lf);
             lf |= finite_flag(nex, ney, nez);
             fl |= ok ? lf : 0u;
@@ -742,7 +732,7 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
             load_psi(last ? p : p + 1, !last, pr);
             bp1 = b[1];
             bp2 = b[2];
-            ec = en;
+            e = n;
         }
     }
     report(a.status, fl);
@@ -810,8 +800,6 @@ static void launch_march_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np,
 {
     if (a.rows == 3)
         launch_march_r<T, RES, 3>(a, dev, np, nonlin, s);
-    else if (a.rows == 5)
-        launch_march_r<T, RES, 5>(a, dev, np, nonlin, s);
     else
         launch_march_r<T, RES, 7>(a, dev, np, nonlin, s);
 }

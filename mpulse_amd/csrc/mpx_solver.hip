@@ -2583,7 +2583,7 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
         if (value < 1) return fail(MPX_ERR_INVALID, "march_seg must be >= 1");
         s->march_seg = value;
     } else if (!std::strcmp(key, "march_rows")) {
-        if (value != 3 && value != 5 && value != 7) return fail(MPX_ERR_INVALID, "march_rows must be 3, 5 or 7");
+        if (value != 3 && value != 7) return fail(MPX_ERR_INVALID, "march_rows must be 3 or 7");
         s->march_rows = value;
     } else if (!std::strcmp(key, "sweep_variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");
