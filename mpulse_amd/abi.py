@@ -349,6 +349,10 @@ class Solver:
         self._check(self.lib.mpx_get_time(self.h, C.byref(t)))
         return t.value
 
+    def set_time(self, t):
+        """The time index the sources read (Process::time); unchanged, it leaves the state as is."""
+        self._check(self.lib.mpx_set_time(self.h, int(t)))
+
     def sync(self):
         self._check(self.lib.mpx_sync(self.h))
 

@@ -1819,6 +1819,7 @@ int mpx_set_time(mpx_solver *s, int t)
 {
     if (!s) return fail(MPX_ERR_INVALID, "null solver");
     MPX_NOT_IN_BATCH(s);
+    if (t == s->time) return MPX_OK;  // nothing changes: a pending pass-path half step stays pending
     if (int rs = settle(s)) return rs;
     s->time = t;
     return MPX_OK;

@@ -1,7 +1,8 @@
 // Command-line driver mirroring the reference's main (src/mpulse.cpp:18-77): reads the setup
 // file given as argv[1] (default "setup.dat"), builds the simulation and runs T-total steps on
-// the GPU.  Options after the file name: --fp32 (FP32-bulk storage), --reference (one kernel
-// per reference loop), --sweep (single-pass kernel).
+// the GPU.  Options after the file name: --fp32 (FP32-bulk storage), --fp32res (float32 words
+// with residual words), --reference (one kernel per reference loop), --sweep (single-pass
+// sweep kernel), --pass (one x-march kernel per step: B of one step and E of the next).
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -16,6 +17,8 @@ int main(int argc, char **argv)
     int precision = MPX_PREC_FP64, path = MPX_PATH_FUSED;
     for (int a = 2; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--fp32")) precision = MPX_PREC_FP32_BULK;
+        else if (!std::strcmp(argv[a], "--fp32res")) precision = MPX_PREC_FP32_RES;
+        else if (!std::strcmp(argv[a], "--pass")) path = MPX_PATH_PASS;
         else if (!std::strcmp(argv[a], "--reference")) path = MPX_PATH_REFERENCE;
         else if (!std::strcmp(argv[a], "--sweep")) path = MPX_PATH_SWEEP;
     }

@@ -169,3 +169,39 @@ def test_pass_probes_fall_back_to_fused_halves():
         b.step(20)
         assert np.array_equal(a.get_probes(), b.get_probes())
         same_state(a, b, FIELDS)
+
+
+def test_pass_gauss_inject_equals_fused():
+    """gauss_inject over three sample blocks (src/gaussinject.cpp): the H sheet of step n and
+    the E sheet of step n + 1 are scheduled before the one launch; fields equal the fused path."""
+    import numpy as np_
+
+    def make(prec, path):
+        d, _ = configs.plrc_kerr_pulse(14, 4, 10.0, 4, 1e9, 60, d_src=6, nz=20, precision=prec, path=path)
+        f = abi.factory_defaults(abi.FACTORY_GAUSS_INJECT)
+        f.d, f.gsize_t, f.length, f.width, f.k0, f.gshift, f.gtime, f.amp, f.eps, f.toff = \
+            6, 48, 1.2, 1.1, 2 * np_.pi, 0.4, 0.3, 0.8, 1.3, 2
+        d.factories[1] = f
+        return d
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_RES):
+        print(f"case: gauss_inject pass vs fused, precision {prec}", flush=True)
+        a, b = pair(make, prec)
+        with a, b:
+            for n in (7, 40, 13):
+                a.step(n)
+                b.step(n)
+                same_state(a, b, FIELDS)
+
+
+def test_pass_with_the_drivers_set_time_every_step():
+    """The host driver sets the time index before every step (src/incsource.t:46); with the
+    solver's own time that changes nothing, so the pending half step stays pending and the
+    fields equal the fused path's."""
+    a, b = pair(pulse(n=24, nz=40), abi.PREC_FP32_RES)
+    with a, b:
+        for t in range(9):
+            a.set_time(t)
+            b.set_time(t)
+            a.step(1)
+            b.step(1)
+        same_state(a, b)
