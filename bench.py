@@ -209,15 +209,18 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
 
 def focused_pulse():
     """The focused-pulse self-focusing workload: 384^2 x 768, two Lorentz poles, Kerr from n2,
-    focused pulse (NA 0.1, 30 fs, 5e16 W/m^2), 16-cell CPML, on the fused path in FP32-res and
-    FP32-bulk, its full 2000 steps timed after 10 untimed ones; peak |E| on the focal plane."""
+    focused pulse (NA 0.1, 30 fs, 5e16 W/m^2), 16-cell CPML, its full 2000 steps timed after 10
+    untimed ones: FP32-res on the pass and fused paths, FP32-bulk on the fused path, and a float64
+    run of the same workload whose fields after the 2000 steps each float32 run is compared with,
+    per component (relative L2, bar 1e-5), with the peak |Ex| on the focal plane."""
     import numpy as np
     import torch
 
     from mpulse_amd import abi, configs
-    out = {}
-    for name, prec in (("fp32_res", abi.PREC_FP32_RES), ("fp32_bulk", abi.PREC_FP32_BULK)):
-        d, meta = configs.config_focus(precision=prec)
+    fields = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+    def run(prec, path, keep):
+        d, meta = configs.config_focus(precision=prec, path=path)
         steps = meta["steps"]
         cells = d.nx * d.ny * d.nz
         with abi.Solver(d) as s:
@@ -229,14 +232,32 @@ def focused_pulse():
             s.sync()
             t = time.perf_counter() - t0
             prof = s.time_steps(20)
-            ex = s.get("Ex")[:, :, d.nz // 2]
+            got = {f: s.get(f) for f in fields} if keep else None
         torch.cuda.empty_cache()
-        out[name] = {"grid": [d.nx, d.ny, d.nz], "poles": 2, "steps": steps, "s": t, "ms_per_step": t / steps * 1e3,
-                     "cell_updates_per_s": cells * steps / t, "kernels_ms": {k: v[0] for k, v in prof["kernels"].items()},
-                     "peak_abs_Ex_focal_plane": float(np.max(np.abs(ex)))}
-        print(f"# focused pulse {d.nx}^2 x {d.nz}, 2 poles, {name}: {t:.2f} s for {steps} steps, "
-              f"{out[name]['ms_per_step']:.3f} ms/step, {out[name]['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s",
-              file=sys.stderr, flush=True)
+        r = {"grid": [d.nx, d.ny, d.nz], "poles": 2, "steps": steps, "s": t, "ms_per_step": t / steps * 1e3,
+             "cell_updates_per_s": cells * steps / t, "kernels_ms": {k: v[0] for k, v in prof["kernels"].items()}}
+        return r, got
+
+    out = {}
+    ref_r, ref = run(abi.PREC_FP64, abi.PATH_FUSED, True)
+    out["fp64_fused"] = ref_r
+    nz = ref_r["grid"][2]
+    ref_r["peak_abs_Ex_focal_plane"] = float(np.max(np.abs(ref["Ex"][:, :, nz // 2])))
+    for name, prec, path in (("fp32_res_pass", abi.PREC_FP32_RES, abi.PATH_PASS),
+                             ("fp32_res_fused", abi.PREC_FP32_RES, abi.PATH_FUSED),
+                             ("fp32_bulk_fused", abi.PREC_FP32_BULK, abi.PATH_FUSED)):
+        r, got = run(prec, path, True)
+        r["peak_abs_Ex_focal_plane"] = float(np.max(np.abs(got["Ex"][:, :, nz // 2])))
+        rel = {f: float(np.linalg.norm(got[f] - ref[f]) / np.linalg.norm(ref[f])) for f in fields}
+        del got
+        r["rel_l2_vs_fp64"] = rel
+        r["bar_met"] = {f: rel[f] <= 1e-5 for f in fields}
+        out[name] = r
+        print(f"# focused pulse {r['grid'][0]}^2 x {r['grid'][2]}, 2 poles, {name}: {r['s']:.2f} s for {r['steps']} "
+              f"steps, {r['ms_per_step']:.3f} ms/step, {r['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s; vs "
+              f"float64: " + ", ".join(f"{f} {rel[f]:.2e}" for f in fields), file=sys.stderr, flush=True)
+    print(f"# focused pulse float64 fused: {ref_r['ms_per_step']:.3f} ms/step; peak |Ex| on the focal plane "
+          f"{ref_r['peak_abs_Ex_focal_plane']:.4f}", file=sys.stderr, flush=True)
     return out
 
 

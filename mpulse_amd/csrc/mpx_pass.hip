@@ -795,10 +795,43 @@ static void launch_march_r(const PassArgs<T> &a, const PassArgs<T> *dev, int np,
 #undef MPX_MS
 }
 
+// Tile height when none is set: 7 rows (8-wave blocks, two per CU) when the variant fits four
+// waves per SIMD (<= 128 VGPRs), else 3 rows (4-wave blocks, three per CU at three waves per
+// SIMD; the two- and three-pole float32 + residual variants).  Read once per variant from the
+// compiled kernel.
+template <typename T, bool RES>
+static int march_auto_rows(int np, bool nonlin, bool cp)
+{
+    static int cache[4][2][2] = {};
+    int &r = cache[np < 0 ? 0 : (np > 3 ? 3 : np)][nonlin][cp];
+    if (!r) {
+        const void *f = nullptr;
+#define MPX_MF(NPV, NL, CP) f = (const void *)&k_march<T, NPV, NL, CP, RES, 7>
+#define MPX_MF_NP(NL, CP)             \
+    switch (np) {                     \
+    case 0: MPX_MF(0, NL, CP); break; \
+    case 1: MPX_MF(1, NL, CP); break; \
+    case 2: MPX_MF(2, NL, CP); break; \
+    default: MPX_MF(3, NL, CP); break; \
+    }
+        if (nonlin) {
+            if (cp) { MPX_MF_NP(true, true) } else { MPX_MF_NP(true, false) }
+        } else {
+            if (cp) { MPX_MF_NP(false, true) } else { MPX_MF_NP(false, false) }
+        }
+#undef MPX_MF_NP
+#undef MPX_MF
+        hipFuncAttributes at{};
+        r = (hipFuncGetAttributes(&at, f) == hipSuccess && at.numRegs > 128) ? 3 : 7;
+    }
+    return r;
+}
+
 template <typename T, bool RES>
 static void launch_march_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
-    if (a.rows == 3)
+    const int rows = a.rows ? a.rows : march_auto_rows<T, RES>(np, nonlin, a.nefaces > 0 || a.nhfaces > 0);
+    if (rows == 3)
         launch_march_r<T, RES, 3>(a, dev, np, nonlin, s);
     else
         launch_march_r<T, RES, 7>(a, dev, np, nonlin, s);

@@ -166,7 +166,7 @@ struct mpx_solver {
     int b_reverse = 1;             // B kernel walks the banded order backwards
     // pass path: 1 = x march (k_march, `march_seg` planes x `march_rows` rows per block),
     // 0 = one cell per thread (k_pass)
-    int pass_kernel = 1, march_seg = 16, march_rows = 7;
+    int pass_kernel = 1, march_seg = 16, march_rows = 0;  // rows 0: chosen per kernel variant
     double *d_stage = nullptr;     // x-fastest float64 copy of one grid for mpx_fetch_field
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
@@ -2584,7 +2584,8 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
         if (value < 1) return fail(MPX_ERR_INVALID, "march_seg must be >= 1");
         s->march_seg = value;
     } else if (!std::strcmp(key, "march_rows")) {
-        if (value != 3 && value != 7) return fail(MPX_ERR_INVALID, "march_rows must be 3 or 7");
+        if (value != 0 && value != 3 && value != 7)
+            return fail(MPX_ERR_INVALID, "march_rows must be 3 or 7 (0: chosen from the kernel's registers)");
         s->march_rows = value;
     } else if (!std::strcmp(key, "sweep_variant")) {
         if (value < 0 || value > 2) return fail(MPX_ERR_INVALID, "sweep_variant must be 0, 1 or 2");

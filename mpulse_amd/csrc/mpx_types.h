@@ -262,7 +262,7 @@ struct PassArgs {
     int ncap;
     int kernel;                   // 0: one cell per thread (k_pass), 1: x march (k_march)
     int seg;                      // k_march: x planes per block
-    int rows;                     // k_march: tile rows per block (3 or 7)
+    int rows;                     // k_march: tile rows per block (3 or 7; 0: chosen per variant)
 };
 
 // Adjoint of one step of the fused float64 PLRC update without Lorentz poles.  The adjoints

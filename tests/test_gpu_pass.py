@@ -205,3 +205,39 @@ def test_pass_with_the_drivers_set_time_every_step():
             a.step(1)
             b.step(1)
         same_state(a, b)
+
+
+def test_pass_focused_pulse_two_poles_equals_fused():
+    """A focused pulse (NA 0.4) into the two-pole silica medium + Kerr behind a CPML: the pass
+    path equals the fused path in float64 and FP32-res (the focused-pulse workload's form)."""
+    def make(prec, path):
+        d = abi.desc_defaults()
+        d.nx, d.ny, d.nz = 40, 36, 70
+        d.dx = d.dy = d.dz = configs.DX
+        d.dt = configs.DT
+        d.solver = abi.SOLVER_PLRC_NONLIN
+        d.precision = prec
+        d.path = path
+        d.eps, d.chi = 1.0, 2e-3
+        for q, (deps, om2, delta) in enumerate(configs.SILICA_POLES):
+            d.l_eps[q], d.l_om2[q], d.l_delta[q] = deps, om2, delta
+        cp = abi.factory_defaults(abi.FACTORY_CPML)
+        cp.d = 6
+        cp.sigma_max = configs.sigma_max(6)
+        abi.add_factory(d, cp)
+        f = abi.factory_defaults(abi.FACTORY_FOCUSED_PULSE)
+        f.d, f.width, f.om0, f.length, f.tshift, f.amp = 9, 5.0, 1.0, 4.0, 6.0, 2.0
+        f.eps = configs.medium_eps_at_laser(1.0, configs.SILICA_POLES)
+        f.zshift = -6 * configs.DX
+        f.phase = 0.3
+        abi.add_factory(d, f)
+        return d
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_RES):
+        print(f"case: focused pulse pass vs fused, precision {prec}", flush=True)
+        a, b = pair(make, prec)
+        with a, b:
+            for n in (31, 60):
+                a.step(n)
+                b.step(n)
+                same_state(a, b, FIELDS + ("PsiR1x", "PsiI2z"))
+            assert np.abs(a.get("Ex")).max() > 0.1
