@@ -376,7 +376,7 @@ FORMS = {
     "fp64_reference": (0, 1, "stepD_ref"),
 }
 DEFAULT_FORMS = ("fp32_res_fused", "fp32_res_pass", "fp32_bulk_fused", "fp32_bulk_pass", "fp64_fused",
-                 "fp64_reference")
+                 "fp64_pass", "fp64_reference")
 PREC_NAME = {0: "fp64", 1: "fp32-bulk", 2: "fp32-res"}
 
 

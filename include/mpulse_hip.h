@@ -285,7 +285,8 @@ int  mpx_state_bytes(mpx_solver *s, size_t *bytes);
    counterpart in the reference: the hook of
    the tile autotuner.  "newton_cap" (>= 1, default 200) bounds the Kerr Newton iterations of a
    cell (see mpx_sync).  Pass path: "pass_kernel" 1, the default, marches each block over
-   "march_seg" x planes (default 16) of a 64-cell x "march_rows" tile (3 or 7; 0, the default, takes
+   "march_seg" x planes (0, the default: 16, halved down to 2 on grids too small to fill the
+   chip) of a 64-cell x "march_rows" tile (3 or 7; 0, the default, takes
    7 where the kernel variant fits four waves per SIMD, else 3);
    0 updates one cell per thread. */
 int  mpx_set_tuning(mpx_solver *s, const char *key, int value);

@@ -166,7 +166,8 @@ struct mpx_solver {
     int b_reverse = 1;             // B kernel walks the banded order backwards
     // pass path: 1 = x march (k_march, `march_seg` planes x `march_rows` rows per block),
     // 0 = one cell per thread (k_pass)
-    int pass_kernel = 1, march_seg = 16, march_rows = 0;  // rows 0: chosen per kernel variant
+    // seg 0: 16 planes, fewer where the grid has too few tiles to fill the chip
+    int pass_kernel = 1, march_seg = 0, march_rows = 0;  // rows 0: chosen per kernel variant
     double *d_stage = nullptr;     // x-fastest float64 copy of one grid for mpx_fetch_field
     // sweep path: second set of buffers (swapped with the first after every step)
     void *E2[3] = {}, *B2[3] = {};
@@ -1524,7 +1525,15 @@ static mpx::PassArgs<T> pass_args(mpx_solver *s)
     a.status = s->d_status;
     a.ncap = s->ncap;
     a.kernel = s->pass_kernel;
-    a.seg = s->march_seg < s->g.n[0] ? s->march_seg : s->g.n[0];
+    int seg = s->march_seg;
+    if (seg <= 0) {
+        // 16 planes per block; shorter segments (down to 2) while fewer than 2048 blocks (eight
+        // per CU) would cover the grid with the 3-row tiles of the smallest blocks
+        const long long tiles = (long long)((s->g.n[2] + 63) / 64) * ((s->g.n[1] + 2) / 3);
+        seg = 16;
+        while (seg > 2 && tiles * ((s->g.n[0] + seg - 1) / seg) < 2048) seg /= 2;
+    }
+    a.seg = seg < s->g.n[0] ? seg : s->g.n[0];
     a.rows = s->march_rows;
     return a;
 }
@@ -2581,7 +2590,7 @@ int mpx_set_tuning(mpx_solver *s, const char *key, int value)
         if (value != 0 && value != 1) return fail(MPX_ERR_INVALID, "pass_kernel must be 0 or 1");
         s->pass_kernel = value;
     } else if (!std::strcmp(key, "march_seg")) {
-        if (value < 1) return fail(MPX_ERR_INVALID, "march_seg must be >= 1");
+        if (value < 0) return fail(MPX_ERR_INVALID, "march_seg must be >= 1 (0: chosen from the grid)");
         s->march_seg = value;
     } else if (!std::strcmp(key, "march_rows")) {
         if (value != 0 && value != 3 && value != 7)
