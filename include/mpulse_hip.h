@@ -76,7 +76,8 @@ extern "C" {
 #define MPX_PATH_SWEEP     2  /* one kernel per step (E and B in a single x sweep; periodic boundary) */
 #define MPX_PATH_PASS      3  /* one kernel per step: B of step n and E of step n+1 in one x sweep, so E,
                                  B and the accumulators cross HBM once per step; the first E half and
-                                 the B half owed at a read run on the fused kernels; periodic boundary */
+                                 the B half owed at a read run on the fused kernels; PLRC solvers,
+                                 either boundary */
 
 /* CurrentFactory registry keys, src/fdtd_plrc.t:431-469 */
 #define MPX_FACTORY_CPML          1  /* "cpml_border"          src/cpml_border.cpp:14-140   */

@@ -512,10 +512,15 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
     const int pm = i0 == 1 ? Nx : i0 - 1;  // the x - 1 plane of the segment, recomputed
     const bool kok = k0 + lane <= Nz;
     const int k = kok ? k0 + lane : Nz;  // lanes past the row end compute a copy of cell Nz
-    // the last lane of the segment and the cell after it (periodic image at the row end)
+    // the last lane of the segment and the cell after it (at the row end: the periodic image,
+    // or with a non-periodic z the ghost cell as stored)
     const int lr = k0 + 63 <= Nz ? 63 : Nz - k0;
-    const int kr = k0 + lr == Nz ? 1 : k0 + 64;
-    const int kc = k0 == 1 ? Nz : k0 - 1;  // the column before the segment
+    const int kr = k0 + lr == Nz ? (a.zper ? 1 : Nz + 1) : k0 + 64;
+    // the column before the segment: the periodic image of the row end, or with a non-periodic
+    // z the cell k = 0, whose B the update covers (FDTD_PLRCSolver::stepB, B over [0, N] in z)
+    const int kc = k0 == 1 ? (a.zper ? Nz : 0) : k0 - 1;
+    const bool kzero = !a.zper && k0 == 1;    // this segment's column is the cell k = 0
+    const bool kend = !a.zper && k0 + lr == Nz;  // this segment ends at the row end
     auto wrap_row = [&](int r) { return (r - 1) % Ny + 1; };
     bool sh = false;  // a source sheet lies on a z plane this wave touches (wave-uniform)
     unsigned fl = 0;
@@ -527,7 +532,7 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
         const bool ok = kok;
         const unsigned pad = (unsigned)(j * pz + lane % 31);
         const unsigned o = ok ? (unsigned)(j * pz + k + z0) : pad;
-        const unsigned ozp = ok ? (unsigned)(j * pz + (k == Nz ? 1 : k + 1) + z0) : pad;
+        const unsigned ozp = ok ? (unsigned)(j * pz + (k == Nz ? (a.zper ? 1 : Nz + 1) : k + 1) + z0) : pad;
         const int ju = wrap_row(j0 + R);  // the row above the tile
         const unsigned ou = ok ? (unsigned)(ju * pz + k + z0) : pad;
         const int jc = j0 + lane;
@@ -536,6 +541,7 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
         const unsigned ocz = col ? (unsigned)(jc * pz + k0 + z0) : 0u;  // kc + 1 is k0, also across the wrap
         const unsigned ocy = col ? (unsigned)((jc == Ny ? 1 : jc + 1) * pz + kc + z0) : 0u;
         const unsigned ocr = col ? (unsigned)(jc * pz + kr + z0) : 0u;
+        const unsigned oce = col ? (unsigned)(jc * pz + Nz + 1 + z0) : 0u;  // the row's ghost k = Nz + 1
         for (int q = 0; q < a.nhsheets; ++q) sh |= a.hsheet[q].kp == k || (col && a.hsheet[q].kp == kc);
         sh = __ballot(sh) != 0;
         const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
@@ -586,10 +592,35 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
                 T h[3];
                 unsigned hw;
                 double b[3];
-                b_cell_k<T, CPML, RES>(a, p, col ? jc : j, col ? kc : k, cell_of(cb0), d, false, h, hw, b, cp, sh,
+                b_cell_k<T, CPML, RES>(a, p, col ? jc : j, col ? kc : k, cell_of(cb0), d, kzero, h, hw, b, cp, sh,
                                        col, kxh, khyc, khzc);
                 bcol[bb].a[lane] = b[0];
                 bcol[bb].b[lane] = b[1];
+                if (kzero && col) {
+                    // B of the cell k = 0 is this segment's to store; E there is a ghost the
+                    // update never writes: carried into the new set as it stands
+                    const unsigned ob = oc * SZ;
+                    stp_once(a.Bn[0] + bp, ob, h[0]);
+                    stp_once(a.Bn[1] + bp, ob, h[1]);
+                    stp_once(a.Bn[2] + bp, ob, h[2]);
+                    if (RES) stp_once(a.Bnlo + bp, ob, hw);
+                    stp_once(a.En[0] + bp, ob, ce0.x);
+                    stp_once(a.En[1] + bp, ob, ce0.y);
+                    stp_once(a.En[2] + bp, ob, ce0.z);
+                    if (RES) stp_once(a.Enlo + bp, ob, ce0.w);
+                }
+            }
+            if (kend && col) {  // the ghosts k = Nz + 1 of E and B: carried into the new set
+                const unsigned ob = oce * SZ;
+                const RW ge = ldp3<T, RES>(a.Eo, a.Eolo, bp, ob), gb = ldp3<T, RES>(a.Bo, a.Bolo, bp, ob);
+                stp_once(a.En[0] + bp, ob, ge.x);
+                stp_once(a.En[1] + bp, ob, ge.y);
+                stp_once(a.En[2] + bp, ob, ge.z);
+                if (RES) stp_once(a.Enlo + bp, ob, ge.w);
+                stp_once(a.Bn[0] + bp, ob, gb.x);
+                stp_once(a.Bn[1] + bp, ob, gb.y);
+                stp_once(a.Bn[2] + bp, ob, gb.z);
+                if (RES) stp_once(a.Bnlo + bp, ob, gb.w);
             }
             if (p < i1) provide(pn, be ^ 1);
             lds_barrier();

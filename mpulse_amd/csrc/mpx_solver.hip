@@ -653,8 +653,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         return fail(MPX_ERR_UNSUPPORTED, "float32 + residual storage runs on the fused path");
     if (s->path == MPX_PATH_SWEEP && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
         return fail(MPX_ERR_UNSUPPORTED, "the sweep path needs a PLRC solver and the single_periodic boundary");
-    if (s->path == MPX_PATH_PASS && (!s->plrc || desc->boundary != MPX_BOUNDARY_PERIODIC))
-        return fail(MPX_ERR_UNSUPPORTED, "the pass path needs a PLRC solver and the single_periodic boundary");
+    if (s->path == MPX_PATH_PASS && !s->plrc)
+        return fail(MPX_ERR_UNSUPPORTED, "the pass path needs a PLRC solver");
     for (int n = 0; n < 3; ++n)
         if (s->plrc && !(desc->l_om2[n] > desc->l_delta[n] * desc->l_delta[n]))
             return fail(MPX_ERR_INVALID, "pole " + std::to_string(n + 1) + ": L*Om2 must exceed L*delta^2");
@@ -1535,6 +1535,7 @@ static mpx::PassArgs<T> pass_args(mpx_solver *s)
     }
     a.seg = seg < s->g.n[0] ? seg : s->g.n[0];
     a.rows = s->march_rows;
+    a.zper = s->periodic[2];
     return a;
 }
 
@@ -1565,6 +1566,8 @@ template <typename T>
 static int pass_launch_t(mpx_solver *s)
 {
     const mpx::PassArgs<T> a = pass_args<T>(s);
+    if (!a.zper && a.kernel == 0)
+        return fail(MPX_ERR_UNSUPPORTED, "the one-cell pass kernel (pass_kernel 0) needs the single_periodic boundary");
     // device copy per buffer parity, uploaded again only when it changed
     const int side = s->parity;
     const unsigned char *p = (const unsigned char *)&a;

@@ -263,6 +263,7 @@ struct PassArgs {
     int kernel;                   // 0: one cell per thread (k_pass), 1: x march (k_march)
     int seg;                      // k_march: x planes per block
     int rows;                     // k_march: tile rows per block (3 or 7; 0: chosen per variant)
+    int zper;                     // z periodic (single_periodic); 0: xy-periodic, z ghosts as stored
 };
 
 // Adjoint of one step of the fused float64 PLRC update without Lorentz poles.  The adjoints

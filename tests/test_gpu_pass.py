@@ -241,3 +241,32 @@ def test_pass_focused_pulse_two_poles_equals_fused():
                 b.step(n)
                 same_state(a, b, FIELDS + ("PsiR1x", "PsiI2z"))
             assert np.abs(a.get("Ex")).max() > 0.1
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_RES])
+def test_pass_xy_periodic_equals_fused(prec):
+    """single_xy_periodic (src/periodic_bound.cpp:75-140): z is not wrapped, B is updated at
+    k = 0 too, the z ghosts of E and B keep what is stored in them.  A set state with non-zero
+    z ghosts, several grids with partial z tiles, the pass path equal to the fused path."""
+    rng = np.random.default_rng(11)
+    for nz in (70, 64, 20):
+        def make(p, path, nz=nz):
+            d = pulse(n=26, nz=nz)(p, path)
+            d.boundary = abi.BOUNDARY_XY_PERIODIC
+            return d
+        print(f"case: xy-periodic pass vs fused, precision {prec}, nz {nz}", flush=True)
+        a, b = pair(make, prec)
+        with a, b:
+            a.step(5)
+            b.step(5)
+            same_state(a, b, FIELDS)
+            for f in ("Ex", "By"):
+                x = a.get(f)
+                x[:, :, 0] = 0.01 * rng.standard_normal(x[:, :, 0].shape)
+                x[:, :, -1] = 0.01 * rng.standard_normal(x[:, :, -1].shape)
+                a.set(f, x)
+                b.set(f, x)
+            for n in (3, 12, 7):
+                a.step(n)
+                b.step(n)
+                same_state(a, b, FIELDS + ("PsiR1x",))
