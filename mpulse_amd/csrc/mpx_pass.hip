@@ -111,9 +111,19 @@ __device__ __forceinline__ void pass_m(const PassArgs<T> &a, int i, int j, int k
 template <typename T, bool CPML, bool RES>
 __device__ __forceinline__ void pass_j(const PassArgs<T> &a, int i, int j, int k, double dBz_y, double dBy_z,
                                        double dBx_z, double dBz_x, double dBy_x, double dBx_y, double &jx,
-                                       double &jy, double &jz, bool cp, bool sh, bool live = true)
+                                       double &jy, double &jz, bool cp, bool sh, bool live = true,
+                                       const double *jpv = nullptr)
 {
     jx = jy = jz = 0.0;
+    // the plasma current (jpv) joins the totals at its place in the list (pcur_pos: before the
+    // first group, between the two, after both)
+    auto add_plasma = [&](int pos) {
+        if (jpv && a.pcur && a.pcur_pos == pos) {
+            jx += jpv[0];
+            jy += jpv[1];
+            jz += jpv[2];
+        }
+    };
     auto add_cpml = [&]() {
         if (CPML && cp) {
             bool in[2];
@@ -155,13 +165,17 @@ __device__ __forceinline__ void pass_j(const PassArgs<T> &a, int i, int j, int k
                 }
         }
     };
+    add_plasma(0);
     if (a.sheets_first) {
         if (sh) add_sheet_planes(a.sheet, a.nsheets, a.g.py, i, j, k, jx, jy, jz);
+        add_plasma(1);
         add_cpml();
     } else {
         add_cpml();
+        add_plasma(1);
         if (sh) add_sheet_planes(a.sheet, a.nsheets, a.g.py, i, j, k, jx, jy, jz);
     }
+    add_plasma(2);
 }
 
 // B^(n+1) at one cell as the B update stores it: the stored (rounded) value of each component
@@ -738,13 +752,32 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
             const double dBz_y = b[2] - bym2, dBx_y = b[0] - bym0;
             const double dBy_z = b[1] - bzm1, dBx_z = b[0] - bzm0;
             const double dBz_x = b[2] - bp2, dBy_x = b[1] - bp1;
+            // the plasma OptField and the plasma current run before the E update, on E of this
+            // step (the fused E kernel's order; each cell's owner updates them in place)
+            double sg = 0.0, jpv[3] = {0.0, 0.0, 0.0};
+            if (a.plasma || a.pcur) {
+                const size_t cl = bp + o, cs = bp + os;
+                double r = gld(a.rho + cl);
+                if (a.plasma) {
+                    sg = a.pw * plasma_density_cell(e.x, e.y, e.z, r, a.pn, a.pm, a.pk - 1);
+                    gst(a.rho + cs, r);
+                    gst(a.sigma + cs, sg);
+                }
+                if (a.pcur) {
+                    const double rt = r + a.rho_bg, ev[3] = {e.x, e.y, e.z};
+                    for (int q = 0; q < 3; ++q) {
+                        jpv[q] = plasma_current_comp(gld(a.jp[q] + cl), ev[q], rt, a.gdtn, a.gdtd, a.emdt);
+                        gst(a.jp[q] + cs, jpv[q]);
+                    }
+                }
+            }
             double jx, jy, jz;
-            pass_j<T, CPML, RES>(a, p, j, k, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, jx, jy, jz, cp, sh, ok);
+            pass_j<T, CPML, RES>(a, p, j, k, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, jx, jy, jz, cp, sh, ok, jpv);
             double idenom = a.idenom, numer = a.numer;
-            if (a.epsf) {  // cell_eps with the prefetched permittivity
-                const double sg = 0.0;  // no plasma resistivity on this path
-                const double denom = NONLIN ? ee + a.sumChi0 - a.sumXi0 + sg : ee + (a.sumChi0 - a.sumXi0) + sg;
-                numer = ee - a.sumXi0 - sg;
+            if (a.epsf || a.plasma) {  // cell_eps with the prefetched permittivity and Sigma dt / 2
+                const double e0 = a.epsf ? ee : a.eps, sgh = 0.5 * sg * a.dt;
+                const double denom = NONLIN ? e0 + a.sumChi0 - a.sumXi0 + sgh : e0 + (a.sumChi0 - a.sumXi0) + sgh;
+                numer = e0 - a.sumXi0 - sgh;
                 idenom = 1.0 / denom;
             }
             double nex = e.x, ney = e.y, nez = e.z;

@@ -975,8 +975,8 @@ static int build(mpx_solver *s, const mpx_desc *desc)
         // the plasma OptField (src/fdtd_plrc.t:431-433, PlasmaDensity) and the grids it shares
         // with the plasma current ("PlasmaDensity", "PlasmaJx".., "Sigma")
         s->plasma = desc->plasma != 0;
-        if ((s->plasma || s->pcur) && (s->path == MPX_PATH_SWEEP || s->path == MPX_PATH_PASS))
-            return fail(MPX_ERR_UNSUPPORTED, "plasma needs the fused or reference-structured path");
+        if ((s->plasma || s->pcur) && s->path == MPX_PATH_SWEEP)
+            return fail(MPX_ERR_UNSUPPORTED, "plasma needs the fused, pass or reference-structured path");
         if (s->plasma && desc->plasma_k < 1) return fail(MPX_ERR_INVALID, "plasma K must be >= 1");
         if (s->plasma || s->pcur) {
             s->d_rho = s->dalloc<double>(s->cells, rc);
@@ -1496,7 +1496,23 @@ static mpx::PassArgs<T> pass_args(mpx_solver *s)
     a.sumChi0 = s->sumChi0;
     a.sumXi0 = s->sumXi0;
     a.epsf = s->d_eps;
-    a.plasma = 0;
+    {  // src/plasmadensity.cpp:30-33, src/plasmacurrent.cpp:57-59 (the fused E kernel's values)
+        const double dt = s->d.dt;
+        a.plasma = s->plasma ? 1 : 0;
+        a.pcur = s->pcur ? 1 : 0;
+        a.pcur_pos = s->pcur_pos;
+        a.pk = s->d.plasma_k;
+        a.pn = 0.5 * dt * s->d.plasma_nu;
+        a.pm = dt * s->d.plasma_mpa;
+        a.pw = s->d.plasma_mpa * s->d.plasma_wion;
+        a.gdtn = 1 - 0.5 * s->pcur_gamma * dt;
+        a.gdtd = 1 + 0.5 * s->pcur_gamma * dt;
+        a.emdt = s->pcur_em * dt;
+        a.rho_bg = s->pcur_rhobg;
+        a.rho = s->d_rho;
+        a.sigma = s->d_sigma;
+        for (int c = 0; c < 3; ++c) a.jp[c] = s->d_jp[c];
+    }
     a.pc = s->pc;
     a.nefaces = (int)s->efaces.size();
     a.nhfaces = (int)s->hfaces.size();
@@ -1566,8 +1582,9 @@ template <typename T>
 static int pass_launch_t(mpx_solver *s)
 {
     const mpx::PassArgs<T> a = pass_args<T>(s);
-    if (!a.zper && a.kernel == 0)
-        return fail(MPX_ERR_UNSUPPORTED, "the one-cell pass kernel (pass_kernel 0) needs the single_periodic boundary");
+    if (a.kernel == 0 && (!a.zper || a.plasma || a.pcur))
+        return fail(MPX_ERR_UNSUPPORTED, "the one-cell pass kernel (pass_kernel 0) takes neither plasma nor the "
+                                         "single_xy_periodic boundary");
     // device copy per buffer parity, uploaded again only when it changed
     const int side = s->parity;
     const unsigned char *p = (const unsigned char *)&a;

@@ -244,7 +244,10 @@ struct PassArgs {
     const double *ikdH[3];
     double dt, eps, chi, denom, numer, idenom, sumChi0, sumXi0;
     const double *epsf;           // per-cell permittivity or null
-    int plasma;                   // always 0 on this path (read by cell_eps)
+    // plasma OptField and plasma current, as in StepArgs (updated in place by the cell's owner)
+    int plasma, pcur, pcur_pos, pk;
+    double pn, pm, pw, gdtn, gdtd, emdt, rho_bg;
+    double *rho, *jp[3], *sigma;
     PoleConsts pc;
     int nefaces, nhfaces;
     CpmlFace eface[kMaxFaces];    // E auxiliaries, in place

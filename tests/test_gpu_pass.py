@@ -270,3 +270,33 @@ def test_pass_xy_periodic_equals_fused(prec):
                 a.step(n)
                 b.step(n)
                 same_state(a, b, FIELDS + ("PsiR1x",))
+
+
+@pytest.mark.parametrize("order", [(0, 1, 2), (1, 0, 2), (0, 2, 1)])
+def test_pass_plasma_equals_fused(order):
+    """Multiphoton ionisation, the resistivity Sigma and a plasma current (src/plasmadensity.cpp,
+    src/plasmacurrent.cpp) in the one-pole Kerr medium, with the plasma current listed after the
+    CPML (between the current groups), first, and last: fields, the plasma grids and Sigma equal
+    the fused path's, in float64 and FP32-res."""
+    def make(prec, path):
+        d, _ = configs.plrc_kerr_pulse(30, 6, 6.0, 6, 4e9, 0, d_src=9, precision=prec, path=path, nz=70,
+                                       poles=((1.1, 4.0, 0.005),))
+        d.ny = 33  # a partial last row group
+        d.factories[1].tshift = 12.0
+        f = abi.factory_defaults(abi.FACTORY_PLASMA_CURRENT)
+        f.em, f.gamma, f.rho_bg = 0.8, 0.2, 0.01
+        facs = [abi.Factory.from_buffer_copy(d.factories[0]), f, abi.Factory.from_buffer_copy(d.factories[1])]
+        for q, n in enumerate(order):
+            d.factories[q] = facs[n]
+        d.n_factories = 3
+        d.plasma, d.plasma_k, d.plasma_nu, d.plasma_mpa, d.plasma_wion = 1, 3, 0.05, 0.003, 0.5
+        return d
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_RES):
+        print(f"case: plasma pass vs fused, factory order {order}, precision {prec}", flush=True)
+        a, b = pair(make, prec)
+        with a, b:
+            for n in (20, 35):
+                a.step(n)
+                b.step(n)
+                same_state(a, b, FIELDS + ("PlasmaDensity", "PlasmaJx", "PlasmaJz", "Sigma"))
+            assert np.abs(a.get("PlasmaDensity")).max() > 0
