@@ -489,7 +489,7 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R>
+template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R, bool PL>
 __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__restrict__ pa, int nbz, int nby,
                                                         int nseg)
 {
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
             // the plasma OptField and the plasma current run before the E update, on E of this
             // step (the fused E kernel's order; each cell's owner updates them in place)
             double sg = 0.0, jpv[3] = {0.0, 0.0, 0.0};
-            if (a.plasma || a.pcur) {
+            if (PL) {
                 const size_t cl = bp + o, cs = bp + os;
                 double r = gld(a.rho + cl);
                 if (a.plasma) {
@@ -772,9 +772,10 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
                 }
             }
             double jx, jy, jz;
-            pass_j<T, CPML, RES>(a, p, j, k, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, jx, jy, jz, cp, sh, ok, jpv);
+            pass_j<T, CPML, RES>(a, p, j, k, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, jx, jy, jz, cp, sh, ok,
+                                 PL ? jpv : nullptr);
             double idenom = a.idenom, numer = a.numer;
-            if (a.epsf || a.plasma) {  // cell_eps with the prefetched permittivity and Sigma dt / 2
+            if (a.epsf || (PL && a.plasma)) {  // cell_eps with the prefetched permittivity and Sigma dt / 2
                 const double e0 = a.epsf ? ee : a.eps, sgh = 0.5 * sg * a.dt;
                 const double denom = NONLIN ? e0 + a.sumChi0 - a.sumXi0 + sgh : e0 + (a.sumChi0 - a.sumXi0) + sgh;
                 numer = e0 - a.sumXi0 - sgh;
@@ -835,14 +836,14 @@ static void launch_pass_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np, 
 }
 
 // x march: 64 z cells x R rows x `seg` planes per block, R + 1 waves
-template <typename T, bool RES, int R>
+template <typename T, bool RES, int R, bool PL>
 static void launch_march_r(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
     const int nbz = (a.g.n[2] + 63) / 64, nby = (a.g.n[1] + R - 1) / R, nseg = (a.g.n[0] + a.seg - 1) / a.seg;
     dim3 grd(nbz * nby * nseg), blk(64, R + 1);
     const bool cp = a.nefaces > 0 || a.nhfaces > 0;
 #define MPX_MS(NPV, NL, CP) \
-    hipLaunchKernelGGL((k_march<T, NPV, NL, CP, RES, R>), grd, blk, 0, s, dev, nbz, nby, nseg)
+    hipLaunchKernelGGL((k_march<T, NPV, NL, CP, RES, R, PL>), grd, blk, 0, s, dev, nbz, nby, nseg)
 #define MPX_MS_NP(NL, CP)             \
     switch (np) {                     \
     case 0: MPX_MS(0, NL, CP); break; \
@@ -861,16 +862,16 @@ static void launch_march_r(const PassArgs<T> &a, const PassArgs<T> *dev, int np,
 
 // Tile height when none is set: 7 rows (8-wave blocks, two per CU) when the variant fits four
 // waves per SIMD (<= 128 VGPRs), else 3 rows (4-wave blocks, three per CU at three waves per
-// SIMD; the two- and three-pole float32 + residual variants).  Read once per variant from the
-// compiled kernel.
-template <typename T, bool RES>
+// SIMD; the two- and three-pole float32 + residual variants, the plasma variants).  Read once
+// per variant from the compiled kernel.
+template <typename T, bool RES, bool PL>
 static int march_auto_rows(int np, bool nonlin, bool cp)
 {
     static int cache[4][2][2] = {};
     int &r = cache[np < 0 ? 0 : (np > 3 ? 3 : np)][nonlin][cp];
     if (!r) {
         const void *f = nullptr;
-#define MPX_MF(NPV, NL, CP) f = (const void *)&k_march<T, NPV, NL, CP, RES, 7>
+#define MPX_MF(NPV, NL, CP) f = (const void *)&k_march<T, NPV, NL, CP, RES, 7, PL>
 #define MPX_MF_NP(NL, CP)             \
     switch (np) {                     \
     case 0: MPX_MF(0, NL, CP); break; \
@@ -891,14 +892,24 @@ static int march_auto_rows(int np, bool nonlin, bool cp)
     return r;
 }
 
+template <typename T, bool RES, bool PL>
+static void launch_march_p(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
+{
+    const int rows = a.rows ? a.rows : march_auto_rows<T, RES, PL>(np, nonlin, a.nefaces > 0 || a.nhfaces > 0);
+    if (rows == 3)
+        launch_march_r<T, RES, 3, PL>(a, dev, np, nonlin, s);
+    else
+        launch_march_r<T, RES, 7, PL>(a, dev, np, nonlin, s);
+}
+
+// the plasma updates are a separate instantiation: they raise the register count
 template <typename T, bool RES>
 static void launch_march_t(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonlin, hipStream_t s)
 {
-    const int rows = a.rows ? a.rows : march_auto_rows<T, RES>(np, nonlin, a.nefaces > 0 || a.nhfaces > 0);
-    if (rows == 3)
-        launch_march_r<T, RES, 3>(a, dev, np, nonlin, s);
+    if (a.plasma || a.pcur)
+        launch_march_p<T, RES, true>(a, dev, np, nonlin, s);
     else
-        launch_march_r<T, RES, 7>(a, dev, np, nonlin, s);
+        launch_march_p<T, RES, false>(a, dev, np, nonlin, s);
 }
 
 template <typename T>
