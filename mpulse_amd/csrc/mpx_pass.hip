@@ -10,13 +10,13 @@
 // so the fields agree with the fused path bit for bit.
 //
 // The E update at a cell needs B^(n+1) of the cell and of its x - 1, y - 1 and z - 1
-// neighbours.  Each thread updates one cell and computes the B components it needs at the
-// three neighbours again from old values (their owners compute and store them too).  The
-// extra reads are E and B words that neighbouring threads read as well (L1 / L2 hits in the
-// banded tile order); the pole accumulators -- two thirds of the bytes -- are read once.
-// Threads never wait for each other.  E, B and the CPML H auxiliaries, which the recomputations read, are
-// ping-ponged ("o" read, "n" written); the accumulators and the CPML E auxiliaries, touched
-// only by their owner, are updated in place.  Periodic boundary only.
+// neighbours.  Two kernels provide them.  k_march (the default, below) walks each tile along x
+// and passes the neighbours' values between the waves of a block through LDS.  k_pass updates
+// one cell per thread and computes the B components it needs at the three neighbours again
+// from old values (their owners compute and store them too; single_periodic boundary, no
+// plasma).  E, B and the CPML H auxiliaries, which neighbours read, are ping-ponged ("o" read,
+// "n" written); the accumulators, the CPML E auxiliaries and the plasma grids, touched only by
+// their owner, are updated in place.
 #include <hip/hip_runtime.h>
 
 #include "mpx_device.h"
@@ -228,7 +228,9 @@ __device__ __forceinline__ void b_cell(const PassArgs<T> &a, int i, int j, int k
                            gld(a.ikdH[2] + k));
 }
 
-// One cell per thread, in the fused kernels' banded tile order (64 z cells x 4 rows per block).
+// k_pass: one cell per thread, in the fused kernels' banded tile order (64 z cells x 4 rows per
+// block).  The extra reads of the recomputations are E and B words that neighbouring threads read
+// as well (L1 / L2 hits); the pole accumulators -- two thirds of the bytes -- are read once.
 // The E update at (i, j, k) takes B^(n+1) at the cell itself (computed and stored here) and at
 // x - 1, y - 1 and z - 1 (computed here again from old values and not stored: their owners
 // store them).  Every grid word the thread needs is loaded before anything is computed or
