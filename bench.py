@@ -126,12 +126,12 @@ def cpu_baseline(n=512, n_sample_steps=20, n_single=192, single_steps=10):
                      "logical_cpus": psutil.cpu_count(), "threads_used": threads}}
 
 
-def batch_sweep(steps=500, nruns=64, autotune_steps=3):
+def batch_sweep(steps=500, nruns=64, autotune_steps=10):
     """Uneven-size parameter sweep: nruns independent cubes (48^3..160^3, one pole + Kerr,
-    10-cell CPML, 500 steps each) stepped together by the batched persistent kernels replayed
-    from captured graphs with autotuned tiles, in float64 and in FP32-bulk, against the same
-    runs launched one after another on the float64 plain (reference-structured) path and on
-    the path of the batch's tuned structure (fused or sweep) in the same precision.  Every
+    10-cell CPML, 500 steps each) stepped together by the batched kernels replayed from
+    captured graphs with autotuned tiles, in float64 and in FP32-bulk, against the same runs
+    launched one after another on the float64 plain (reference-structured) path and on the
+    path of the batch's tuned structure (fused, sweep or pass) in the same precision.  Every
     batched run's E and B after the last step are compared bit for bit with that run on its
     own (SHA-256 of the six fields).  Times cover the steps only (set-up excluded), wall clock
     around each set."""
@@ -185,7 +185,7 @@ def batch_sweep(steps=500, nruns=64, autotune_steps=3):
     for name, prec in (("fp64", abi.PREC_FP64), ("fp32_bulk", abi.PREC_FP32_BULK)):
         descs = [d for d, _ in configs.config_sweep(nruns=nruns, steps=steps, precision=prec)]
         t_b, tune, got = batched(descs)
-        same_path = abi.PATH_FUSED if tune["structure"] == "fused" else abi.PATH_SWEEP
+        same_path = {"fused": abi.PATH_FUSED, "sweep": abi.PATH_SWEEP, "pass": abi.PATH_PASS}[tune["structure"]]
         want = []
         t_f = sequential(descs, same_path, want)
         identical = [g == w for g, w in zip(got, want)]

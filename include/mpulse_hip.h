@@ -301,7 +301,7 @@ int  mpx_copy_probe(size_t bytes, int reps, double *ms_per_copy);
    No counterpart in the reference, which runs one simulation per process.  For uneven-size
    parameter sweeps (pulse energy, chi3, grid size): every run is an FDTD_PLRC solver on a
    periodic domain built exactly as mpx_create builds it (sweep path), and all runs advance in
-   lockstep, one persistent launch per step for all of them, replayed from a captured HIP
+   lockstep, one launch per kernel and step for all of them, replayed from a captured HIP
    graph.  The runs must share precision, solver (lin / nonlin), number of active poles and
    whether a CPML is present.  mpx_batch_run hands out a run's handle for reading and writing
    fields (mpx_get_field, mpx_energy, ...); stepping, timing or destroying such a handle
@@ -315,16 +315,20 @@ int  mpx_batch_run(mpx_batch *b, int run, mpx_solver **out);
 int  mpx_batch_step(mpx_batch *b, int nsteps);
 int  mpx_batch_sync(mpx_batch *b);
 int  mpx_batch_get_time(mpx_batch *b, int *time);
-/* "structure" (0: fused E and fused B kernels over all runs, the default; 1: single-pass sweep),
-   "march_b" (fused B: x planes per thread, 1 / 2 / 4), "fused_blocks_per_cu" (fused persistent
-   grid = value x CUs), "variant" (sweep block shape 0..2, as sweep_variant), "seg" (sweep x
-   planes per tile), "blocks_per_cu" (sweep persistent grid), "graph_steps" (steps per captured
-   graph; 0 = none) */
+/* "structure" (0: fused E and fused B kernels over all runs, the default; 1: single-pass sweep;
+   2: x-march pass kernel, one launch per step for B of one step and E of the next, with the
+   fused E kernel opening and the fused B kernel closing every mpx_batch_step call, so that E
+   and B belong to the same step between calls), "march_b" (fused B: x planes per thread,
+   1 / 2 / 4), "fused_blocks_per_cu" (fused persistent grid = value x CUs), "variant" (sweep
+   block shape 0..2, as sweep_variant), "seg" (sweep x planes per tile), "blocks_per_cu" (sweep
+   persistent grid), "pass_seg" (x-march planes per block), "pass_rows" (x-march rows per
+   tile: 3, 7, or 0 for the variant's default), "graph_steps" (steps per captured graph;
+   0 = none) */
 int  mpx_batch_set_tuning(mpx_batch *b, const char *key, int value);
-/* Times the candidate launch shapes of both structures for steps_per_trial steps each, keeps
-   the fastest, then returns every run to its all-zero state at time 0.  result = {structure,
-   then march_b, blocks per CU, 0 (fused) or variant, seg, blocks per CU (sweep)}.  Only
-   before the first step. */
+/* Times the candidate launch shapes of the three structures for steps_per_trial steps each,
+   keeps the fastest, then returns every run to its all-zero state at time 0.  result =
+   {structure, then march_b, blocks per CU, 0 (fused) or variant, seg, blocks per CU (sweep) or
+   pass_seg, pass_rows, 0 (x-march)}.  Only before the first step. */
 int  mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double *best_ms_per_step);
 /* Advance nsteps and report the mean device time per step (HIP events on the batch stream). */
 int  mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step);

@@ -520,6 +520,8 @@ class Batch:
         self._check(self.lib.mpx_batch_autotune(self.h, steps_per_trial, res, C.byref(ms)))
         if res[0] == 0:
             return {"structure": "fused", "march_b": res[1], "blocks_per_cu": res[2], "ms_per_step": ms.value}
+        if res[0] == 2:
+            return {"structure": "pass", "pass_seg": res[1], "pass_rows": res[2], "ms_per_step": ms.value}
         return {"structure": "sweep", "variant": res[1], "seg": res[2], "blocks_per_cu": res[3],
                 "ms_per_step": ms.value}
 

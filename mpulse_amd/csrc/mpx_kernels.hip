@@ -144,10 +144,12 @@ __global__ __launch_bounds__(256) void k_sheet_eval(SheetEval a, FaddeevaTables 
 
 // The sheets of many runs in one launch.  Sheet s owns blocks first[s] .. first[s+1]-1, one
 // block per 64 cells of a row; the step is *time + tstep (the counter lives on the device so
-// that a captured graph can be replayed at later steps).
+// that a captured graph can be replayed at later steps), hshift less for the H currents.  Only
+// the sheets of the sides in `sides` (bit 0: E currents, bit 1: H currents) are evaluated.
 __global__ __launch_bounds__(64) void k_sheet_eval_batch(const SheetEval *__restrict__ list,
                                                          const int *__restrict__ first, int n, FaddeevaTables ft,
-                                                         const int *__restrict__ time, int tstep)
+                                                         const int *__restrict__ time, int tstep, int hshift,
+                                                         int sides)
 {
     const int b = blockIdx.x;
     int lo = 0, hi = n - 1;  // last sheet with first[s] <= b
@@ -156,11 +158,12 @@ __global__ __launch_bounds__(64) void k_sheet_eval_batch(const SheetEval *__rest
         if (first[mid] <= b) lo = mid; else hi = mid - 1;
     }
     const SheetEval &a = list[lo];
+    if (!(sides >> (a.isH ? 1 : 0) & 1)) return;
     const int nyb = (a.ny2 + 63) >> 6;
     const int local = b - first[lo];
     const int i = local / nyb, j = (local - i * nyb) * 64 + threadIdx.x;
     if (i >= a.nx2 || j >= a.ny2) return;
-    sheet_eval_point(a, ft, i, j, *time + tstep);
+    sheet_eval_point(a, ft, i, j, *time + tstep - (a.isH ? hshift : 0));
 }
 
 __global__ void k_advance_time(int *time, int k)
@@ -169,9 +172,10 @@ __global__ void k_advance_time(int *time, int k)
 }
 
 void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
-                             const int *time, int tstep, hipStream_t s)
+                             const int *time, int tstep, hipStream_t s, int hshift, int sides)
 {
-    hipLaunchKernelGGL(k_sheet_eval_batch, dim3(blocks), dim3(64), 0, s, list, first, n, ft, time, tstep);
+    hipLaunchKernelGGL(k_sheet_eval_batch, dim3(blocks), dim3(64), 0, s, list, first, n, ft, time, tstep, hshift,
+                       sides);
 }
 
 void launch_advance_time(int *time, int k, hipStream_t s)

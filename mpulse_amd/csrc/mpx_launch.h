@@ -75,8 +75,16 @@ void launch_fused_E_batch(const StepArgs<T> *args, const BatchRun *runs, int nru
 template <typename T>
 void launch_fused_B_batch(const StepArgs<T> *args, const BatchRun *runs, int nruns, int total, int blocks, int L,
                           bool cpml, hipStream_t s);
+// the sheets of the sides in `sides` (bit 0: E currents, bit 1: H currents); H currents at
+// step *time + tstep - hshift, E currents at *time + tstep
 void launch_sheet_eval_batch(const SheetEval *list, const int *first, int n, int blocks, const FaddeevaTables &ft,
-                             const int *time, int tstep, hipStream_t s);
+                             const int *time, int tstep, hipStream_t s, int hshift = 0, int sides = 3);
+// batched x march (mpx_pass.hip): run r's tiles are runs[r].first .. of a list of `total`
+template <typename T>
+int march_batch_rows(int rows, int np, bool nonlin, bool cp);
+template <typename T>
+void launch_march_batch(const PassArgs<T> *table, const BatchRun *runs, int nruns, int total, int rows, int np,
+                        bool nonlin, bool cp, hipStream_t s);
 void launch_advance_time(int *time, int k, hipStream_t s);
 void launch_adjoint_step(const AdjArgs &a, hipStream_t s);
 void launch_plane_objective(const GridDims &g, const double *const E[3], int kobj, double w, double *acc,

@@ -491,13 +491,13 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// One tile (64 z cells x R rows from (k0, j0) = (64 tz + 1, R ty + 1), x planes
+// seg ts + 1 .. seg (ts + 1)) of the march; the kernels below map their blocks to tiles.
 template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R, bool PL>
-__global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__restrict__ pa, int nbz, int nby,
-                                                        int nseg)
+__device__ __forceinline__ void march_tile(const PassArgs<T> &a, int tz, int ty, int ts)
 {
     using RW = Raw3<T, RES>;
     using C3 = Cell3<T, RES>;
-    const PassArgs<T> &a = *pa;
     const GridDims &g = a.g;
     const int Nx = g.n[0], Ny = g.n[1], Nz = g.n[2];
     const size_t sx = (size_t)g.sx;
@@ -508,19 +508,6 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
     // tile row l; ecol[b][l]: E (x, y) of the right-end neighbour of tile row l
     __shared__ RowD erow[2][R + 2], brow[2][R + 1], bcol[2], ecol[2];
 
-    // tile of this block: each XCD (blocks are dealt round-robin over the 8 XCDs) takes a
-    // contiguous share of the order z tiles fastest, then tile rows, then x segments, so the
-    // tiles whose rows border each other run at the same time on the same L2
-    int tz, ty, ts;
-    {
-        const unsigned lin = blockIdx.x, total = (unsigned)nbz * nby * nseg;
-        const unsigned per = total / 8u, xcd = lin & 7u, idx = lin >> 3;
-        unsigned t = (lin < per * 8u) ? xcd * per + idx : lin;
-        tz = (int)(t % (unsigned)nbz);
-        t /= (unsigned)nbz;
-        ty = (int)(t % (unsigned)nby);
-        ts = (int)(t / (unsigned)nby);
-    }
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // wave-uniform role
     const int k0 = tz * 64 + 1, j0 = ty * R + 1;
@@ -805,6 +792,44 @@ __global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__res
     report(a.status, fl);
 }
 
+// Block -> tile: each XCD (blocks are dealt round-robin over the 8 XCDs) takes a contiguous
+// share of the tile list, so the tiles whose rows border each other run at the same time on
+// the same L2
+__device__ __forceinline__ unsigned xcd_share(unsigned lin, unsigned total)
+{
+    const unsigned per = total / 8u, xcd = lin & 7u, idx = lin >> 3;
+    return (lin < per * 8u) ? xcd * per + idx : lin;
+}
+
+// one grid: tiles in the order z tiles fastest, then tile rows, then x segments
+template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R, bool PL>
+__global__ __launch_bounds__(64 * (R + 1)) void k_march(const PassArgs<T> *__restrict__ pa, int nbz, int nby,
+                                                        int nseg)
+{
+    unsigned t = xcd_share(blockIdx.x, (unsigned)nbz * nby * nseg);
+    const int tz = (int)(t % (unsigned)nbz);
+    t /= (unsigned)nbz;
+    march_tile<T, NP, NONLIN, CPML, RES, R, PL>(*pa, tz, (int)(t % (unsigned)nby), (int)(t / (unsigned)nby));
+}
+
+// Independent runs in one launch: run r owns the tiles runs[r].first .. runs[r+1].first - 1
+// of the list, in its own grid's order; its argument block is pa[r].
+template <typename T, int NP, bool NONLIN, bool CPML, bool RES, int R, bool PL>
+__global__ __launch_bounds__(64 * (R + 1)) void k_march_batch(const PassArgs<T> *__restrict__ pa,
+                                                              const BatchRun *__restrict__ runs, int nruns,
+                                                              int total)
+{
+    const int t = (int)xcd_share(blockIdx.x, (unsigned)total);
+    int lo = 0, hi = nruns - 1;  // last run with first <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (runs[mid].first <= t) lo = mid; else hi = mid - 1;
+    }
+    const BatchRun &B = runs[lo];
+    const int local = t - B.first, tz = local % B.ntz, rest = local / B.ntz;
+    march_tile<T, NP, NONLIN, CPML, RES, R, PL>(pa[lo], tz, rest % B.nty, rest / B.nty);
+}
+
 // tile grid: 64 z cells x 4 rows x 1 plane per block
 void pass_tiles(const GridDims &g, int &nbx, int &nby, int &nbz)
 {
@@ -933,5 +958,53 @@ void launch_pass(const PassArgs<T> &a, const PassArgs<T> *dev, int np, bool nonl
 }
 template void launch_pass<double>(const PassArgs<double> &, const PassArgs<double> *, int, bool, hipStream_t);
 template void launch_pass<float>(const PassArgs<float> &, const PassArgs<float> *, int, bool, hipStream_t);
+
+// Batched x march (no residual words, no plasma): tile height of the one-grid kernel of the
+// same variant unless `rows` (3 or 7) is given
+template <typename T>
+int march_batch_rows(int rows, int np, bool nonlin, bool cp)
+{
+    return rows ? rows : march_auto_rows<T, false, false>(np, nonlin, cp);
+}
+
+template <typename T, int R>
+static void launch_march_batch_r(const PassArgs<T> *table, const BatchRun *runs, int nruns, int total, int np,
+                                 bool nonlin, bool cp, hipStream_t s)
+{
+    dim3 grd(total), blk(64, R + 1);
+#define MPX_MB(NPV, NL, CP)                                                                                      \
+    hipLaunchKernelGGL((k_march_batch<T, NPV, NL, CP, false, R, false>), grd, blk, 0, s, table, runs, nruns, \
+                       total)
+#define MPX_MB_NP(NL, CP)             \
+    switch (np) {                     \
+    case 0: MPX_MB(0, NL, CP); break; \
+    case 1: MPX_MB(1, NL, CP); break; \
+    case 2: MPX_MB(2, NL, CP); break; \
+    default: MPX_MB(3, NL, CP); break; \
+    }
+    if (nonlin) {
+        if (cp) { MPX_MB_NP(true, true) } else { MPX_MB_NP(true, false) }
+    } else {
+        if (cp) { MPX_MB_NP(false, true) } else { MPX_MB_NP(false, false) }
+    }
+#undef MPX_MB_NP
+#undef MPX_MB
+}
+
+template <typename T>
+void launch_march_batch(const PassArgs<T> *table, const BatchRun *runs, int nruns, int total, int rows, int np,
+                        bool nonlin, bool cp, hipStream_t s)
+{
+    if (rows == 3)
+        launch_march_batch_r<T, 3>(table, runs, nruns, total, np, nonlin, cp, s);
+    else
+        launch_march_batch_r<T, 7>(table, runs, nruns, total, np, nonlin, cp, s);
+}
+template int march_batch_rows<double>(int, int, bool, bool);
+template int march_batch_rows<float>(int, int, bool, bool);
+template void launch_march_batch<double>(const PassArgs<double> *, const BatchRun *, int, int, int, int, bool, bool,
+                                         hipStream_t);
+template void launch_march_batch<float>(const PassArgs<float> *, const BatchRun *, int, int, int, int, bool, bool,
+                                        hipStream_t);
 
 }  // namespace mpx

@@ -1078,8 +1078,10 @@ static int build(mpx_solver *s, const mpx_desc *desc)
             }
         }
     }
-    if (s->path == MPX_PATH_PASS) {
-        for (int c = 0; c < 3; ++c) {
+    // the pass path's second set; the sweep path (whose E2 and B2 are already there) takes the
+    // H auxiliaries' second set too, so that a batch of sweep-path runs can run the pass
+    if (s->path == MPX_PATH_PASS || (s->path == MPX_PATH_SWEEP && s->plrc)) {
+        for (int c = 0; c < 3 && s->path == MPX_PATH_PASS; ++c) {
             if (s->esz == 8) {
                 s->E2[c] = s->dalloc<double>(s->cells, rc);
                 s->B2[c] = s->dalloc<double>(s->cells, rc);
@@ -2644,18 +2646,28 @@ struct mpx_batch {
     size_t esz = 8;
     int np = 0;
     bool nonlin = false, cpml = false;
-    int structure = 0;               // 0: fused E + fused B kernels, 1: single-pass sweep
+    // 0: fused E + fused B kernels, 1: single-pass sweep, 2: x-march pass kernel (B of one step
+    // and E of the next per launch; the fused E kernel opens a call, the fused B kernel closes it)
+    int structure = 0;
     int variant = 2, seg = 32, blocks_per_cu = 2, graph_steps = 50;  // sweep tiling
     int march_b = 4, fused_bpc = 8;  // fused B planes per thread, fused blocks per CU
+    int pass_seg = 16, pass_rows = 0;  // x-march: planes per block, rows per tile (0: per variant)
+    int prows = 7;                     // x-march rows in use
     int cus = 256;
     int time = 0;
     int parity = 0;  // steps taken since creation, mod 2
     void *d_args[2] = {nullptr, nullptr};
     mpx::BatchRun *d_runs = nullptr;
     int total_tiles = 0, blocks = 0;
-    void *d_fargs[2] = {nullptr, nullptr};           // fused E / B argument tables
+    // fused E / B argument tables [side][arrangement]: the x-march swaps the field sets every
+    // step, and the fused kernels run on whichever set holds the state (the other one, as laid
+    // out by swap_pass_buffers, only under structure 2)
+    void *d_fargs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     mpx::BatchRun *d_fruns[2] = {nullptr, nullptr};  // fused E / B tile tables
     int ftotal[2] = {0, 0}, fblocks[2] = {0, 0};
+    void *d_pargs[2] = {nullptr, nullptr};  // x-march argument tables per arrangement
+    mpx::BatchRun *d_pruns = nullptr;
+    int ptotal = 0;
     mpx::SheetEval *d_sheets = nullptr;
     int *d_sheet_first = nullptr;
     int nsheets = 0, sheet_blocks = 0;
@@ -2732,31 +2744,76 @@ static int batch_prepare_t(mpx_batch *b)
         return ((want < need ? want : need) + 7) / 8 * 8;
     };
     b->blocks = grid(b->blocks_per_cu, total);
-    // fused structure: E side (one plane per thread) and B side (march_b planes per thread)
+    // fused structure: E side (one plane per thread) and B side (march_b planes per thread),
+    // for the arrangement of the field sets as they are (index b->parity) and as the next
+    // x-march pass leaves them (b->parity ^ 1)
     for (int side = 0; side < 2; ++side) {
-        std::vector<StepArgs<T>> fa;
+        std::vector<StepArgs<T>> fa[2];
         std::vector<mpx::BatchRun> fr(n);
         int ft = 0;
         for (int r = 0; r < n; ++r) {
             mpx_solver *s = b->runs[r];
-            StepArgs<T> sa = step_args<T>(s, side == 1);
-            sa.march = side ? b->march_b : 1;
-            fa.push_back(sa);
+            for (int q = 0; q < 2; ++q) {
+                if (q) swap_pass_buffers(s);
+                StepArgs<T> sa = step_args<T>(s, side == 1);
+                sa.march = side ? b->march_b : 1;
+                fa[b->parity ^ q].push_back(sa);
+                if (q) swap_pass_buffers(s);
+            }
             fr[r].first = ft;
-            mpx::fused_tiles(s->g, sa.blo, side == 1, sa.march, fr[r].ntz, fr[r].nty, fr[r].nseg);
+            mpx::fused_tiles(s->g, fa[0].back().blo, side == 1, fa[0].back().march, fr[r].ntz, fr[r].nty,
+                             fr[r].nseg);
             long long t = (long long)fr[r].ntz * fr[r].nty * fr[r].nseg;
             if (ft + t > (1LL << 30)) return fail(MPX_ERR_UNSUPPORTED, "batch has too many tiles");
             ft += (int)t;
         }
         size_t fb = n * sizeof(StepArgs<T>);
-        if (!b->d_fargs[side] && b->alloc(&b->d_fargs[side], fb)) return MPX_ERR_NOMEM;
+        for (int q = 0; q < 2; ++q) {
+            if (!b->d_fargs[side][q] && b->alloc(&b->d_fargs[side][q], fb)) return MPX_ERR_NOMEM;
+            MPX_HIP(hipMemcpyAsync(b->d_fargs[side][q], fa[q].data(), fb, hipMemcpyHostToDevice, b->stream));
+        }
         if (!b->d_fruns[side] && b->alloc((void **)&b->d_fruns[side], n * sizeof(mpx::BatchRun))) return MPX_ERR_NOMEM;
-        MPX_HIP(hipMemcpyAsync(b->d_fargs[side], fa.data(), fb, hipMemcpyHostToDevice, b->stream));
         MPX_HIP(hipMemcpyAsync(b->d_fruns[side], fr.data(), n * sizeof(mpx::BatchRun), hipMemcpyHostToDevice,
                                b->stream));
         MPX_HIP(hipStreamSynchronize(b->stream));
         b->ftotal[side] = ft;
         b->fblocks[side] = grid(b->fused_bpc, ft);
+    }
+    // x-march structure: one block per tile of 64 z cells x prows rows x pass_seg planes
+    {
+        b->prows = mpx::march_batch_rows<T>(b->pass_rows, b->np, b->nonlin, b->cpml);
+        std::vector<mpx::PassArgs<T>> pa[2];
+        std::vector<mpx::BatchRun> pr(n);
+        int pt = 0;
+        for (int r = 0; r < n; ++r) {
+            mpx_solver *s = b->runs[r];
+            for (int q = 0; q < 2; ++q) {
+                if (q) swap_pass_buffers(s);
+                mpx::PassArgs<T> a = pass_args<T>(s);
+                a.kernel = 1;
+                a.seg = b->pass_seg < s->g.n[0] ? b->pass_seg : s->g.n[0];
+                a.rows = b->prows;
+                pa[b->parity ^ q].push_back(a);
+                if (q) swap_pass_buffers(s);
+            }
+            const mpx::PassArgs<T> &a = pa[0].back();
+            pr[r].first = pt;
+            pr[r].ntz = (s->g.n[2] + 63) / 64;
+            pr[r].nty = (s->g.n[1] + b->prows - 1) / b->prows;
+            pr[r].nseg = (s->g.n[0] + a.seg - 1) / a.seg;
+            long long t = (long long)pr[r].ntz * pr[r].nty * pr[r].nseg;
+            if (pt + t > (1LL << 30)) return fail(MPX_ERR_UNSUPPORTED, "batch has too many tiles");
+            pt += (int)t;
+        }
+        size_t pb = n * sizeof(mpx::PassArgs<T>);
+        for (int q = 0; q < 2; ++q) {
+            if (!b->d_pargs[q] && b->alloc(&b->d_pargs[q], pb)) return MPX_ERR_NOMEM;
+            MPX_HIP(hipMemcpyAsync(b->d_pargs[q], pa[q].data(), pb, hipMemcpyHostToDevice, b->stream));
+        }
+        if (!b->d_pruns && b->alloc((void **)&b->d_pruns, n * sizeof(mpx::BatchRun))) return MPX_ERR_NOMEM;
+        MPX_HIP(hipMemcpyAsync(b->d_pruns, pr.data(), n * sizeof(mpx::BatchRun), hipMemcpyHostToDevice, b->stream));
+        MPX_HIP(hipStreamSynchronize(b->stream));
+        b->ptotal = pt;
     }
     return MPX_OK;
 }
@@ -2766,25 +2823,56 @@ static int batch_prepare(mpx_batch *b)
     return b->esz == 8 ? batch_prepare_t<double>(b) : batch_prepare_t<float>(b);
 }
 
-// enqueue step j of a sequence that starts at the device counter, with argument parity par
-static void batch_enqueue(mpx_batch *b, int j, int par)
+// the fused E (side 0) or fused B (side 1) kernel over every run, field arrangement par
+static void batch_fused(mpx_batch *b, int side, int par)
+{
+    const int n = (int)b->runs.size();
+    if (b->esz == 8) {
+        if (side == 0)
+            mpx::launch_fused_E_batch<double>((const StepArgs<double> *)b->d_fargs[0][par], b->d_fruns[0], n,
+                                              b->ftotal[0], b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
+        else
+            mpx::launch_fused_B_batch<double>((const StepArgs<double> *)b->d_fargs[1][par], b->d_fruns[1], n,
+                                              b->ftotal[1], b->fblocks[1], b->march_b, b->cpml, b->stream);
+    } else {
+        if (side == 0)
+            mpx::launch_fused_E_batch<float>((const StepArgs<float> *)b->d_fargs[0][par], b->d_fruns[0], n,
+                                             b->ftotal[0], b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
+        else
+            mpx::launch_fused_B_batch<float>((const StepArgs<float> *)b->d_fargs[1][par], b->d_fruns[1], n,
+                                             b->ftotal[1], b->fblocks[1], b->march_b, b->cpml, b->stream);
+    }
+}
+
+// the sheets of the sides in `sides` (bit 0: E currents, bit 1: H currents) at the counter
+// + j, the H currents hshift steps earlier
+static void batch_sheets(mpx_batch *b, int j, int hshift, int sides)
 {
     if (b->nsheets)
         mpx::launch_sheet_eval_batch(b->d_sheets, b->d_sheet_first, b->nsheets, b->sheet_blocks, b->ft, b->d_time, j,
-                                     b->stream);
+                                     b->stream, hshift, sides);
+}
+
+// enqueue step j of a sequence that starts at the device counter, with argument parity par.
+// Structure 2: the pass at counter + j (the B half of step counter + j - 1, the E half of
+// step counter + j).
+static void batch_enqueue(mpx_batch *b, int j, int par)
+{
     const int n = (int)b->runs.size();
+    if (b->structure == 2) {
+        batch_sheets(b, j, 1, 3);
+        if (b->esz == 8)
+            mpx::launch_march_batch<double>((const mpx::PassArgs<double> *)b->d_pargs[par], b->d_pruns, n, b->ptotal,
+                                            b->prows, b->np, b->nonlin, b->cpml, b->stream);
+        else
+            mpx::launch_march_batch<float>((const mpx::PassArgs<float> *)b->d_pargs[par], b->d_pruns, n, b->ptotal,
+                                           b->prows, b->np, b->nonlin, b->cpml, b->stream);
+        return;
+    }
+    batch_sheets(b, j, 0, 3);
     if (b->structure == 0) {
-        if (b->esz == 8) {
-            mpx::launch_fused_E_batch<double>((const StepArgs<double> *)b->d_fargs[0], b->d_fruns[0], n, b->ftotal[0],
-                                              b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
-            mpx::launch_fused_B_batch<double>((const StepArgs<double> *)b->d_fargs[1], b->d_fruns[1], n, b->ftotal[1],
-                                              b->fblocks[1], b->march_b, b->cpml, b->stream);
-        } else {
-            mpx::launch_fused_E_batch<float>((const StepArgs<float> *)b->d_fargs[0], b->d_fruns[0], n, b->ftotal[0],
-                                             b->fblocks[0], 1, b->np, b->nonlin, b->cpml, b->stream);
-            mpx::launch_fused_B_batch<float>((const StepArgs<float> *)b->d_fargs[1], b->d_fruns[1], n, b->ftotal[1],
-                                             b->fblocks[1], b->march_b, b->cpml, b->stream);
-        }
+        batch_fused(b, 0, par);
+        batch_fused(b, 1, par);
         return;
     }
     if (b->esz == 8)
@@ -2795,11 +2883,13 @@ static void batch_enqueue(mpx_batch *b, int j, int par)
                                        b->total_tiles, b->variant, b->blocks, b->np, b->nonlin, b->cpml, b->stream);
 }
 
+// a graph of graph_steps steps (structure 2: passes at counter + 1 .. counter + graph_steps)
 static int batch_capture(mpx_batch *b, int par)
 {
     hipGraph_t gr = nullptr;
+    const int j0 = b->structure == 2 ? 1 : 0;
     MPX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < b->graph_steps; ++j) batch_enqueue(b, j, b->structure == 1 ? (par + j) & 1 : par);
+    for (int j = 0; j < b->graph_steps; ++j) batch_enqueue(b, j0 + j, b->structure ? (par + j) & 1 : par);
     mpx::launch_advance_time(b->d_time, b->graph_steps, b->stream);
     hipError_t e = hipStreamEndCapture(b->stream, &gr);
     if (e != hipSuccess) return fail(MPX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -2812,30 +2902,49 @@ static int batch_capture(mpx_batch *b, int par)
     return MPX_OK;
 }
 
-// nsteps in lockstep; graph replays for whole groups of graph_steps, direct launches after
+// nsteps in lockstep; graph replays for whole groups of graph_steps, direct launches after.
+// Structure 2 takes the E half of the first step with the fused E kernel, nsteps - 1 passes
+// and the B half of the last step with the fused B kernel, so that E and B belong to the same
+// step between calls.
 static int batch_step(mpx_batch *b, int nsteps, bool use_graph)
 {
+    if (nsteps == 0) return MPX_OK;
+    const bool march = b->structure == 2;
+    if (march) {
+        batch_sheets(b, 0, 0, 1);
+        batch_fused(b, 0, b->parity);
+        MPX_LAUNCHED("batch E half");
+    }
+    const int work = march ? nsteps - 1 : nsteps;
+    const int j0 = march ? 1 : 0;
     int done = 0;
     const int K = b->graph_steps;
     if (use_graph && K > 0)
-        while (nsteps - done >= K) {
+        while (work - done >= K) {
             if (!b->graph[b->parity]) {
                 int rc = batch_capture(b, b->parity);
                 if (rc) return rc;
             }
             MPX_HIP(hipGraphLaunch(b->graph[b->parity], b->stream));
-            if (b->structure == 1) b->parity = (b->parity + K) & 1;
+            if (b->structure) b->parity = (b->parity + K) & 1;
             done += K;
         }
-    for (; done < nsteps; ++done) {
-        batch_enqueue(b, 0, b->parity);
+    for (; done < work; ++done) {
+        batch_enqueue(b, j0, b->parity);
         mpx::launch_advance_time(b->d_time, 1, b->stream);
         MPX_LAUNCHED("batch step");
-        if (b->structure == 1) b->parity ^= 1;
+        if (b->structure) b->parity ^= 1;
+    }
+    if (march) {
+        batch_sheets(b, 0, 0, 2);
+        batch_fused(b, 1, b->parity);
+        mpx::launch_advance_time(b->d_time, 1, b->stream);
+        MPX_LAUNCHED("batch B half");
     }
     for (mpx_solver *s : b->runs) {
         s->time += nsteps;
         if (b->structure == 1 && (nsteps & 1)) swap_sweep_buffers(s);
+        if (march && (work & 1)) swap_pass_buffers(s);
     }
     b->time += nsteps;
     return MPX_OK;
@@ -3002,8 +3111,15 @@ int mpx_batch_set_tuning(mpx_batch *b, const char *key, int value)
 {
     if (!b || !key) return fail(MPX_ERR_INVALID, "null argument");
     if (!std::strcmp(key, "structure")) {
-        if (value < 0 || value > 1) return fail(MPX_ERR_INVALID, "structure must be 0 (fused) or 1 (sweep)");
+        if (value < 0 || value > 2)
+            return fail(MPX_ERR_INVALID, "structure must be 0 (fused), 1 (sweep) or 2 (x-march pass)");
         b->structure = value;
+    } else if (!std::strcmp(key, "pass_seg")) {
+        if (value < 1) return fail(MPX_ERR_INVALID, "pass_seg must be >= 1");
+        b->pass_seg = value;
+    } else if (!std::strcmp(key, "pass_rows")) {
+        if (value != 0 && value != 3 && value != 7) return fail(MPX_ERR_INVALID, "pass_rows must be 0, 3 or 7");
+        b->pass_rows = value;
     } else if (!std::strcmp(key, "march_b")) {
         if (value != 1 && value != 2 && value != 4) return fail(MPX_ERR_INVALID, "march_b must be 1, 2 or 4");
         b->march_b = value;
@@ -3048,11 +3164,13 @@ int mpx_batch_time_steps(mpx_batch *b, int nsteps, double *ms_per_step)
 }
 
 // Autotuner: times every candidate launch shape -- the fused structure with 1, 2 or 4 B planes
-// per thread and 4 or 8 blocks per CU, and the sweep with block shapes 64x16 / 64x8 / 64x4, x
-// segments of 8 / 16 / 32 planes and 1 or 2 blocks per CU -- for `steps_per_trial` steps on the
-// batch itself, keeps the fastest, then clears every run back to its initial all-zero state
-// at time 0.  Allowed only before the first step.  result = {structure, a, b, c}: fused
-// {0, march_b, blocks_per_cu, 0}, sweep {1, variant, seg, blocks_per_cu}.
+// per thread and 4 or 8 blocks per CU, the sweep with block shapes 64x16 / 64x8 / 64x4, x
+// segments of 8 / 16 / 32 planes and 1 or 2 blocks per CU, and the x-march pass with segments
+// of 8 / 16 / 32 planes and tiles of 3 or 7 rows -- for `steps_per_trial` steps on the batch
+// itself, keeps the fastest, then clears every run back to its initial all-zero state at time
+// 0.  Allowed only before the first step.  result = {structure, a, b, c}: fused
+// {0, march_b, blocks_per_cu, 0}, sweep {1, variant, seg, blocks_per_cu}, x-march
+// {2, pass_seg, rows, 0}.
 int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double *best_ms)
 {
     if (!b || steps_per_trial < 1) return fail(MPX_ERR_INVALID, "invalid arguments");
@@ -3066,20 +3184,26 @@ int mpx_batch_autotune(mpx_batch *b, int steps_per_trial, int result[4], double 
     for (int v : {2, 0, 1})
         for (int sg : {8, 16, 32})
             for (int bpc : {1, 2}) cands.push_back({1, v, sg, bpc});
+    for (int sg : {8, 16, 32})
+        for (int rows : {3, 7}) cands.push_back({2, sg, rows, 0});
     auto apply = [&](const Cand &c) {
         b->structure = c.structure;
         if (c.structure == 0) {
             b->march_b = c.p0;
             b->fused_bpc = c.p1;
-        } else {
+        } else if (c.structure == 1) {
             b->variant = c.p0;
             b->seg = c.p1;
             b->blocks_per_cu = c.p2;
+        } else {
+            b->pass_seg = c.p0;
+            b->pass_rows = c.p1;
         }
     };
     double best = 1e300;
-    Cand bc = {b->structure, b->structure ? b->variant : b->march_b, b->structure ? b->seg : b->fused_bpc,
-               b->structure ? b->blocks_per_cu : 0};
+    Cand bc = b->structure == 0   ? Cand{0, b->march_b, b->fused_bpc, 0}
+              : b->structure == 1 ? Cand{1, b->variant, b->seg, b->blocks_per_cu}
+                                  : Cand{2, b->pass_seg, b->pass_rows, 0};
     hipEvent_t e0, e1;
     MPX_HIP(hipEventCreate(&e0));
     MPX_HIP(hipEventCreate(&e1));

@@ -37,9 +37,13 @@ def singles(descs, steps, path=abi.PATH_SWEEP):
 
 
 # (structure, tuning, graph steps): the fused structure is compared with the fused path run
-# on its own, the sweep structure with the sweep path
+# on its own, the sweep structure with the sweep path, the x-march structure with the pass path
 CASES = [(0, {"march_b": 4, "fused_blocks_per_cu": 8}, 7), (0, {"march_b": 1, "fused_blocks_per_cu": 1}, 0),
-         (1, {"variant": 2, "seg": 32}, 7), (1, {"variant": 0, "seg": 5}, 4), (1, {"variant": 1, "seg": 16}, 0)]
+         (1, {"variant": 2, "seg": 32}, 7), (1, {"variant": 0, "seg": 5}, 4), (1, {"variant": 1, "seg": 16}, 0),
+         (2, {"pass_seg": 16, "pass_rows": 7}, 7), (2, {"pass_seg": 5, "pass_rows": 3}, 4),
+         (2, {"pass_seg": 32, "pass_rows": 0}, 0)]
+SINGLE_PATH = {0: abi.PATH_FUSED, 1: abi.PATH_SWEEP, 2: abi.PATH_PASS}
+STRUCTURE = {"fused": 0, "sweep": 1, "pass": 2}
 
 
 @pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_BULK])
@@ -48,13 +52,14 @@ def test_batch_matches_single_runs(prec, structure, tuning, graph):
     descs = small_runs(prec)
     steps = 45
     print(f"case: batch prec {prec} structure {structure} {tuning} graph {graph}", flush=True)
-    want = singles(descs, steps, abi.PATH_FUSED if structure == 0 else abi.PATH_SWEEP)
+    want = singles(descs, steps, SINGLE_PATH[structure])
     with abi.Batch(descs) as b:
         b.tune("structure", structure)
         for k, v in tuning.items():
             b.tune(k, v)
         b.tune("graph_steps", graph)
-        b.step(20)
+        b.step(1)  # the x-march structure: a call of one step takes the two fused halves only
+        b.step(19)
         b.step(steps - 20)
         assert b.time == steps
         for r in range(len(descs)):
@@ -68,7 +73,7 @@ def test_batch_matches_single_runs(prec, structure, tuning, graph):
 def test_batch_run_against_oracle():
     descs = small_runs(abi.PREC_FP64)
     steps = 40
-    for structure in (0, 1):
+    for structure in (0, 1, 2):
         with abi.Batch(descs) as b:
             b.tune("structure", structure)
             b.tune("graph_steps", 10)
@@ -88,8 +93,8 @@ def test_batch_autotune_restores_initial_state():
     with abi.Batch(descs) as b:
         best = b.autotune(steps_per_trial=2)
         print("autotune", best)
-        assert best["structure"] in ("fused", "sweep")
-        want = singles(descs, 30, abi.PATH_FUSED if best["structure"] == "fused" else abi.PATH_SWEEP)
+        assert best["structure"] in STRUCTURE
+        want = singles(descs, 30, SINGLE_PATH[STRUCTURE[best["structure"]]])
         assert b.time == 0
         assert not np.any(b.run(0).get("Ex"))
         b.step(30)
