@@ -38,6 +38,31 @@ __host__ __device__ __forceinline__ size_t psi_at(size_t c)
 }
 // double2 elements of one pole's block over `cells` storage cells
 inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
+// Accumulators in the float32 + residual form (MPX_PREC_FP32_RES): a pole's block keeps the
+// float64 layout's chunks (3072 bytes per 64 cells), of which the first 2048 bytes are used.
+// Cell l of a chunk holds the high-word pair (re, im) of component q at byte 512 q + 8 l and
+// its two residual words at byte 1536 + 8 l: word 0 has the slots of x.re, x.im, y.re, word 1
+// those of y.im, z.re, z.im.  psr_byte(psi_at(c)) is the byte offset of cell c's component-0
+// pair from the pole's base; the other words sit 512, 1024 and 1536 bytes on.
+__host__ __device__ __forceinline__ size_t psr_byte(size_t pq) { return 16 * pq - 8 * (pq & 63); }
+// An accumulator value x is held as the upper 32 bits of the float64 x rounded to 30 mantissa
+// bits (half away from zero, by adding 2^21 to its bit pattern; a carry out of the mantissa
+// is the rounding up to the next binade) plus the next 10 mantissa bits in slot q of a
+// residual word: relative error <= 2^-31.  Infinite and NaN values are kept as they are
+// (upper word and slot cut, no rounding).  Both run on the host (get/set of the grids) and
+// on the device, in integer operations only.
+__host__ __device__ __forceinline__ unsigned psr_encode(double x, unsigned &w, int q)
+{
+    unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    if ((b & 0x7ff0000000000000ull) != 0x7ff0000000000000ull) b += 1ull << 21;
+    w |= (unsigned)((b >> 22) & 0x3ffu) << (10 * q);
+    return (unsigned)(b >> 32);
+}
+__host__ __device__ __forceinline__ double psr_decode(unsigned hi, unsigned w, int q)
+{
+    const unsigned long long b = ((unsigned long long)hi << 32) | ((unsigned long long)((w >> (10 * q)) & 0x3ffu) << 22);
+    return __builtin_bit_cast(double, b);
+}
 
 // Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word
 // hi = fl32(x) plus a 10-bit two's-complement residual q = rint((x - hi) 2^(160 - e)) in
@@ -46,10 +71,6 @@ inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >
 // components of one cell (E or B: three, a CPML face: its two auxiliaries) share a 32-bit
 // word, slot q in bits 10q .. 10q + 9.  Both helpers run on the host (get/set of fields) and
 // on the device and give the same bits.
-__host__ __device__ __forceinline__ double res_pow2(int e)
-{
-    return __builtin_bit_cast(double, (unsigned long long)(e + 1023) << 52);  // 2^e, normal range
-}
 __host__ __device__ __forceinline__ int res_exp(float hi)
 {
     const int e = (int)((__builtin_bit_cast(unsigned, hi) >> 23) & 0xffu);
@@ -58,18 +79,18 @@ __host__ __device__ __forceinline__ int res_exp(float hi)
 __host__ __device__ __forceinline__ double res_decode(float hi, unsigned w, int q)
 {
     const int r = (int)(w << (22 - 10 * q)) >> 22;  // sign-extended slot q
-    return (double)hi + (double)r * res_pow2(res_exp(hi) - 160);
+    return (double)hi + ldexp((double)r, res_exp(hi) - 160);  // r 2^(e - 160), exact
 }
 // the float32 word of x; its residual is OR-ed into slot q of w (w starts at zero)
 __host__ __device__ __forceinline__ float res_encode(double x, unsigned &w, int q)
 {
     const float hi = (float)x;
     const int e = res_exp(hi);
-    if (e < 255) {  // finite word (an infinite or NaN word keeps q = 0)
-        double r = rint((x - (double)hi) * res_pow2(160 - e));
-        r = r > 511.0 ? 511.0 : (r < -512.0 ? -512.0 : r);
-        w |= ((unsigned)(int)r & 0x3ffu) << (10 * q);
-    }
+    // an infinite or NaN word keeps q = 0; |rint(.)| <= 512 for a finite one
+    const double t = e < 255 ? rint(ldexp(x - (double)hi, 160 - e)) : 0.0;
+    int r = (int)t;
+    r = r > 511 ? 511 : (r < -512 ? -512 : r);
+    w |= ((unsigned)r & 0x3ffu) << (10 * q);
     return hi;
 }
 
