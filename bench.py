@@ -6,9 +6,8 @@ Yee step (E side then B side) of the whole grid; value = cell-updates/s = Nx*Ny*
 with all state resident in HBM.  The timed steps start after a pre-roll of 880 steps, when the
 pulse has entered the grid (field everywhere near the source plane, Kerr solves iterating).
 
-Headline form: FP32-res -- E, B and the CPML auxiliaries as float32 words each backed by a
-10-bit residual slot, the PLRC accumulators as 32-bit high words of their float64 values with
-a 10-bit residual slot (30 mantissa bits), all arithmetic (the Kerr solve included) in
+Headline form: FP32-res -- E, B, the CPML auxiliaries and the PLRC accumulators as float32
+words each backed by a 10-bit residual slot, all arithmetic (the Kerr solve included) in
 float64 -- the float32-storage form that meets the per-field accuracy bar against float64
 (plain float32 words do not: their cross-polarised components sit at the float32 storage
 floor; FP32-bulk keeps float64 accumulators).  The faster of its fused and single-pass
@@ -39,8 +38,8 @@ def step_bytes(prec, n, cpml_d, n_poles):
     """Algorithmic HBM bytes of one step: every mutable state word read once and written once,
     neighbour re-reads free, static tables ignored.  Per interior cell 2 (6 w_f + r + P a),
     w_f = 8 (float64) or 4 (float32 forms), r = 8 for the two residual words of FP32-res (E, B),
-    a = 48 bytes of accumulators per pole (six float64 values; FP32-res: six 32-bit high words
-    and two residual words, 32); per CPML face-layer cell (2 d (NxNy + NyNz + NxNz) of them) 8 w_c,
+    a = 48 bytes of accumulators per pole (six float64 values; FP32-res: six float32 words and
+    two residual words, 32); per CPML face-layer cell (2 d (NxNy + NyNz + NxNz) of them) 8 w_c,
     plus 16 for the residual words of the E- and H-side auxiliaries in FP32-res.  Returns
     (total, E side, B side): the E side is E, the accumulators and the E auxiliaries, the B
     side B and the H auxiliaries."""
@@ -466,8 +465,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32-res (f32 E/B/CPML words + 10-bit residual slots; PLRC psi as 32-bit high words + "
-                  "10-bit residual slots, 30-bit mantissa; f64 arithmetic and Kerr solve)" if prec == 2
+        "dtype": ("fp32-res (f32 E/B/CPML/PLRC-psi words + 10-bit residual slots; f64 arithmetic and Kerr "
+                  "solve)" if prec == 2
                   else "fp32-bulk (f32 E/B/CPML, f64 psi and Kerr, f64 arithmetic)"),
         "data": "synthetic (zero initial fields, analytic short-pulse source; timed after a pre-roll of "
                 f"{args.preroll} steps with the pulse in the grid)",

@@ -61,13 +61,14 @@ __device__ __forceinline__ void store_once(T *p, T v)
 // Accumulators of a pole in the float32 + residual form (psr_byte, psr_encode): the three
 // components of cell pq decoded from, or encoded into, the block at `base` (the pole's
 // component-0 pointer)
+typedef float float2v __attribute__((ext_vector_type(2)));
 typedef unsigned uint2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void psi_load_res(const double2 *base, size_t pq, double2 v[3])
 {
     const char *p = (const char *)base + psr_byte(pq);
-    const uint2v h0 = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)p);
-    const uint2v h1 = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)(p + 512));
-    const uint2v h2 = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)(p + 1024));
+    const float2v h0 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)p);
+    const float2v h1 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 512));
+    const float2v h2 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 1024));
     const uint2v w = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)(p + 1536));
     v[0] = make_double2(psr_decode(h0.x, w.x, 0), psr_decode(h0.y, w.x, 1));
     v[1] = make_double2(psr_decode(h1.x, w.x, 2), psr_decode(h1.y, w.y, 0));
@@ -76,7 +77,8 @@ __device__ __forceinline__ void psi_load_res(const double2 *base, size_t pq, dou
 __device__ __forceinline__ void psi_store_res(double2 *base, size_t pq, double2 x, double2 y, double2 z)
 {
     unsigned w0 = 0, w1 = 0;
-    uint2v h0, h1, h2, w;
+    float2v h0, h1, h2;
+    uint2v w;
     h0.x = psr_encode(x.x, w0, 0);
     h0.y = psr_encode(x.y, w0, 1);
     h1.x = psr_encode(y.x, w0, 2);
@@ -86,9 +88,9 @@ __device__ __forceinline__ void psi_store_res(double2 *base, size_t pq, double2 
     w.x = w0;
     w.y = w1;
     char *p = (char *)base + psr_byte(pq);
-    __builtin_nontemporal_store(h0, (MPX_GLOBAL uint2v *)p);
-    __builtin_nontemporal_store(h1, (MPX_GLOBAL uint2v *)(p + 512));
-    __builtin_nontemporal_store(h2, (MPX_GLOBAL uint2v *)(p + 1024));
+    __builtin_nontemporal_store(h0, (MPX_GLOBAL float2v *)p);
+    __builtin_nontemporal_store(h1, (MPX_GLOBAL float2v *)(p + 512));
+    __builtin_nontemporal_store(h2, (MPX_GLOBAL float2v *)(p + 1024));
     __builtin_nontemporal_store(w, (MPX_GLOBAL uint2v *)(p + 1536));
 }
 // the pole bases of float32 + residual accumulators, for the E update's loads and stores

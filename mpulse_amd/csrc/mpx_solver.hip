@@ -2057,7 +2057,8 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
                 size_t dst = (i * ny2 + j) * nz2 + k;
                 if (rpsi) {
                     const unsigned char *q = buf.data() + mpx::psr_byte(mpx::psi_at(src));
-                    unsigned hi, w;
+                    float hi;
+                    unsigned w;
                     std::memcpy(&hi, q + 512 * comp + 4 * part, 4);
                     std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
                     host[dst] = mpx::psr_decode(hi, w, s6 % 3);
@@ -2211,7 +2212,7 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
                     unsigned char *q = buf.data() + mpx::psr_byte(mpx::psi_at(dst));
                     unsigned w, nw = 0;
                     std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
-                    const unsigned hi = mpx::psr_encode(host[src], nw, s6 % 3);
+                    const float hi = mpx::psr_encode(host[src], nw, s6 % 3);
                     w = (w & ~(0x3ffu << (10 * (s6 % 3)))) | nw;
                     std::memcpy(q + 512 * comp + 4 * part, &hi, 4);
                     std::memcpy(q + 1536 + 4 * (s6 / 3), &w, 4);

@@ -40,28 +40,36 @@ __host__ __device__ __forceinline__ size_t psi_at(size_t c)
 inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
 // Accumulators in the float32 + residual form (MPX_PREC_FP32_RES): a pole's block keeps the
 // float64 layout's chunks (3072 bytes per 64 cells), of which the first 2048 bytes are used.
-// Cell l of a chunk holds the high-word pair (re, im) of component q at byte 512 q + 8 l and
+// Cell l of a chunk holds the float32 pair (re, im) of component q at byte 512 q + 8 l and
 // its two residual words at byte 1536 + 8 l: word 0 has the slots of x.re, x.im, y.re, word 1
 // those of y.im, z.re, z.im.  psr_byte(psi_at(c)) is the byte offset of cell c's component-0
 // pair from the pole's base; the other words sit 512, 1024 and 1536 bytes on.
 __host__ __device__ __forceinline__ size_t psr_byte(size_t pq) { return 16 * pq - 8 * (pq & 63); }
-// An accumulator value x is held as the upper 32 bits of the float64 x rounded to 30 mantissa
-// bits (half away from zero, by adding 2^21 to its bit pattern; a carry out of the mantissa
-// is the rounding up to the next binade) plus the next 10 mantissa bits in slot q of a
-// residual word: relative error <= 2^-31.  Infinite and NaN values are kept as they are
-// (upper word and slot cut, no rounding).  Both run on the host (get/set of the grids) and
-// on the device, in integer operations only.
-__host__ __device__ __forceinline__ unsigned psr_encode(double x, unsigned &w, int q)
+// An accumulator value x is held as hi = fl32(x) plus, in slot q of a residual word, the
+// 10-bit two's-complement r = round((bits(x) - bits(hi)) / 2^19), the difference of the
+// float64 bit patterns of x and of hi widened to float64 (below 2^29 in magnitude: the low
+// words suffice), clamped to [-512, 511]; decoding adds r 2^19 to the bit pattern of hi.  In
+// the binade of hi a unit of r is 2^-10 ulp(hi), so the stored value is within 2^-10 ulp(hi)
+// of x, the clamp included (2^-33 relative); both sides work on the magnitude bits, so the
+// sign and a hi rounded up across a power of two need no special case.  Zero, denormal, infinite and NaN words keep
+// r = 0.  Integer operations only; runs on the host (get/set of the grids) and the device.
+__host__ __device__ __forceinline__ float psr_encode(double x, unsigned &w, int q)
 {
-    unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    if ((b & 0x7ff0000000000000ull) != 0x7ff0000000000000ull) b += 1ull << 21;
-    w |= (unsigned)((b >> 22) & 0x3ffu) << (10 * q);
-    return (unsigned)(b >> 32);
+    const float hi = (float)x;
+    const unsigned e = (__builtin_bit_cast(unsigned, hi) >> 23) & 0xffu;
+    const unsigned lx = (unsigned)__builtin_bit_cast(unsigned long long, x);
+    const unsigned lh = (unsigned)__builtin_bit_cast(unsigned long long, (double)hi);
+    int r = ((int)(lx - lh) + (1 << 18)) >> 19;
+    r = r > 511 ? 511 : (r < -512 ? -512 : r);
+    r = (e == 0u || e == 255u) ? 0 : r;
+    w |= ((unsigned)r & 0x3ffu) << (10 * q);
+    return hi;
 }
-__host__ __device__ __forceinline__ double psr_decode(unsigned hi, unsigned w, int q)
+__host__ __device__ __forceinline__ double psr_decode(float hi, unsigned w, int q)
 {
-    const unsigned long long b = ((unsigned long long)hi << 32) | ((unsigned long long)((w >> (10 * q)) & 0x3ffu) << 22);
-    return __builtin_bit_cast(double, b);
+    const int r = (int)(w << (22 - 10 * q)) >> 22;  // sign-extended slot q
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, (double)hi) +
+                                          ((unsigned long long)(long long)r << 19));
 }
 
 // Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word

@@ -144,18 +144,21 @@ def test_fp32_res_field_access():
         assert pr[-1, 0] == E["Ey"][tuple(c[0])] and pr[-1, 2] == E["By"][tuple(c[2])]
         lay, lo, hi = s.get_layer("CPMLPsiEx", abi.NORTH)
         assert np.any(lay) and np.all(np.isfinite(lay))
-        # the accumulators: a 32-bit word and a residual slot each, the float64 value rounded to
-        # 30 mantissa bits (half away from zero); a set value reads back so rounded and leaves
-        # the other five values of the cells alone
+        # the accumulators: a float32 word and a residual slot each (the residual in units of
+        # 2^19 of the float64 bit pattern); a set value reads back as so stored and leaves the
+        # other five values of the cells alone
         names = [f"Psi{p}1{c}" for p in "RI" for c in "xyz"]
         before = {q: s.get(q) for q in names}
         assert any(np.any(v) for v in before.values())
         y = rng.standard_normal(shape) * 1e2
         s.set("PsiI1y", y)
-        want = ((y.view(np.uint64) + np.uint64(1 << 21)) & ~np.uint64((1 << 22) - 1)).view(np.float64)
+        hd = y.astype(np.float32).astype(np.float64)
+        d = ((y.view(np.uint64) - hd.view(np.uint64)) & np.uint64(0xffffffff)).astype(np.uint32).view(np.int32)
+        r = np.clip((d.astype(np.int64) + (1 << 18)) >> 19, -512, 511)
+        want = (hd.view(np.uint64) + (r << 19).astype(np.uint64)).view(np.float64)
         got = s.get("PsiI1y")
         assert np.array_equal(got, want)
-        assert np.max(np.abs(got - y) / np.abs(y)) <= 2.0 ** -31
+        assert np.max(np.abs(got - y) / np.abs(y)) <= 2.0 ** -32
         for q in names:
             if q != "PsiI1y":
                 assert np.array_equal(s.get(q), before[q]), q
