@@ -58,7 +58,7 @@ __device__ __forceinline__ void store_once(T *p, T v)
     __builtin_nontemporal_store(v, (MPX_GLOBAL T *)p);
 }
 
-// Accumulators of a pole in the float32 + residual form (psr_byte, psr_encode): the three
+// Accumulators of a pole in the float32 + residual form (psr_byte, res_encode): the three
 // components of cell pq decoded from, or encoded into, the block at `base` (the pole's
 // component-0 pointer)
 typedef float float2v __attribute__((ext_vector_type(2)));
@@ -70,21 +70,21 @@ __device__ __forceinline__ void psi_load_res(const double2 *base, size_t pq, dou
     const float2v h1 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 512));
     const float2v h2 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 1024));
     const uint2v w = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)(p + 1536));
-    v[0] = make_double2(psr_decode(h0.x, w.x, 0), psr_decode(h0.y, w.x, 1));
-    v[1] = make_double2(psr_decode(h1.x, w.x, 2), psr_decode(h1.y, w.y, 0));
-    v[2] = make_double2(psr_decode(h2.x, w.y, 1), psr_decode(h2.y, w.y, 2));
+    v[0] = make_double2(res_decode(h0.x, w.x, 0), res_decode(h0.y, w.x, 1));
+    v[1] = make_double2(res_decode(h1.x, w.x, 2), res_decode(h1.y, w.y, 0));
+    v[2] = make_double2(res_decode(h2.x, w.y, 1), res_decode(h2.y, w.y, 2));
 }
 __device__ __forceinline__ void psi_store_res(double2 *base, size_t pq, double2 x, double2 y, double2 z)
 {
     unsigned w0 = 0, w1 = 0;
     float2v h0, h1, h2;
     uint2v w;
-    h0.x = psr_encode(x.x, w0, 0);
-    h0.y = psr_encode(x.y, w0, 1);
-    h1.x = psr_encode(y.x, w0, 2);
-    h1.y = psr_encode(y.y, w1, 0);
-    h2.x = psr_encode(z.x, w1, 1);
-    h2.y = psr_encode(z.y, w1, 2);
+    h0.x = res_encode(x.x, w0, 0);
+    h0.y = res_encode(x.y, w0, 1);
+    h1.x = res_encode(y.x, w0, 2);
+    h1.y = res_encode(y.y, w1, 0);
+    h2.x = res_encode(z.x, w1, 1);
+    h2.y = res_encode(z.y, w1, 2);
     w.x = w0;
     w.y = w1;
     char *p = (char *)base + psr_byte(pq);

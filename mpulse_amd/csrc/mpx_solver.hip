@@ -2061,7 +2061,7 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
                     unsigned w;
                     std::memcpy(&hi, q + 512 * comp + 4 * part, 4);
                     std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
-                    host[dst] = mpx::psr_decode(hi, w, s6 % 3);
+                    host[dst] = mpx::res_decode(hi, w, s6 % 3);
                 } else if (isPsi)
                     host[dst] = ((const double *)buf.data())[2 * mpx::psi_at(src) + part];
                 else if (elem == 8)
@@ -2212,7 +2212,7 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
                     unsigned char *q = buf.data() + mpx::psr_byte(mpx::psi_at(dst));
                     unsigned w, nw = 0;
                     std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
-                    const float hi = mpx::psr_encode(host[src], nw, s6 % 3);
+                    const float hi = mpx::res_encode(host[src], nw, s6 % 3);
                     w = (w & ~(0x3ffu << (10 * (s6 % 3)))) | nw;
                     std::memcpy(q + 512 * comp + 4 * part, &hi, 4);
                     std::memcpy(q + 1536 + 4 * (s6 / 3), &w, 4);

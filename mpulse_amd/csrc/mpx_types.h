@@ -38,22 +38,18 @@ __host__ __device__ __forceinline__ size_t psi_at(size_t c)
 }
 // double2 elements of one pole's block over `cells` storage cells
 inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
-// Accumulators in the float32 + residual form (MPX_PREC_FP32_RES): a pole's block keeps the
-// float64 layout's chunks (3072 bytes per 64 cells), of which the first 2048 bytes are used.
-// Cell l of a chunk holds the float32 pair (re, im) of component q at byte 512 q + 8 l and
-// its two residual words at byte 1536 + 8 l: word 0 has the slots of x.re, x.im, y.re, word 1
-// those of y.im, z.re, z.im.  psr_byte(psi_at(c)) is the byte offset of cell c's component-0
-// pair from the pole's base; the other words sit 512, 1024 and 1536 bytes on.
-__host__ __device__ __forceinline__ size_t psr_byte(size_t pq) { return 16 * pq - 8 * (pq & 63); }
-// An accumulator value x is held as hi = fl32(x) plus, in slot q of a residual word, the
-// 10-bit two's-complement r = round((bits(x) - bits(hi)) / 2^19), the difference of the
-// float64 bit patterns of x and of hi widened to float64 (below 2^29 in magnitude: the low
-// words suffice), clamped to [-512, 511]; decoding adds r 2^19 to the bit pattern of hi.  In
-// the binade of hi a unit of r is 2^-10 ulp(hi), so the stored value is within 2^-10 ulp(hi)
-// of x, the clamp included (2^-33 relative); both sides work on the magnitude bits, so the
-// sign and a hi rounded up across a power of two need no special case.  Zero, denormal, infinite and NaN words keep
-// r = 0.  Integer operations only; runs on the host (get/set of the grids) and the device.
-__host__ __device__ __forceinline__ float psr_encode(double x, unsigned &w, int q)
+// Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word
+// hi = fl32(x) plus, in a 10-bit slot of a residual word, the two's-complement
+// r = round((bits(x) - bits(hi)) / 2^19): the difference of the float64 bit patterns of x and
+// of hi widened to float64 (below 2^29 in magnitude, so the low words suffice), clamped to
+// [-512, 511]; decoding adds r 2^19 to the bit pattern of hi.  In the binade of hi a unit of r
+// is 2^-10 ulp(hi), so the stored value is within 2^-10 ulp(hi) of x, the clamp included
+// (2^-33 relative); both sides work on the magnitude bits, so the sign and a hi rounded up
+// across a power of two need no special case.  Zero, denormal, infinite and NaN words keep
+// r = 0.  The residuals of the components of one cell (E or B: three, a CPML face: its two
+// auxiliaries) share a 32-bit word, slot q in bits 10q .. 10q + 9.  Integer operations only;
+// both helpers run on the host (get/set of fields) and on the device and give the same bits.
+__host__ __device__ __forceinline__ float res_encode(double x, unsigned &w, int q)
 {
     const float hi = (float)x;
     const unsigned e = (__builtin_bit_cast(unsigned, hi) >> 23) & 0xffu;
@@ -65,42 +61,19 @@ __host__ __device__ __forceinline__ float psr_encode(double x, unsigned &w, int 
     w |= ((unsigned)r & 0x3ffu) << (10 * q);
     return hi;
 }
-__host__ __device__ __forceinline__ double psr_decode(float hi, unsigned w, int q)
+__host__ __device__ __forceinline__ double res_decode(float hi, unsigned w, int q)
 {
     const int r = (int)(w << (22 - 10 * q)) >> 22;  // sign-extended slot q
     return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, (double)hi) +
                                           ((unsigned long long)(long long)r << 19));
 }
-
-// Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word
-// hi = fl32(x) plus a 10-bit two's-complement residual q = rint((x - hi) 2^(160 - e)) in
-// units of 2^-10 of hi's ulp (e = biased exponent of hi, at least 1), clamped to [-512, 511];
-// the stored value hi + q 2^(e - 160) is within 2^-10 ulp(hi) of x.  The residuals of the
-// components of one cell (E or B: three, a CPML face: its two auxiliaries) share a 32-bit
-// word, slot q in bits 10q .. 10q + 9.  Both helpers run on the host (get/set of fields) and
-// on the device and give the same bits.
-__host__ __device__ __forceinline__ int res_exp(float hi)
-{
-    const int e = (int)((__builtin_bit_cast(unsigned, hi) >> 23) & 0xffu);
-    return e < 1 ? 1 : e;
-}
-__host__ __device__ __forceinline__ double res_decode(float hi, unsigned w, int q)
-{
-    const int r = (int)(w << (22 - 10 * q)) >> 22;  // sign-extended slot q
-    return (double)hi + ldexp((double)r, res_exp(hi) - 160);  // r 2^(e - 160), exact
-}
-// the float32 word of x; its residual is OR-ed into slot q of w (w starts at zero)
-__host__ __device__ __forceinline__ float res_encode(double x, unsigned &w, int q)
-{
-    const float hi = (float)x;
-    const int e = res_exp(hi);
-    // an infinite or NaN word keeps q = 0; |rint(.)| <= 512 for a finite one
-    const double t = e < 255 ? rint(ldexp(x - (double)hi, 160 - e)) : 0.0;
-    int r = (int)t;
-    r = r > 511 ? 511 : (r < -512 ? -512 : r);
-    w |= ((unsigned)r & 0x3ffu) << (10 * q);
-    return hi;
-}
+// The accumulators of a pole in this form keep the float64 layout's chunks (3072 bytes per
+// 64 cells), of which the first 2048 bytes are used: cell l of a chunk holds the float32 pair
+// (re, im) of component q at byte 512 q + 8 l and its two residual words at byte 1536 + 8 l
+// (word 0: the slots of x.re, x.im, y.re; word 1: y.im, z.re, z.im).  psr_byte(psi_at(c)) is
+// the byte offset of cell c's component-0 pair from the pole's base; the other words sit 512,
+// 1024 and 1536 bytes on.
+__host__ __device__ __forceinline__ size_t psr_byte(size_t pq) { return 16 * pq - 8 * (pq & 63); }
 
 // One CPML border current (a face of CPMLBorder{E,H}Current): its layer box, its two live
 // auxiliaries, and the b/c coefficient lines indexed by the normal coordinate.

@@ -80,20 +80,25 @@ static void grid_fill(grid_t *g, double val)
 
 /* Storage rounding (not part of the reference): the value a field word holds after it is
    stored in a narrower form.  Mode 1: float32 (round to nearest even).  Mode 2: float32 plus a
-   10-bit two's-complement residual q in units of 2^-10 of the float32 word's ulp,
-   x ~ hi + q 2^(max(e,1) - 160) for the biased exponent e of hi, |q| <= 511.  The checker uses
-   it to show what a storage form costs in accuracy against float64 storage. */
+   10-bit two's-complement residual r = round((bits(x) - bits(hi)) / 2^19), the difference of
+   the float64 bit patterns of x and of hi = fl32(x), clamped to [-512, 511] and added back to
+   the bit pattern of hi (in the binade of hi a unit of r is 2^-10 of the float32 word's ulp);
+   zero and denormal words keep no residual.  The checker uses it to show what a storage form
+   costs in accuracy against float64 storage. */
 static double store_round(double x, int mode)
 {
     float hi = (float)x;
     if (mode == 1 || !isfinite(hi)) return (double)hi;
     union { float f; unsigned u; } b = {hi};
-    int e = (int)((b.u >> 23) & 0xffu);
-    if (e < 1) e = 1;
-    double q = rint(ldexp(x - (double)hi, 160 - e));
-    if (q > 511.0) q = 511.0;
-    if (q < -512.0) q = -512.0;
-    return (double)hi + ldexp(q, e - 160);
+    if (((b.u >> 23) & 0xffu) == 0) return (double)hi;
+    union { double d; unsigned long long u; } X = {x}, H = {(double)hi};
+    long long d = (long long)X.u - (long long)H.u; /* same sign bit: the magnitude difference */
+    long long r = (d + (1LL << 18)) / (1LL << 19);
+    if (d + (1LL << 18) < 0 && (d + (1LL << 18)) % (1LL << 19)) r -= 1; /* floor division */
+    if (r > 511) r = 511;
+    if (r < -512) r = -512;
+    H.u += (unsigned long long)(r * (1LL << 19));
+    return H.d;
 }
 
 static void grid_round(grid_t *g, int mode)

@@ -30,9 +30,10 @@ def test_store_round_float32_is_the_float32_cast():
 
 def test_store_round_residual_error_bound():
     """float32 + residual: the float32 word's rounding error is at most ulp/2 = 512 residual
-    units; q is that error rounded to the nearest unit (|x - stored| <= 2^-11 ulp) and clamped
-    to 511 when it rounds to +512 (then |x - stored| <= 2^-10 ulp), so the stored value is at
-    least 512x closer than the float32 word alone."""
+    units (2^19 steps of the float64 bit pattern each); r is that error rounded to a unit
+    (|x - stored| <= 2^-11 ulp) and clamped to 511 when it rounds to +512 (then |x - stored| <=
+    2^-10 ulp), so a normal word's stored value is at least 512x closer than the float32 word
+    alone; zero and denormal words keep no residual (|x - stored| <= ulp/2)."""
     rng = np.random.default_rng(6)
     xs = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-20, 20, 2000),
                          rng.uniform(-1, 1, 2000),
@@ -43,13 +44,16 @@ def test_store_round_residual_error_bound():
     worst32, worst = 0.0, 0.0
     for x in xs:
         hi = np.float32(x)
-        e = max(int((hi.view(np.uint32) >> 23) & 0xFF), 1)
+        e0 = int((hi.view(np.uint32) >> 23) & 0xFF)
+        e = max(e0, 1)
         ulp = 2.0 ** (e - 150)
         r = store_round(x, 2)
+        if e0 == 0:
+            assert r == float(hi), (x, r)
+            continue
         assert abs(r - x) <= ulp * 2.0 ** -10 * (1 + 1e-12), (x, r)
-        if x != 0:
-            worst32 = max(worst32, abs(float(hi) - x) / abs(x))
-            worst = max(worst, abs(r - x) / abs(x))
+        worst32 = max(worst32, abs(float(hi) - x) / abs(x))
+        worst = max(worst, abs(r - x) / abs(x))
     assert worst * 512 < worst32
 
 
