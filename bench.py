@@ -6,12 +6,11 @@ Yee step (E side then B side) of the whole grid; value = cell-updates/s = Nx*Ny*
 with all state resident in HBM.  The timed steps start after a pre-roll of 880 steps, when the
 pulse has entered the grid (field everywhere near the source plane, Kerr solves iterating).
 
-Headline form: FP32-res -- E, B, the CPML auxiliaries and the PLRC accumulators as float32
-words each backed by a 10-bit residual slot, all arithmetic (the Kerr solve included) in
+Headline form: FP32-res -- E, B and the CPML auxiliaries as float32 words each backed by a
+10-bit residual slot, the PLRC accumulators and all arithmetic (the Kerr solve included) in
 float64 -- the float32-storage form that meets the per-field accuracy bar against float64
 (plain float32 words do not: their cross-polarised components sit at the float32 storage
-floor; FP32-bulk keeps float64 accumulators).  The faster of its fused and single-pass
-structures is the headline line.  FP32-bulk (plain float32), the
+floor).  The faster of its fused and single-pass structures is the headline line.  FP32-bulk (plain float32), the
 float64 fused, single-pass and reference-structured forms are timed beside it; the CPU oracle
 (OpenMP) is timed on a bounded sample of the same workload, and on one core on a 192^3 sample.
 The 64-run batched parameter sweep (see batch_sweep), the adjoint-gradient workload (see
@@ -36,19 +35,16 @@ HBM_PEAK_GBS = 8000.0
 
 def step_bytes(prec, n, cpml_d, n_poles):
     """Algorithmic HBM bytes of one step: every mutable state word read once and written once,
-    neighbour re-reads free, static tables ignored.  Per interior cell 2 (6 w_f + r + P a),
-    w_f = 8 (float64) or 4 (float32 forms), r = 8 for the two residual words of FP32-res (E, B),
-    a = 48 bytes of accumulators per pole (six float64 values; FP32-res: six float32 words and
-    two residual words, 32); per CPML face-layer cell (2 d (NxNy + NyNz + NxNz) of them) 8 w_c,
-    plus 16 for the residual words of the E- and H-side auxiliaries in FP32-res.  Returns
-    (total, E side, B side): the E side is E, the accumulators and the E auxiliaries, the B
-    side B and the H auxiliaries."""
+    neighbour re-reads free, static tables ignored.  Per interior cell 2 (6 w_f + r + 6 P 8),
+    w_f = 8 (float64) or 4 (float32 forms), r = 8 for the two residual words of FP32-res (E, B);
+    per CPML face-layer cell (2 d (NxNy + NyNz + NxNz) of them) 8 w_c, plus 16 for the residual
+    words of the E- and H-side auxiliaries in FP32-res.  Returns (total, E side, B side): the
+    E side is E, the accumulators and the E auxiliaries, the B side B and the H auxiliaries."""
     cells = n ** 3
     wf = 8 if prec == 0 else 4
     r = 4 if prec == 2 else 0  # one residual word per cell and field group
-    acc = 32 if prec == 2 else 48
     layer = 2 * cpml_d * 3 * n * n
-    e_side = cells * (2 * (3 * wf + r) + 2 * acc * n_poles) + layer * (4 * wf + 2 * r)
+    e_side = cells * (2 * (3 * wf + r) + 2 * 6 * 8 * n_poles) + layer * (4 * wf + 2 * r)
     b_side = cells * 2 * (3 * wf + r) + layer * (4 * wf + 2 * r)
     return e_side + b_side, e_side, b_side
 
@@ -465,8 +461,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp32-res (f32 E/B/CPML/PLRC-psi words + 10-bit residual slots; f64 arithmetic and Kerr "
-                  "solve)" if prec == 2
+        "dtype": ("fp32-res (f32 E/B/CPML words + 10-bit residual slots, f64 PLRC psi and Kerr solve, "
+                  "f64 arithmetic)" if prec == 2
                   else "fp32-bulk (f32 E/B/CPML, f64 psi and Kerr, f64 arithmetic)"),
         "data": "synthetic (zero initial fields, analytic short-pulse source; timed after a pre-roll of "
                 f"{args.preroll} steps with the pulse in the grid)",

@@ -68,10 +68,8 @@ extern "C" {
 #define MPX_PREC_FP32_RES  2  /* as FP32_BULK, each E, B and CPML psi word backed by a 10-bit residual
                                  (three of a cell's E or B components, or a face's two auxiliaries,
                                  share one 32-bit word): stored values within 2^-10 float32 ulp,
-                                 about 1000x closer to float64 storage; the PLRC accumulators as
-                                 float32 words with 10-bit residual slots too (within 2^-33, two
-                                 residual words per pole and cell); all arithmetic float64; fused
-                                 and pass paths */
+                                 about 1000x closer to float64 storage; PLRC accumulators and
+                                 all arithmetic float64; fused and pass paths */
 
 /* kernel structure */
 #define MPX_PATH_FUSED     0  /* one fused E-side and one fused B-side kernel per step */

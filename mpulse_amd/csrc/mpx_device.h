@@ -58,74 +58,6 @@ __device__ __forceinline__ void store_once(T *p, T v)
     __builtin_nontemporal_store(v, (MPX_GLOBAL T *)p);
 }
 
-// Accumulators of a pole in the float32 + residual form (psr_byte, res_encode): the three
-// components of cell pq decoded from, or encoded into, the block at `base` (the pole's
-// component-0 pointer)
-typedef float float2v __attribute__((ext_vector_type(2)));
-typedef unsigned uint2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void psi_load_res(const double2 *base, size_t pq, double2 v[3])
-{
-    const char *p = (const char *)base + psr_byte(pq);
-    const float2v h0 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)p);
-    const float2v h1 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 512));
-    const float2v h2 = __builtin_nontemporal_load((const MPX_GLOBAL float2v *)(p + 1024));
-    const uint2v w = __builtin_nontemporal_load((const MPX_GLOBAL uint2v *)(p + 1536));
-    v[0] = make_double2(res_decode(h0.x, w.x, 0), res_decode(h0.y, w.x, 1));
-    v[1] = make_double2(res_decode(h1.x, w.x, 2), res_decode(h1.y, w.y, 0));
-    v[2] = make_double2(res_decode(h2.x, w.y, 1), res_decode(h2.y, w.y, 2));
-}
-__device__ __forceinline__ void psi_store_res(double2 *base, size_t pq, double2 x, double2 y, double2 z)
-{
-    unsigned w0 = 0, w1 = 0;
-    float2v h0, h1, h2;
-    uint2v w;
-    h0.x = res_encode(x.x, w0, 0);
-    h0.y = res_encode(x.y, w0, 1);
-    h1.x = res_encode(y.x, w0, 2);
-    h1.y = res_encode(y.y, w1, 0);
-    h2.x = res_encode(z.x, w1, 1);
-    h2.y = res_encode(z.y, w1, 2);
-    w.x = w0;
-    w.y = w1;
-    char *p = (char *)base + psr_byte(pq);
-    __builtin_nontemporal_store(h0, (MPX_GLOBAL float2v *)p);
-    __builtin_nontemporal_store(h1, (MPX_GLOBAL float2v *)(p + 512));
-    __builtin_nontemporal_store(h2, (MPX_GLOBAL float2v *)(p + 1024));
-    __builtin_nontemporal_store(w, (MPX_GLOBAL uint2v *)(p + 1536));
-}
-// the pole bases of float32 + residual accumulators, for the E update's loads and stores
-template <int NP>
-struct PsiRes {
-    double2 *b[NP > 0 ? NP : 1];
-};
-// the E update reads a pole's three components through psi_get and writes them back: float64
-// accumulators through any table indexable as psi[pole][component][element], float32 +
-// residual ones through PsiRes
-template <typename PIN>
-__device__ __forceinline__ void psi_get(const PIN &in, int n, size_t pq, double2 &x, double2 &y, double2 &z)
-{
-    x = in[n][0][pq];
-    y = in[n][1][pq];
-    z = in[n][2][pq];
-}
-template <int NP>
-__device__ __forceinline__ void psi_get(const PsiRes<NP> &in, int n, size_t pq, double2 &x, double2 &y, double2 &z)
-{
-    double2 v[3];
-    psi_load_res(in.b[n], pq, v);
-    x = v[0];
-    y = v[1];
-    z = v[2];
-}
-template <typename P>
-struct is_psi_res {
-    static constexpr bool value = false;
-};
-template <int NP>
-struct is_psi_res<PsiRes<NP>> {
-    static constexpr bool value = true;
-};
-
 // The value a neighbouring lane of the wavefront holds, by DPP: lane_prev gives lane l the
 // value of lane l - 1 (wave_shr:1), lane_next that of lane l + 1 (wave_shl:1).  Lanes of a
 // fused-kernel wave hold consecutive z cells of one row, so the z neighbour of a cell is in
@@ -226,8 +158,7 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
     const size_t pq = psi_at(c);
 #pragma unroll
     for (int n = 0; n < NP; ++n) {
-        double2 px, py, pz;
-        psi_get(psi_in, n, pq, px, py, pz);
+        double2 px = psi_in[n][0][pq], py = psi_in[n][1][pq], pz = psi_in[n][2][pq];
         const double2 D = pc.dd[n], X = pc.dxi0[n], C = pc.crec[n];
         double pxr = D.x * ex + px.x, pxi = D.y * ex + px.y;
         double pyr = D.x * ey + py.x, pyi = D.y * ey + py.y;
@@ -236,16 +167,9 @@ __device__ __forceinline__ void plrc_e_update_pp(double &ex, double &ey, double 
         Psiy += pyr;
         Psiz += pzr;
         if (store) {
-            if constexpr (is_psi_res<POUT>::value) {
-                psi_store_res(psi.b[n], pq,
-                              make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)),
-                              make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)),
-                              make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
-            } else {
-                store_once(&psi[n][0][pq], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
-                store_once(&psi[n][1][pq], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
-                store_once(&psi[n][2][pq], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
-            }
+            store_once(&psi[n][0][pq], make_double2(X.x * ex + (C.x * pxr - C.y * pxi), X.y * ex + (C.x * pxi + C.y * pxr)));
+            store_once(&psi[n][1][pq], make_double2(X.x * ey + (C.x * pyr - C.y * pyi), X.y * ey + (C.x * pyi + C.y * pyr)));
+            store_once(&psi[n][2][pq], make_double2(X.x * ez + (C.x * pzr - C.y * pzi), X.y * ez + (C.x * pzi + C.y * pzr)));
         }
     }
     if (NONLIN) {

@@ -540,16 +540,10 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
         double ex = (double)rd_once<T, RES>(a.E[0], we, 0, c), ey = (double)rd_once<T, RES>(a.E[1], we, 1, c);
         double ez = (double)rd_once<T, RES>(a.E[2], we, 2, c);
         PsiRegs<NP> pr;
-        PsiRes<NP> prs;  // float32 + residual accumulators: the pole bases
 #pragma unroll
-        for (int n = 0; n < NP; ++n) {
-            prs.b[n] = a.psi[n][0];
-            if (RES)
-                psi_load_res(a.psi[n][0], psi_at(c), pr.v[n]);
-            else
+        for (int n = 0; n < NP; ++n)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
-        }
+            for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
         const unsigned wj = lo_ld<RES>(a.Blo, c + ojm);
         const double dBz_y = bz - (double)rd<T, RES>(Bz, wj, 2, c + ojm);
         const double dBx_y = bx - (double)rd<T, RES>(Bx, wj, 0, c + ojm);
@@ -639,14 +633,9 @@ __device__ __forceinline__ void fused_E_tile(const StepArgs<T> &a, int tbx, int 
             idenom = 1.0 / denom;
         }
         unsigned fl = 0;
-        if (RES)
-            plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i),
-                                               kdy, kdz, jx, jy, jz, pr, prs, (size_t)c, true, a.pc, a.dt, idenom,
-                                               numer, a.chi, a.ncap, &fl);
-        else
-            plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i),
-                                               kdy, kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom,
-                                               numer, a.chi, a.ncap, &fl);
+        plrc_e_update_pp<NP, NONLIN, true>(ex, ey, ez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i), kdy,
+                                           kdz, jx, jy, jz, pr, a.psi, (size_t)c, true, a.pc, a.dt, idenom, numer,
+                                           a.chi, a.ncap, &fl);
         report(a.status, fl | finite_flag(ex, ey, ez));
         unsigned wn = 0;
         const T hx = put<T, RES>(ex, wn, 0), hy = put<T, RES>(ey, wn, 1), hz = put<T, RES>(ez, wn, 2);

@@ -299,16 +299,10 @@ __global__ __launch_bounds__(256) void k_pass(const PassArgs<T> *__restrict__ pa
     const C3 bxm = ld3<T, RES>(Bo, a.Bolo, c + ox_m), bym = ld3<T, RES>(Bo, a.Bolo, c + oy_m),
              bzm = ld3<T, RES>(Bo, a.Bolo, c + oz_m);
     PsiRegs<NP> pr;
-    PsiRes<NP> prs;  // float32 + residual accumulators: the pole bases
 #pragma unroll
-    for (int n = 0; n < NP; ++n) {
-        prs.b[n] = a.psi[n][0];
-        if (RES)
-            psi_load_res(a.psi[n][0], psi_at(c), pr.v[n]);
-        else
+    for (int n = 0; n < NP; ++n)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
-    }
+        for (int q = 0; q < 3; ++q) pr.v[n][q] = load_once(&a.psi[n][q][psi_at(c)]);
 
     // z + 1 neighbours from the next lane
     {
@@ -369,14 +363,9 @@ __global__ __launch_bounds__(256) void k_pass(const PassArgs<T> *__restrict__ pa
     }
     double nex = ex, ney = ey, nez = ez;
     unsigned fl = 0;
-    if (RES)
-        plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i),
-                                           gld(a.ikdE[1] + j), gld(a.ikdE[2] + k), jx, jy, jz, pr, prs, (size_t)c,
-                                           true, a.pc, a.dt, idenom, numer, a.chi, a.ncap, &fl);
-    else
-        plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i),
-                                           gld(a.ikdE[1] + j), gld(a.ikdE[2] + k), jx, jy, jz, pr, a.psi, (size_t)c,
-                                           true, a.pc, a.dt, idenom, numer, a.chi, a.ncap, &fl);
+    plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, gld(a.ikdE[0] + i),
+                                       gld(a.ikdE[1] + j), gld(a.ikdE[2] + k), jx, jy, jz, pr, a.psi, (size_t)c, true,
+                                       a.pc, a.dt, idenom, numer, a.chi, a.ncap, &fl);
     report(a.status, fl | finite_flag(nex, ney, nez));
     unsigned ew = 0;
     const T hx = put<T, RES>(nex, ew, 0), hy = put<T, RES>(ney, ew, 1), hz = put<T, RES>(nez, ew, 2);
@@ -653,9 +642,8 @@ __device__ __forceinline__ void march_tile(const PassArgs<T> &a, int tz, int ty,
         for (int q = 0; q < a.nsheets; ++q) sh |= a.sheet[q].kp == k;
         sh = __ballot(sh) != 0;
         PsiBase<NP> psb;
-        PsiRes<NP> prs;  // float32 + residual accumulators: the pole bases
 #pragma unroll
-        for (int n = 0; n < NP; ++n) psb.b[n] = prs.b[n] = a.psi[n][0];
+        for (int n = 0; n < NP; ++n) psb.b[n] = a.psi[n][0];
         const double khy = kload(a.ikdH[1] + j), khz = gld(a.ikdH[2] + k);
         const double key = kload(a.ikdE[1] + j), kez = gld(a.ikdE[2] + k);
         // E^(n+1) of the row at p + 1 and B^n at p (real: at the lane's cell, else at the
@@ -676,11 +664,8 @@ __device__ __forceinline__ void march_tile(const PassArgs<T> &a, int tz, int ty,
             const size_t pq = real ? psi_at((size_t)p * sx + o) : 0;
 #pragma unroll
             for (int n = 0; n < NP; ++n)
-                if (RES)
-                    psi_load_res(a.psi[n][0], pq, pr.v[n]);
-                else
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) pr.v[n][r] = load_once(a.psi[n][0] + pq + 64 * r);
+                for (int r = 0; r < 3; ++r) pr.v[n][r] = load_once(a.psi[n][0] + pq + 64 * r);
         };
         // B^(n+1) of the row at plane p from E^(n+1) at p (e, decoded), at p + 1 (n) and the
         // E rows above and to the right (LDS buffer be); CPML H auxiliaries stored with `store`
@@ -787,14 +772,9 @@ __device__ __forceinline__ void march_tile(const PassArgs<T> &a, int tz, int ty,
             }
             double nex = e.x, ney = e.y, nez = e.z;
             unsigned lf = 0;
-            if (RES)
-                plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kxe, key,
-                                                   kez, jx, jy, jz, pr, prs, bp + os, true, a.pc, a.dt, idenom, numer,
-                                                   a.chi, a.ncap, &lf);
-            else
-                plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kxe, key,
-                                                   kez, jx, jy, jz, pr, psb, bp + os, true, a.pc, a.dt, idenom, numer,
-                                                   a.chi, a.ncap, &lf);
+            plrc_e_update_pp<NP, NONLIN, true>(nex, ney, nez, dBz_y, dBy_z, dBx_z, dBz_x, dBy_x, dBx_y, kxe, key,
+                                               kez, jx, jy, jz, pr, psb, bp + os, true, a.pc, a.dt, idenom, numer,
+                                               a.chi, a.ncap, &lf);
             lf |= finite_flag(nex, ney, nez);
             fl |= ok ? lf : 0u;
             unsigned ew = 0;

@@ -1905,8 +1905,7 @@ static int fetch_words(mpx_solver *s, const unsigned *d, size_t n, std::vector<u
 }
 
 // resolve a full-storage grid name; *elem = element size; nullptr + MPX_OK means "all zero"
-static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, bool *isPsi, int *psiPart,
-                     int *psiComp = nullptr)
+static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, bool *isPsi, int *psiPart)
 {
     *ptr = nullptr;
     *elem = s->esz;
@@ -1930,7 +1929,6 @@ static int find_grid(mpx_solver *s, const char *name, void **ptr, size_t *elem, 
                 if (!s->plrc) return MPX_ERR_NAME;
                 *isPsi = true;
                 *psiPart = r ? 0 : 1;
-                if (psiComp) *psiComp = c;
                 *elem = 8;
                 for (int a = 0; a < s->np; ++a)
                     if (s->pole_ref[a] == n) *ptr = s->psi[a][c];
@@ -2026,8 +2024,8 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
     void *ptr;
     size_t elem;
     bool isPsi;
-    int part = 0, comp = 0;
-    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part, &comp);
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
     if (rc) return fail(rc, std::string("unknown grid ") + name);
     if (!ptr) {
         if (!std::strcmp(name, "Eps"))
@@ -2037,32 +2035,20 @@ int mpx_get_field(mpx_solver *s, const char *name, double *host, size_t count)
         return MPX_OK;
     }
     if ((rc = materialise_ghosts(s))) return rc;
-    // an accumulator component spans its pole's block from its own offset (psi_at); float32 +
-    // residual accumulators are read from the pole's base (psr_byte)
-    const bool rpsi = isPsi && s->res;
-    if (rpsi) ptr = (double2 *)ptr - 64 * comp;
-    std::vector<unsigned char> buf(rpsi ? mpx::psi_block_elems(s->cells) * 16
-                                        : isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
+    // an accumulator component spans its pole's block from its own offset (psi_at)
+    std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     int slot = 0;
-    const unsigned *dlo = isPsi ? nullptr : res_of(s, ptr, &slot);
+    const unsigned *dlo = res_of(s, ptr, &slot);
     std::vector<unsigned> lo;
     if (dlo && (rc = fetch_words(s, dlo, s->cells, lo))) return rc;
-    const int s6 = 2 * comp + part;  // residual slot of the value among the cell's six
     for (size_t i = 0; i < nx2; ++i)
         for (size_t j = 0; j < ny2; ++j)
             for (size_t k = 0; k < nz2; ++k) {
                 size_t src = g.at((int)i, (int)j, (int)k);
                 size_t dst = (i * ny2 + j) * nz2 + k;
-                if (rpsi) {
-                    const unsigned char *q = buf.data() + mpx::psr_byte(mpx::psi_at(src));
-                    float hi;
-                    unsigned w;
-                    std::memcpy(&hi, q + 512 * comp + 4 * part, 4);
-                    std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
-                    host[dst] = mpx::res_decode(hi, w, s6 % 3);
-                } else if (isPsi)
+                if (isPsi)
                     host[dst] = ((const double *)buf.data())[2 * mpx::psi_at(src) + part];
                 else if (elem == 8)
                     host[dst] = ((const double *)buf.data())[src];
@@ -2186,37 +2172,24 @@ int mpx_set_field(mpx_solver *s, const char *name, const double *host, size_t co
     void *ptr;
     size_t elem;
     bool isPsi;
-    int part = 0, comp = 0;
-    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part, &comp);
+    int part = 0;
+    int rc = find_grid(s, name, &ptr, &elem, &isPsi, &part);
     if (rc) return fail(rc, std::string("unknown grid ") + name);
     if (!ptr) return fail(MPX_ERR_UNSUPPORTED, std::string("grid ") + name + " is not stored (inactive pole or Sigma)");
-    // an accumulator component spans its pole's block from its own offset (psi_at); float32 +
-    // residual accumulators are written from the pole's base (psr_byte)
-    const bool rpsi = isPsi && s->res;
-    if (rpsi) ptr = (double2 *)ptr - 64 * comp;
-    std::vector<unsigned char> buf(rpsi ? mpx::psi_block_elems(s->cells) * 16
-                                        : isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
+    // an accumulator component spans its pole's block from its own offset (psi_at)
+    std::vector<unsigned char> buf(isPsi ? (mpx::psi_at(s->cells - 1) + 1) * 16 : s->cells * elem);
     MPX_HIP(hipMemcpyAsync(buf.data(), ptr, buf.size(), hipMemcpyDeviceToHost, s->stream));
     MPX_HIP(hipStreamSynchronize(s->stream));
     int slot = 0;
-    unsigned *dlo = isPsi ? nullptr : const_cast<unsigned *>(res_of(s, ptr, &slot));
+    unsigned *dlo = const_cast<unsigned *>(res_of(s, ptr, &slot));
     std::vector<unsigned> lo;
     if (dlo && (rc = fetch_words(s, dlo, s->cells, lo))) return rc;
-    const int s6 = 2 * comp + part;  // residual slot of the value among the cell's six
     for (size_t i = 0; i < nx2; ++i)
         for (size_t j = 0; j < ny2; ++j)
             for (size_t k = 0; k < nz2; ++k) {
                 size_t dst = g.at((int)i, (int)j, (int)k);
                 size_t src = (i * ny2 + j) * nz2 + k;
-                if (rpsi) {
-                    unsigned char *q = buf.data() + mpx::psr_byte(mpx::psi_at(dst));
-                    unsigned w, nw = 0;
-                    std::memcpy(&w, q + 1536 + 4 * (s6 / 3), 4);
-                    const float hi = mpx::res_encode(host[src], nw, s6 % 3);
-                    w = (w & ~(0x3ffu << (10 * (s6 % 3)))) | nw;
-                    std::memcpy(q + 512 * comp + 4 * part, &hi, 4);
-                    std::memcpy(q + 1536 + 4 * (s6 / 3), &w, 4);
-                } else if (isPsi)
+                if (isPsi)
                     ((double *)buf.data())[2 * mpx::psi_at(dst) + part] = host[src];
                 else if (elem == 8)
                     ((double *)buf.data())[dst] = host[src];

@@ -38,6 +38,7 @@ __host__ __device__ __forceinline__ size_t psi_at(size_t c)
 }
 // double2 elements of one pole's block over `cells` storage cells
 inline size_t psi_block_elems(size_t cells) { return (((cells - 1 + PSI_SHIFT) >> 6) + 1) * 192; }
+
 // Float32 + residual storage (MPX_PREC_FP32_RES): a value x is held as the float32 word
 // hi = fl32(x) plus, in a 10-bit slot of a residual word, the two's-complement
 // r = round((bits(x) - bits(hi)) / 2^19): the difference of the float64 bit patterns of x and
@@ -67,13 +68,6 @@ __host__ __device__ __forceinline__ double res_decode(float hi, unsigned w, int 
     return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, (double)hi) +
                                           ((unsigned long long)(long long)r << 19));
 }
-// The accumulators of a pole in this form keep the float64 layout's chunks (3072 bytes per
-// 64 cells), of which the first 2048 bytes are used: cell l of a chunk holds the float32 pair
-// (re, im) of component q at byte 512 q + 8 l and its two residual words at byte 1536 + 8 l
-// (word 0: the slots of x.re, x.im, y.re; word 1: y.im, z.re, z.im).  psr_byte(psi_at(c)) is
-// the byte offset of cell c's component-0 pair from the pole's base; the other words sit 512,
-// 1024 and 1536 bytes on.
-__host__ __device__ __forceinline__ size_t psr_byte(size_t pq) { return 16 * pq - 8 * (pq & 63); }
 
 // One CPML border current (a face of CPMLBorder{E,H}Current): its layer box, its two live
 // auxiliaries, and the b/c coefficient lines indexed by the normal coordinate.
